@@ -1,0 +1,153 @@
+/* rgf.h -- C ABI of the MI355X recursive Green's function (RGF) engine.
+ *
+ * The engine solves, per energy E, the open block-tridiagonal device of the
+ * reference's green_RGF_RS (src/green_rgf.f90:3950-4164): Sancho-Rubio lead
+ * decimation (src/green_rgf.f90:4451-4537), forward left-connected sweep,
+ * backward sweep for the diagonal and first off-diagonal blocks of G^R, G^<,
+ * G^>, the bond-current blocks and the Meir-Wingreen spectra tr/tre.
+ *
+ * Conventions shared by every entry point:
+ *   - complex arrays are complex128 as interleaved (re, im) doubles in Fortran
+ *     order: a (nm, nm, nx) array holds the nx blocks of nm x nm column-major
+ *     matrices back to back (the layout of a Fortran complex(8) array);
+ *   - Hii(:,:,l) is the onsite block of slab l, H1i(:,:,l) the coupling
+ *     H_{l,l-1} (H1i(:,:,1) also couples the left lead, H1i(:,:,nx) the right);
+ *   - optional arrays are passed as NULL;
+ *   - functions return 0 on success and a nonzero RGF_E* code on error, with a
+ *     message from rgf_last_error().  Numerical failures follow the reference:
+ *     a singular slab matrix is replaced by zero (src/green_rgf.f90:4429-4437)
+ *     and an unconverged decimation falls back to the bare surface
+ *     (src/green_rgf.f90:4513-4518); both are counted in the per-energy info.
+ * All arithmetic is complex FP64 on the GPU (HIP, gfx950).
+ */
+#ifndef LITEQT_RGF_H
+#define LITEQT_RGF_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RGF_MODE_COMPAT 0   /* bit-for-bit semantics of green_RGF_RS, quirks included */
+#define RGF_MODE_PHYSICAL 1 /* consistent coupling indexing, true G^> off-diagonal, T_L/T_R not swapped */
+
+#define RGF_OK 0
+#define RGF_EINVAL 1   /* invalid argument (sizes, pointers, mode) */
+#define RGF_EHIP 2     /* HIP runtime error */
+#define RGF_ENOMEM 3   /* device memory exhausted */
+#define RGF_EUNSUP 4   /* unsupported size for this entry point */
+
+/* Numerical parameters.  The reference hard-codes them; rgf_params_reference
+ * fills its values:
+ *   eta_dev  = 0       (z = dcmplx(E, 0.0d-6), src/green_rgf.f90:3974)
+ *   eta_lead = 1e-3    (src/green_rgf.f90:4474)
+ *   dec_tol  = 1e-100  (src/green_rgf.f90:4457)
+ *   dec_nmax = 100     (src/green_rgf.f90:4473)
+ *   boltz    = 8.61734e-5 eV/K (src/green_rgf.f90:3968)
+ *   mode     = RGF_MODE_COMPAT */
+typedef struct rgf_params {
+  double eta_dev;
+  double eta_lead;
+  double dec_tol;
+  int dec_nmax;
+  double boltz;
+  int mode;
+} rgf_params;
+
+void rgf_params_reference(rgf_params *p);
+
+/* Message for the last error on the calling thread (empty when none). */
+const char *rgf_last_error(void);
+
+/* Library version string. */
+const char *rgf_version(void);
+
+/* Drop-in for green_RGF_RS (src/green_rgf.f90:3950-3958), same argument order
+ * and meaning, host arrays, reference parameters.  Scalars are passed by
+ * value.  sigG is accepted and unused, as in the reference.  GRi1/GLi1/GGi1
+ * (the absent-able optional arguments) are written for l = 1..nx-1, block nx
+ * untouched.  Thread-safe: concurrent callers (the reference's OpenMP energy
+ * loops, src/green_rgf.f90:1305-1316) are serialised on the device. */
+int rgf_rs(const double temp[2], int nm, int nx, double E, const double mu[2],
+           const double *Hii, const double *H1i, const double *sigL,
+           const double *sigG, const double *sigR, double *ndens, double *pdens,
+           double *ldos, double *tr, double *tre, double *cur, double *GRi1,
+           double *GLi1, double *GGi1);
+
+/* rgf_rs with explicit parameters. */
+int rgf_rs_ex(const rgf_params *p, const double temp[2], int nm, int nx, double E,
+              const double mu[2], const double *Hii, const double *H1i,
+              const double *sigL, const double *sigG, const double *sigR,
+              double *ndens, double *pdens, double *ldos, double *tr, double *tre,
+              double *cur, double *GRi1, double *GLi1, double *GGi1);
+
+/* Replacement of the drivers' energy loops `do ie ... call green_RGF_RS(...)`
+ * (src/green_rgf.f90:3174-3180, 2914-2927, 1301-1318): all nE energies in one
+ * batched GPU solve.  Host arrays with a trailing energy index, i.e. the
+ * caller's (nm, nm, nx, nen) slices.  sig* have energy stride nm*nm*nx when
+ * sig_per_energy != 0, else one (nm, nm, nx) set is shared.  tr/tre have nE
+ * entries.  info (optional, 4*nE ints) receives per energy: decimation
+ * iterations left, right, nmax-hit bits (1 left, 2 right), singular-matrix
+ * count. */
+int rgf_rs_energies(const rgf_params *p, const double temp[2], int nm, int nx,
+                    int nE, const double *E, const double mu[2],
+                    const double *Hii, const double *H1i, const double *sigL,
+                    const double *sigG, const double *sigR, int sig_per_energy,
+                    double *ndens, double *pdens, double *ldos, double *tr,
+                    double *tre, double *cur, double *GRi1, double *GLi1,
+                    double *GGi1, int *info);
+
+/* Device-resident batched solve (the engine's native entry point).  Every
+ * pointer is device memory; the call is asynchronous on `stream` (a
+ * hipStream_t, NULL for the default stream).  nm must be a multiple of 32 and
+ * at most 256 (the host-array entry points pad other sizes).  Per-energy
+ * arrays have a trailing energy index with stride nm*nm*nx (complex for the G
+ * blocks, real for cur).  dens_n / dens_p (nx per energy) receive
+ * Im Tr G^<_ll and -Im Tr G^>_ll, the integrands of the electron and hole
+ * densities of src/green_rgf.f90:3919-3945.  max_slots > 0 caps the number of
+ * energies in flight (each holds 2*nx stored left-connected blocks in HBM);
+ * 0 lets the engine size it from free memory and the CU count. */
+typedef struct rgf_batch {
+  int nm, nx, nE;
+  int max_slots;
+  const double *E;
+  double temp[2];
+  double mu[2];
+  const double *Hii, *H1i;
+  const double *sigL, *sigG, *sigR;
+  long long sig_stride; /* complex elements between energies; 0 = shared */
+  double *ldos, *ndens, *pdens, *GRi1, *GLi1, *GGi1;
+  double *cur;
+  double *tr, *tre;
+  double *dens_n, *dens_p;
+  int *info;
+} rgf_batch;
+
+int rgf_solve_batched(const rgf_params *p, const rgf_batch *b, void *stream);
+
+/* Bytes of device workspace rgf_solve_batched uses for `slots` energies in
+ * flight at block size nm and nx slabs. */
+long long rgf_workspace_bytes(int nm, int nx, int slots);
+
+/* Batched block operations of the engine, one workgroup per matrix, device
+ * arrays of nbatch contiguous n x n column-major complex matrices, asynchronous
+ * on `stream`; n a multiple of 32 in [32, 256].  They are the building blocks
+ * every slab step is made of (the reference's zgemm and invert call sites,
+ * src/green_rgf.f90:4002-4155 and 4421-4441), exposed for validation and
+ * for callers that assemble their own sweeps.
+ *   rgf_zgemm_batched: C = op(A) op(B), op 0 = none, 1 = conjugate transpose;
+ *                      C row-major when c_rowmajor.
+ *   rgf_zinv_batched:  Ainv = A^-1 by Gauss-Jordan with partial row pivoting;
+ *                      info[b] = 0 or k+1 for a zero pivot in column k (then
+ *                      Ainv = 0, the reference's convention). */
+int rgf_zgemm_batched(int n, int nbatch, int opA, const double *A, int opB, const double *B,
+                      double *C, int c_rowmajor, void *stream);
+int rgf_zinv_batched(int n, int nbatch, const double *A, double *Ainv, int *info, void *stream);
+
+/* Release cached device buffers of this library on the current device. */
+void rgf_release(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LITEQT_RGF_H */

@@ -1,0 +1,466 @@
+// rgf_engine.hip -- host side of the MI355X RGF engine: the C ABI of
+// include/rgf.h, kernel-variant selection, workspace admission and the
+// host-array entry points that stage data through HBM.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/rgf.h"
+#include "rgf_sweep.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+std::mutex g_mu;  // serialises the host-array entry points (shared staging buffers)
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                      \
+  do {                                                                                    \
+    hipError_t _e = (expr);                                                               \
+    if (_e != hipSuccess)                                                                 \
+      return fail(_e == hipErrorOutOfMemory ? RGF_ENOMEM : RGF_EHIP,                      \
+                  std::string(#expr) + ": " + hipGetErrorString(_e));                     \
+  } while (0)
+
+// Device buffer that only grows; one set per device for the engine workspace
+// and one for the host-array staging area.
+struct Buf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t want) {
+    if (want <= bytes) return 0;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return fail(e == hipErrorOutOfMemory ? RGF_ENOMEM : RGF_EHIP,
+                  std::string("hipMalloc workspace: ") + hipGetErrorString(e));
+    }
+    bytes = want;
+    return 0;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
+constexpr int kMaxDev = 16;
+Buf g_ws[kMaxDev];
+std::mutex g_ws_mu;
+
+// Copy nblocks n x n blocks into np x np blocks (np >= n): zero padding and
+// `fill` on the padded diagonal, which keeps the padded orbitals decoupled and
+// every padded slab matrix invertible.
+__global__ void pad_blocks(const double2* __restrict__ src, int n, double2* __restrict__ dst, int np,
+                           long long nblocks, double2 fill) {
+  const size_t tot = (size_t)nblocks * np * np;
+  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < tot;
+       idx += (size_t)gridDim.x * blockDim.x) {
+    const size_t b = idx / ((size_t)np * np);
+    const int r = (int)(idx % ((size_t)np * np));
+    const int i = r % np, j = r / np;
+    double2 v = make_double2(0, 0);
+    if (i < n && j < n) v = src[b * n * n + i + (size_t)j * n];
+    else if (i == j) v = fill;
+    dst[idx] = v;
+  }
+}
+
+template <typename V>
+__global__ void unpad_blocks(const V* __restrict__ src, int np, V* __restrict__ dst, int n, long long nblocks) {
+  const size_t tot = (size_t)nblocks * n * n;
+  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < tot;
+       idx += (size_t)gridDim.x * blockDim.x) {
+    const size_t b = idx / ((size_t)n * n);
+    const int r = (int)(idx % ((size_t)n * n));
+    const int i = r % n, j = r / n;
+    dst[idx] = src[b * np * np + i + (size_t)j * np];
+  }
+}
+
+Buf g_stage[kMaxDev];
+
+typedef void (*SweepLaunch)(const rgf::SweepArgs&, int grid, hipStream_t);
+
+template <int T, int NMAX>
+void launch_sweep(const rgf::SweepArgs& a, int grid, hipStream_t s) {
+  hipLaunchKernelGGL((rgf::rgf_sweep<T, NMAX>), dim3(grid), dim3(rgf::NT), 0, s, a);
+}
+
+template <int T, int NMAX>
+const void* sweep_fn() {
+  return reinterpret_cast<const void*>(&rgf::rgf_sweep<T, NMAX>);
+}
+
+struct Variant {
+  SweepLaunch launch;
+  const void* fn;
+  int T, nmax;
+};
+
+// Tile variant per block size: wave tile (16T)^2, macro tile (32T)^2; the
+// inverse's LDS panel is sized for NMAX rows.
+bool pick_variant(int n, Variant* v) {
+  if (n % 32 != 0 || n < 32 || n > 256) return false;
+  switch (n) {
+    case 32: *v = {launch_sweep<1, 32>, sweep_fn<1, 32>(), 1, 32}; return true;
+    case 64: *v = {launch_sweep<2, 64>, sweep_fn<2, 64>(), 2, 64}; return true;
+    case 96: *v = {launch_sweep<3, 96>, sweep_fn<3, 96>(), 3, 96}; return true;
+    case 128: *v = {launch_sweep<4, 128>, sweep_fn<4, 128>(), 4, 128}; return true;
+    case 192: *v = {launch_sweep<3, 192>, sweep_fn<3, 192>(), 3, 192}; return true;
+    case 256: *v = {launch_sweep<4, 256>, sweep_fn<4, 256>(), 4, 256}; return true;
+    default: *v = {launch_sweep<1, 256>, sweep_fn<1, 256>(), 1, 256}; return true;
+  }
+}
+
+long long slot_elems(int n, int nx) { return (long long)(2 * nx + rgf::kTmpBlocks) * n * n; }
+
+int check_params(const rgf_params* p) {
+  if (!p) return fail(RGF_EINVAL, "params is NULL");
+  if (p->mode != RGF_MODE_COMPAT && p->mode != RGF_MODE_PHYSICAL) return fail(RGF_EINVAL, "unknown mode");
+  if (p->dec_nmax < 1) return fail(RGF_EINVAL, "dec_nmax must be >= 1");
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+void rgf_params_reference(rgf_params* p) {
+  p->eta_dev = 0.0;
+  p->eta_lead = 1.0e-3;
+  p->dec_tol = 1.0e-100;
+  p->dec_nmax = 100;
+  p->boltz = 8.61734e-05;
+  p->mode = RGF_MODE_COMPAT;
+}
+
+const char* rgf_last_error(void) { return g_err.c_str(); }
+
+const char* rgf_version(void) { return "liteqt_amd rgf 0.1 (gfx950, complex FP64 MFMA)"; }
+
+long long rgf_workspace_bytes(int nm, int nx, int slots) {
+  return slot_elems(nm, nx) * 16LL * (long long)slots;
+}
+
+int rgf_solve_batched(const rgf_params* p, const rgf_batch* b, void* stream) {
+  g_err.clear();
+  if (int rc = check_params(p)) return rc;
+  if (!b) return fail(RGF_EINVAL, "batch is NULL");
+  if (b->nx < 2) return fail(RGF_EINVAL, "nx must be >= 2 (the reference reads Gl(:,:,nx-1))");
+  if (b->nE < 0) return fail(RGF_EINVAL, "nE < 0");
+  if (b->nE == 0) return 0;
+  if (!b->E || !b->Hii || !b->H1i) return fail(RGF_EINVAL, "E, Hii and H1i are required");
+  Variant v;
+  if (!pick_variant(b->nm, &v))
+    return fail(RGF_EUNSUP, "rgf_solve_batched: nm must be a multiple of 32 in [32, 256]");
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDev) return fail(RGF_EINVAL, "device index out of range");
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, dev));
+  int per_cu = 0;
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, v.fn, rgf::NT, 0));
+  if (per_cu < 1) return fail(RGF_EHIP, "sweep kernel does not fit on a CU");
+  const long long slot_bytes = slot_elems(b->nm, b->nx) * 16LL;
+  long long slots = std::min<long long>(b->nE, (long long)prop.multiProcessorCount * per_cu);
+  if (b->max_slots > 0) slots = std::min<long long>(slots, b->max_slots);
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  Buf& ws = g_ws[dev];
+  if ((size_t)(slots * slot_bytes) > ws.bytes) {
+    // HBM admission: fit the stored left-connected blocks into free memory
+    size_t fr = 0, tot = 0;
+    HIPCHK(hipMemGetInfo(&fr, &tot));
+    const size_t avail = (size_t)((double)(fr + ws.bytes) * 0.9);
+    const long long fit = (long long)(avail / (size_t)slot_bytes);
+    if (fit < 1) return fail(RGF_ENOMEM, "not enough device memory for one energy slot");
+    slots = std::min(slots, fit);
+    ws.release();
+    if (int rc = ws.ensure((size_t)(slots * slot_bytes))) return rc;
+  }
+  rgf::SweepArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.n = b->nm;
+  a.nx = b->nx;
+  a.nE = b->nE;
+  a.mode = p->mode;
+  a.dec_nmax = p->dec_nmax;
+  a.eta_dev = p->eta_dev;
+  a.eta_lead = p->eta_lead;
+  a.dec_tol = p->dec_tol;
+  a.boltz = p->boltz;
+  a.temp[0] = b->temp[0];
+  a.temp[1] = b->temp[1];
+  a.mu[0] = b->mu[0];
+  a.mu[1] = b->mu[1];
+  a.E = b->E;
+  a.Hii = reinterpret_cast<const double2*>(b->Hii);
+  a.H1i = reinterpret_cast<const double2*>(b->H1i);
+  a.sigL = reinterpret_cast<const double2*>(b->sigL);
+  a.sigR = reinterpret_cast<const double2*>(b->sigR);
+  a.sig_stride = b->sig_stride;
+  a.ldos = reinterpret_cast<double2*>(b->ldos);
+  a.ndens = reinterpret_cast<double2*>(b->ndens);
+  a.pdens = reinterpret_cast<double2*>(b->pdens);
+  a.GRi1 = reinterpret_cast<double2*>(b->GRi1);
+  a.GLi1 = reinterpret_cast<double2*>(b->GLi1);
+  a.GGi1 = reinterpret_cast<double2*>(b->GGi1);
+  a.cur = b->cur;
+  a.tr = b->tr;
+  a.tre = b->tre;
+  a.dens_n = b->dens_n;
+  a.dens_p = b->dens_p;
+  a.info = b->info;
+  a.ws = reinterpret_cast<double2*>(ws.p);
+  a.ws_slot = slot_elems(b->nm, b->nx);
+  v.launch(a, (int)slots, reinterpret_cast<hipStream_t>(stream));
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// Host-array solve of nE energies: stage (and pad) inputs in HBM, run the
+// batched engine chunk by chunk, copy (and unpad) the outputs back.
+int rgf_rs_energies(const rgf_params* p, const double temp[2], int nm, int nx, int nE, const double* E,
+                    const double mu[2], const double* Hii, const double* H1i, const double* sigL,
+                    const double* sigG, const double* sigR, int sig_per_energy, double* ndens,
+                    double* pdens, double* ldos, double* tr, double* tre, double* cur, double* GRi1,
+                    double* GLi1, double* GGi1, int* info) {
+  (void)sigG;  // accepted and unused, as green_RGF_RS (src/green_rgf.f90:3953)
+  g_err.clear();
+  if (int rc = check_params(p)) return rc;
+  if (nm < 1 || nx < 2 || nE < 0) return fail(RGF_EINVAL, "need nm >= 1, nx >= 2, nE >= 0");
+  if (!temp || !mu || !Hii || !H1i || (nE > 0 && !E)) return fail(RGF_EINVAL, "missing required array");
+  if (nE == 0) return 0;
+  const int np = (nm + 31) / 32 * 32;
+  if (np > 256) return fail(RGF_EUNSUP, "nm > 256 is not supported");
+  std::lock_guard<std::mutex> lk(g_mu);
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDev) return fail(RGF_EINVAL, "device index out of range");
+  const size_t bn = (size_t)nm * nm, bp = (size_t)np * np;
+  const size_t cslab = bp * nx * 16, rslab = bp * nx * 8;       // padded, per energy
+  const size_t cslab_u = bn * nx * 16, rslab_u = bn * nx * 8;   // caller layout, per energy
+  const bool padded = (np != nm);
+  double* outs[6] = {ldos, ndens, pdens, GRi1, GLi1, GGi1};
+  int nout = 0;
+  for (double* o : outs) nout += (o != nullptr);
+  const int nsig = (sigL ? 1 : 0) + (sigR ? 1 : 0);
+  // bytes per energy in a chunk, then chunk size from free memory
+  const size_t per_e = (size_t)nout * cslab + (cur ? rslab : 0) + (sig_per_energy ? nsig * cslab : 0) +
+                       (padded ? cslab_u : 0) + 64;
+  size_t fr = 0, tot = 0;
+  HIPCHK(hipMemGetInfo(&fr, &tot));
+  const size_t fixed = 2 * cslab + (sig_per_energy ? 0 : nsig * cslab) + (padded ? cslab_u : 0);
+  const size_t ws_reserve = (size_t)rgf_workspace_bytes(np, nx, 256);
+  const size_t budget = (size_t)((double)(fr + g_stage[dev].bytes) * 0.45);
+  long long C = nE;
+  if (budget > fixed + ws_reserve / 2) C = std::min<long long>(C, (long long)((budget - fixed - ws_reserve / 2) / per_e));
+  else C = 1;
+  C = std::max<long long>(1, std::min<long long>(C, 8192));
+  // staging layout
+  size_t off = 0;
+  auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) / 256 * 256; return o; };
+  const size_t oH = take(cslab), oC = take(cslab);
+  const size_t sig_n = sig_per_energy ? (size_t)C : 1;
+  const size_t oSL = sigL ? take(cslab * sig_n) : 0, oSR = sigR ? take(cslab * sig_n) : 0;
+  size_t oOut[6];
+  for (int k = 0; k < 6; ++k) oOut[k] = outs[k] ? take(cslab * C) : 0;
+  const size_t oCur = cur ? take(rslab * C) : 0;
+  const size_t oE = take(8 * C), oTr = take(8 * C), oTre = take(8 * C), oInfo = take(16 * C);
+  const size_t oU = padded ? take(cslab_u * std::max<size_t>(C, 1)) : 0;
+  if (int rc = g_stage[dev].ensure(off)) return rc;
+  char* base = (char*)g_stage[dev].p;
+  auto D = [&](size_t o) { return (void*)(base + o); };
+  hipStream_t s = nullptr;
+  const double2 fill = make_double2(1.0e3, 0.0);
+  // H2D of (count) energies' (nm,nm,nx) complex slabs into a padded device slab
+  auto h2d_c = [&](const double* src, size_t dst_off, size_t count) -> int {
+    if (!padded) {
+      HIPCHK(hipMemcpyAsync(D(dst_off), src, cslab_u * count, hipMemcpyHostToDevice, s));
+    } else {
+      HIPCHK(hipMemcpyAsync(D(oU), src, cslab_u * count, hipMemcpyHostToDevice, s));
+      hipLaunchKernelGGL(pad_blocks, dim3(1024), dim3(256), 0, s, (const double2*)D(oU), nm, (double2*)D(dst_off),
+                         np, (long long)(nx * count), fill);
+      HIPCHK(hipGetLastError());
+    }
+    return 0;
+  };
+  if (int rc = h2d_c(Hii, oH, 1)) return rc;
+  if (int rc = h2d_c(H1i, oC, 1)) return rc;
+  if (padded) {
+    // couplings pad with zeros (no diagonal fill)
+    HIPCHK(hipMemcpyAsync(D(oU), H1i, cslab_u, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(pad_blocks, dim3(1024), dim3(256), 0, s, (const double2*)D(oU), nm, (double2*)D(oC), np,
+                       (long long)nx, make_double2(0, 0));
+    HIPCHK(hipGetLastError());
+  }
+  auto h2d_sig = [&](const double* src, size_t dst_off, size_t count) -> int {
+    if (!padded) {
+      HIPCHK(hipMemcpyAsync(D(dst_off), src, cslab_u * count, hipMemcpyHostToDevice, s));
+    } else {
+      HIPCHK(hipMemcpyAsync(D(oU), src, cslab_u * count, hipMemcpyHostToDevice, s));
+      hipLaunchKernelGGL(pad_blocks, dim3(1024), dim3(256), 0, s, (const double2*)D(oU), nm, (double2*)D(dst_off),
+                         np, (long long)(nx * count), make_double2(0, 0));
+      HIPCHK(hipGetLastError());
+    }
+    return 0;
+  };
+  if (!sig_per_energy) {
+    if (sigL) if (int rc = h2d_sig(sigL, oSL, 1)) return rc;
+    if (sigR) if (int rc = h2d_sig(sigR, oSR, 1)) return rc;
+  }
+  for (long long e0 = 0; e0 < nE; e0 += C) {
+    const long long ce = std::min<long long>(C, nE - e0);
+    if (sig_per_energy) {
+      if (sigL) if (int rc = h2d_sig(sigL + 2 * bn * nx * e0, oSL, ce)) return rc;
+      if (sigR) if (int rc = h2d_sig(sigR + 2 * bn * nx * e0, oSR, ce)) return rc;
+    }
+    HIPCHK(hipMemcpyAsync(D(oE), E + e0, 8 * ce, hipMemcpyHostToDevice, s));
+    rgf_batch b;
+    std::memset(&b, 0, sizeof(b));
+    b.nm = np; b.nx = nx; b.nE = (int)ce;
+    b.E = (const double*)D(oE);
+    b.temp[0] = temp[0]; b.temp[1] = temp[1]; b.mu[0] = mu[0]; b.mu[1] = mu[1];
+    b.Hii = (const double*)D(oH); b.H1i = (const double*)D(oC);
+    b.sigL = sigL ? (const double*)D(oSL) : nullptr;
+    b.sigR = sigR ? (const double*)D(oSR) : nullptr;
+    b.sig_stride = sig_per_energy ? (long long)(bp * nx) : 0;
+    b.ldos = ldos ? (double*)D(oOut[0]) : nullptr;
+    b.ndens = ndens ? (double*)D(oOut[1]) : nullptr;
+    b.pdens = pdens ? (double*)D(oOut[2]) : nullptr;
+    b.GRi1 = GRi1 ? (double*)D(oOut[3]) : nullptr;
+    b.GLi1 = GLi1 ? (double*)D(oOut[4]) : nullptr;
+    b.GGi1 = GGi1 ? (double*)D(oOut[5]) : nullptr;
+    b.cur = cur ? (double*)D(oCur) : nullptr;
+    b.tr = (double*)D(oTr); b.tre = (double*)D(oTre);
+    b.info = (int*)D(oInfo);
+    if (int rc = rgf_solve_batched(p, &b, s)) return rc;
+    // outputs: diagonal blocks for l = 1..nx, off-diagonal and cur for l = 1..nx-1
+    for (int k = 0; k < 6; ++k) {
+      if (!outs[k]) continue;
+      const bool offd = (k >= 3);
+      const size_t nb_copy = offd ? (size_t)(nx - 1) : (size_t)nx;
+      double* host = outs[k] + 2 * bn * nx * e0;
+      const void* src = D(oOut[k]);
+      if (padded) {
+        hipLaunchKernelGGL(unpad_blocks<double2>, dim3(1024), dim3(256), 0, s, (const double2*)src, np,
+                           (double2*)D(oU), nm, (long long)(nx * ce));
+        HIPCHK(hipGetLastError());
+        src = D(oU);
+      }
+      HIPCHK(hipMemcpy2DAsync(host, cslab_u, src, cslab_u, nb_copy * bn * 16, ce, hipMemcpyDeviceToHost, s));
+    }
+    if (cur) {
+      double* host = cur + bn * nx * e0;
+      const void* src = D(oCur);
+      if (padded) {
+        hipLaunchKernelGGL(unpad_blocks<double>, dim3(1024), dim3(256), 0, s, (const double*)src, np,
+                           (double*)D(oU), nm, (long long)(nx * ce));
+        HIPCHK(hipGetLastError());
+        src = D(oU);
+      }
+      HIPCHK(hipMemcpy2DAsync(host, rslab_u, src, rslab_u, (size_t)(nx - 1) * bn * 8, ce, hipMemcpyDeviceToHost, s));
+    }
+    if (tr) HIPCHK(hipMemcpyAsync(tr + e0, D(oTr), 8 * ce, hipMemcpyDeviceToHost, s));
+    if (tre) HIPCHK(hipMemcpyAsync(tre + e0, D(oTre), 8 * ce, hipMemcpyDeviceToHost, s));
+    if (info) HIPCHK(hipMemcpyAsync(info + 4 * e0, D(oInfo), 16 * ce, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (cur)  // the reference zeroes cur on entry and never writes block nx
+      for (long long e = 0; e < ce; ++e) std::memset(cur + bn * nx * (e0 + e) + bn * (nx - 1), 0, bn * 8);
+  }
+  return 0;
+}
+
+int rgf_rs_ex(const rgf_params* p, const double temp[2], int nm, int nx, double E, const double mu[2],
+              const double* Hii, const double* H1i, const double* sigL, const double* sigG, const double* sigR,
+              double* ndens, double* pdens, double* ldos, double* tr, double* tre, double* cur, double* GRi1,
+              double* GLi1, double* GGi1) {
+  return rgf_rs_energies(p, temp, nm, nx, 1, &E, mu, Hii, H1i, sigL, sigG, sigR, 0, ndens, pdens, ldos, tr, tre,
+                         cur, GRi1, GLi1, GGi1, nullptr);
+}
+
+int rgf_rs(const double temp[2], int nm, int nx, double E, const double mu[2], const double* Hii,
+           const double* H1i, const double* sigL, const double* sigG, const double* sigR, double* ndens,
+           double* pdens, double* ldos, double* tr, double* tre, double* cur, double* GRi1, double* GLi1,
+           double* GGi1) {
+  rgf_params p;
+  rgf_params_reference(&p);
+  return rgf_rs_ex(&p, temp, nm, nx, E, mu, Hii, H1i, sigL, sigG, sigR, ndens, pdens, ldos, tr, tre, cur, GRi1,
+                   GLi1, GGi1);
+}
+
+int rgf_zgemm_batched(int n, int nbatch, int opA, const double* A, int opB, const double* B, double* C,
+                      int c_rowmajor, void* stream) {
+  g_err.clear();
+  if (nbatch < 0 || (opA != 0 && opA != 1) || (opB != 0 && opB != 1)) return fail(RGF_EINVAL, "bad arguments");
+  if (nbatch == 0) return 0;
+  if (!A || !B || !C) return fail(RGF_EINVAL, "NULL matrix");
+  Variant v;
+  if (!pick_variant(n, &v)) return fail(RGF_EUNSUP, "n must be a multiple of 32 in [32, 256]");
+  const int grid = std::min(nbatch, 4096);
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  const double2* a = reinterpret_cast<const double2*>(A);
+  const double2* b = reinterpret_cast<const double2*>(B);
+  double2* c = reinterpret_cast<double2*>(C);
+  switch (v.T) {
+    case 1: hipLaunchKernelGGL(rgf::k_gemm_batched<1>, dim3(grid), dim3(rgf::NT), 0, s, n, nbatch, opA, a, opB, b, c, c_rowmajor); break;
+    case 2: hipLaunchKernelGGL(rgf::k_gemm_batched<2>, dim3(grid), dim3(rgf::NT), 0, s, n, nbatch, opA, a, opB, b, c, c_rowmajor); break;
+    case 3: hipLaunchKernelGGL(rgf::k_gemm_batched<3>, dim3(grid), dim3(rgf::NT), 0, s, n, nbatch, opA, a, opB, b, c, c_rowmajor); break;
+    default: hipLaunchKernelGGL(rgf::k_gemm_batched<4>, dim3(grid), dim3(rgf::NT), 0, s, n, nbatch, opA, a, opB, b, c, c_rowmajor); break;
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int rgf_zinv_batched(int n, int nbatch, const double* A, double* Ainv, int* info, void* stream) {
+  g_err.clear();
+  if (nbatch < 0) return fail(RGF_EINVAL, "nbatch < 0");
+  if (nbatch == 0) return 0;
+  if (!A || !Ainv) return fail(RGF_EINVAL, "NULL matrix");
+  Variant v;
+  if (!pick_variant(n, &v)) return fail(RGF_EUNSUP, "n must be a multiple of 32 in [32, 256]");
+  const int grid = std::min(nbatch, 1024);
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  // the row-major work buffers live in the engine workspace
+  if (int rc = g_ws[dev].ensure((size_t)grid * n * n * 16)) return rc;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  const double2* a = reinterpret_cast<const double2*>(A);
+  double2* o = reinterpret_cast<double2*>(Ainv);
+  double2* w = reinterpret_cast<double2*>(g_ws[dev].p);
+  switch (n) {
+    case 32: hipLaunchKernelGGL((rgf::k_inv_batched<1, 32>), dim3(grid), dim3(rgf::NT), 0, s, n, nbatch, a, o, w, info); break;
+    case 64: hipLaunchKernelGGL((rgf::k_inv_batched<2, 64>), dim3(grid), dim3(rgf::NT), 0, s, n, nbatch, a, o, w, info); break;
+    case 96: hipLaunchKernelGGL((rgf::k_inv_batched<3, 96>), dim3(grid), dim3(rgf::NT), 0, s, n, nbatch, a, o, w, info); break;
+    case 128: hipLaunchKernelGGL((rgf::k_inv_batched<4, 128>), dim3(grid), dim3(rgf::NT), 0, s, n, nbatch, a, o, w, info); break;
+    case 192: hipLaunchKernelGGL((rgf::k_inv_batched<3, 192>), dim3(grid), dim3(rgf::NT), 0, s, n, nbatch, a, o, w, info); break;
+    case 256: hipLaunchKernelGGL((rgf::k_inv_batched<4, 256>), dim3(grid), dim3(rgf::NT), 0, s, n, nbatch, a, o, w, info); break;
+    default: hipLaunchKernelGGL((rgf::k_inv_batched<1, 256>), dim3(grid), dim3(rgf::NT), 0, s, n, nbatch, a, o, w, info); break;
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+void rgf_release(void) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDev) return;
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  g_ws[dev].release();
+  g_stage[dev].release();
+}
+
+}  // extern "C"

@@ -1,0 +1,248 @@
+"""Python binding of the MI355X RGF engine (librgf_hip.so, C ABI of include/rgf.h).
+
+Mirrors the reference's operator for this path, green_RGF_RS
+(src/green_rgf.f90:3950-3958): same argument order and meaning, complex128
+arrays in Fortran layout (nm, nm, nx), optional off-diagonal outputs.  Three
+levels:
+  * rgf_rs(...)            one energy, host (numpy) arrays, drop-in semantics;
+  * rgf_rs_energies(...)   the drivers' energy loop (green_rgf.f90:3174-3180)
+                           as one batched GPU solve, host arrays;
+  * solve_batched(...)     device-resident torch tensors, asynchronous on the
+                           current HIP stream (the benchmark / SCBA path).
+There is no CPU fallback: every call runs the HIP kernels and raises when the
+library or a GPU is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librgf_hip.so")
+
+MODE_COMPAT = 0
+MODE_PHYSICAL = 1
+
+ERRORS = {1: "EINVAL", 2: "EHIP", 3: "ENOMEM", 4: "EUNSUP"}
+
+
+class RgfError(RuntimeError):
+    pass
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("eta_dev", ctypes.c_double), ("eta_lead", ctypes.c_double),
+                ("dec_tol", ctypes.c_double), ("dec_nmax", ctypes.c_int),
+                ("boltz", ctypes.c_double), ("mode", ctypes.c_int)]
+
+
+class Batch(ctypes.Structure):
+    _fields_ = [("nm", ctypes.c_int), ("nx", ctypes.c_int), ("nE", ctypes.c_int),
+                ("max_slots", ctypes.c_int),
+                ("E", ctypes.c_void_p), ("temp", ctypes.c_double * 2), ("mu", ctypes.c_double * 2),
+                ("Hii", ctypes.c_void_p), ("H1i", ctypes.c_void_p),
+                ("sigL", ctypes.c_void_p), ("sigG", ctypes.c_void_p), ("sigR", ctypes.c_void_p),
+                ("sig_stride", ctypes.c_longlong),
+                ("ldos", ctypes.c_void_p), ("ndens", ctypes.c_void_p), ("pdens", ctypes.c_void_p),
+                ("GRi1", ctypes.c_void_p), ("GLi1", ctypes.c_void_p), ("GGi1", ctypes.c_void_p),
+                ("cur", ctypes.c_void_p), ("tr", ctypes.c_void_p), ("tre", ctypes.c_void_p),
+                ("dens_n", ctypes.c_void_p), ("dens_p", ctypes.c_void_p), ("info", ctypes.c_void_p)]
+
+
+# every function declared in include/rgf.h
+EXPORTS = ("rgf_params_reference", "rgf_last_error", "rgf_version", "rgf_rs", "rgf_rs_ex",
+           "rgf_rs_energies", "rgf_solve_batched", "rgf_workspace_bytes", "rgf_release",
+           "rgf_zgemm_batched", "rgf_zinv_batched")
+
+_lib = None
+
+
+def lib():
+    """Load librgf_hip.so (built by __graft_entry__.build()); raise if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RgfError(f"HIP engine library missing: {LIB_PATH} (run __graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    L.rgf_params_reference.argtypes = [ctypes.POINTER(Params)]
+    L.rgf_last_error.restype = ctypes.c_char_p
+    L.rgf_version.restype = ctypes.c_char_p
+    L.rgf_rs.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, vp] + [vp] * 14
+    L.rgf_rs_ex.argtypes = [ctypes.POINTER(Params), vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, vp] + [vp] * 14
+    L.rgf_rs_energies.argtypes = ([ctypes.POINTER(Params), vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp,
+                                   vp, vp, vp, vp, vp, ctypes.c_int] + [vp] * 10)
+    L.rgf_solve_batched.argtypes = [ctypes.POINTER(Params), ctypes.POINTER(Batch), vp]
+    L.rgf_zgemm_batched.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, vp, vp,
+                                    ctypes.c_int, vp]
+    L.rgf_zinv_batched.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp, vp, vp]
+    L.rgf_workspace_bytes.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.rgf_workspace_bytes.restype = ctypes.c_longlong
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        msg = lib().rgf_last_error().decode(errors="replace")
+        raise RgfError(f"rgf error {ERRORS.get(rc, rc)}: {msg}")
+
+
+def params(mode=MODE_COMPAT, eta_dev=None, eta_lead=None, dec_tol=None, dec_nmax=None, boltz=None):
+    """Reference parameters (green_rgf.f90:3968, 3974, 4457, 4473, 4474) unless overridden."""
+    p = Params()
+    lib().rgf_params_reference(ctypes.byref(p))
+    p.mode = mode
+    for k, v in (("eta_dev", eta_dev), ("eta_lead", eta_lead), ("dec_tol", dec_tol),
+                 ("dec_nmax", dec_nmax), ("boltz", boltz)):
+        if v is not None:
+            setattr(p, k, v)
+    return p
+
+
+def _cf(a):
+    return None if a is None else np.asfortranarray(a, dtype=np.complex128)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def rgf_rs(temp, E, mu, Hii, H1i, sigL=None, sigG=None, sigR=None, p=None, offdiag=True):
+    """One energy with green_RGF_RS semantics on the GPU.  Returns a dict with
+    ndens, pdens, ldos, cur, tr, tre (and GRi1, GLi1, GGi1 when offdiag)."""
+    Hii, H1i = _cf(Hii), _cf(H1i)
+    sigL, sigG, sigR = _cf(sigL), _cf(sigG), _cf(sigR)
+    nm, _, nx = Hii.shape
+    shp = (nm, nm, nx)
+    out = {k: np.zeros(shp, np.complex128, order="F") for k in ("ndens", "pdens", "ldos")}
+    out["cur"] = np.zeros(shp, np.float64, order="F")
+    if offdiag:
+        for k in ("GRi1", "GLi1", "GGi1"):
+            out[k] = np.zeros(shp, np.complex128, order="F")
+    tr = np.zeros(1)
+    tre = np.zeros(1)
+    t = np.ascontiguousarray(temp, np.float64)
+    m = np.ascontiguousarray(mu, np.float64)
+    if p is None:
+        p = params()
+    rc = lib().rgf_rs_ex(ctypes.byref(p), _ptr(t), nm, nx, float(E), _ptr(m), _ptr(Hii), _ptr(H1i), _ptr(sigL),
+                         _ptr(sigG), _ptr(sigR), _ptr(out["ndens"]), _ptr(out["pdens"]), _ptr(out["ldos"]),
+                         _ptr(tr), _ptr(tre), _ptr(out["cur"]), _ptr(out.get("GRi1")), _ptr(out.get("GLi1")),
+                         _ptr(out.get("GGi1")))
+    _check(rc)
+    out["tr"] = float(tr[0])
+    out["tre"] = float(tre[0])
+    return out
+
+
+def rgf_rs_energies(p, temp, E, mu, Hii, H1i, sigL=None, sigG=None, sigR=None,
+                    outputs=("ndens", "pdens", "ldos", "cur", "GRi1", "GLi1", "GGi1")):
+    """All energies of E in one batched GPU solve (host arrays).  sig* either
+    (nm,nm,nx) shared or (nm,nm,nx,nE) per energy.  Returns a dict with a
+    trailing energy axis, plus tr, tre and info (nE, 4)."""
+    Hii, H1i = _cf(Hii), _cf(H1i)
+    nm, _, nx = Hii.shape
+    E = np.ascontiguousarray(E, np.float64)
+    nE = E.shape[0]
+    sigs = [_cf(s) for s in (sigL, sigG, sigR)]
+    per_e = any(s is not None and s.ndim == 4 for s in sigs)
+    if per_e:
+        sigs = [None if s is None else (s if s.ndim == 4 else
+                np.asfortranarray(np.repeat(s[..., None], nE, axis=3))) for s in sigs]
+    shp = (nm, nm, nx, nE)
+    out = {}
+    for k in ("ndens", "pdens", "ldos", "GRi1", "GLi1", "GGi1"):
+        out[k] = np.zeros(shp, np.complex128, order="F") if k in outputs else None
+    out["cur"] = np.zeros(shp, np.float64, order="F") if "cur" in outputs else None
+    tr = np.zeros(nE)
+    tre = np.zeros(nE)
+    info = np.zeros((nE, 4), np.int32)
+    t = np.ascontiguousarray(temp, np.float64)
+    m = np.ascontiguousarray(mu, np.float64)
+    rc = lib().rgf_rs_energies(ctypes.byref(p), _ptr(t), nm, nx, nE, _ptr(E), _ptr(m), _ptr(Hii), _ptr(H1i),
+                               _ptr(sigs[0]), _ptr(sigs[1]), _ptr(sigs[2]), int(per_e),
+                               _ptr(out["ndens"]), _ptr(out["pdens"]), _ptr(out["ldos"]), _ptr(tr), _ptr(tre),
+                               _ptr(out["cur"]), _ptr(out["GRi1"]), _ptr(out["GLi1"]), _ptr(out["GGi1"]),
+                               _ptr(info))
+    _check(rc)
+    out["tr"] = tr
+    out["tre"] = tre
+    out["info"] = info
+    return {k: v for k, v in out.items() if v is not None}
+
+
+def _tp(t):
+    return None if t is None else t.data_ptr()
+
+
+def solve_batched(p, E, temp, mu, Hii, H1i, sigL=None, sigR=None, out=None, max_slots=0, stream=None):
+    """Device-resident batched solve.  E, Hii, H1i, sig*, out[...] are torch
+    tensors on the HIP device: complex blocks as complex128 tensors of shape
+    (nE, nx, nm, nm) holding column-major blocks (i.e. the memory of a Fortran
+    (nm,nm,nx,nE) array; use to_device_blocks()).  out maps names in
+    {ldos, ndens, pdens, GRi1, GLi1, GGi1, cur, tr, tre, dens_n, dens_p, info}
+    to preallocated tensors.  Asynchronous on `stream` (default: torch's current
+    stream)."""
+    import torch
+    out = out or {}
+    nE = int(E.shape[0])
+    nx, nm = int(Hii.shape[-3]), int(Hii.shape[-1])
+    b = Batch()
+    b.nm, b.nx, b.nE, b.max_slots = nm, nx, nE, int(max_slots)
+    b.E = _tp(E)
+    b.temp[0], b.temp[1] = float(temp[0]), float(temp[1])
+    b.mu[0], b.mu[1] = float(mu[0]), float(mu[1])
+    b.Hii, b.H1i = _tp(Hii), _tp(H1i)
+    b.sigL, b.sigR = _tp(sigL), _tp(sigR)
+    stride = 0
+    for s in (sigL, sigR):
+        if s is not None and s.dim() == 4:
+            stride = nm * nm * nx
+    b.sig_stride = stride
+    for k in ("ldos", "ndens", "pdens", "GRi1", "GLi1", "GGi1", "cur", "tr", "tre", "dens_n", "dens_p", "info"):
+        setattr(b, k, _tp(out.get(k)))
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    rc = lib().rgf_solve_batched(ctypes.byref(p), ctypes.byref(b), ctypes.c_void_p(stream.cuda_stream))
+    _check(rc)
+    return out
+
+
+def to_device_blocks(a, device="cuda"):
+    """numpy Fortran (nm,nm,nx[,nE]) complex array -> torch complex128 tensor whose
+    memory is the same Fortran layout: shape (nx, nm, nm) / (nE, nx, nm, nm) with
+    the last two axes (col, row)."""
+    import torch
+    a = np.asfortranarray(a, dtype=np.complex128)
+    t = np.ascontiguousarray(a.transpose(tuple(range(a.ndim - 1, -1, -1))))
+    return torch.from_numpy(t).to(device)
+
+
+def from_device_blocks(t):
+    """Inverse of to_device_blocks."""
+    a = t.detach().cpu().numpy()
+    return np.asfortranarray(a.transpose(tuple(range(a.ndim - 1, -1, -1))))
+
+
+def zgemm_batched(A, B, opA=0, opB=0, rowmajor=False, stream=None):
+    """C[b] = op(A[b]) op(B[b]) for torch complex128 tensors (nb, n, n) holding
+    column-major matrices (see to_device_blocks)."""
+    import torch
+    C = torch.empty_like(A)
+    s = stream or torch.cuda.current_stream()
+    _check(lib().rgf_zgemm_batched(int(A.shape[-1]), int(A.shape[0]), int(opA), A.data_ptr(), int(opB),
+                                   B.data_ptr(), C.data_ptr(), int(bool(rowmajor)), ctypes.c_void_p(s.cuda_stream)))
+    return C
+
+
+def zinv_batched(A, stream=None):
+    """(A[b]^-1, info) for torch complex128 tensors (nb, n, n) of column-major matrices."""
+    import torch
+    C = torch.empty_like(A)
+    info = torch.zeros(A.shape[0], dtype=torch.int32, device=A.device)
+    s = stream or torch.cuda.current_stream()
+    _check(lib().rgf_zinv_batched(int(A.shape[-1]), int(A.shape[0]), A.data_ptr(), C.data_ptr(), info.data_ptr(),
+                                  ctypes.c_void_p(s.cuda_stream)))
+    return C, info

@@ -1,0 +1,42 @@
+"""The C-ABI library loads and exports every function include/rgf.h declares
+(no compute call: runs without a GPU)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from liteqt_amd import rgf
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    src = open(os.path.join(ROOT, "include", "rgf.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(rgf_[a-z_]+)\s*\(", src)))
+
+
+def test_header_declares_expected_entry_points():
+    assert header_functions() == sorted(rgf.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(rgf.LIB_PATH):
+        pytest.fail("librgf_hip.so not built (run __graft_entry__.build())")
+    L = ctypes.CDLL(rgf.LIB_PATH)
+    for name in header_functions():
+        assert hasattr(L, name), name
+
+
+def test_reference_parameters():
+    p = rgf.params()
+    assert p.eta_dev == 0.0 and p.eta_lead == 1e-3 and p.dec_tol == 1e-100
+    assert p.dec_nmax == 100 and p.boltz == 8.61734e-05 and p.mode == rgf.MODE_COMPAT
+
+
+def test_workspace_bytes():
+    # (2*nx + temporaries) blocks of nm^2 complex128 per energy slot
+    b1 = rgf.lib().rgf_workspace_bytes(256, 200, 1)
+    assert b1 % (256 * 256 * 16) == 0 and b1 // (256 * 256 * 16) >= 400
+    assert rgf.lib().rgf_workspace_bytes(256, 200, 10) == 10 * b1

@@ -1,0 +1,156 @@
+"""GPU parity: the HIP engine against the CPU oracle on the same seeded inputs.
+
+Bar: relative Frobenius error <= 1e-10 on every G block (diagonal and first
+off-diagonal of G^R, G^<, G^>), on cur and on tr/tre; quantities the reference
+forms by cancellation (G^> = G^< + G - G^H and what is built from it) are
+measured against the scale of the cancelling terms (tests/parity.py).
+"""
+import numpy as np
+import pytest
+
+from liteqt_amd import configs
+from oracle import pyoracle as orc
+from tests import parity
+
+pytestmark = pytest.mark.gpu
+
+TOL = parity.TOL
+
+
+def _rgf():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.fail("no HIP device visible")
+    from liteqt_amd import rgf
+    return rgf
+
+
+def relerr(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / nb if nb > 0 else np.linalg.norm(a - b)
+
+
+def compare(gpu, ref, H1i, mode, tag=""):
+    return parity.check(gpu, ref, H1i, mode, tag)
+
+
+def case(nm, nx, seed, uniform, scatter=True):
+    Hii, H1i = configs.random_device(nm, nx, seed, identical_leads=False)
+    if uniform:
+        H1i = configs.uniform_coupling(H1i)
+    s = configs.scattering_self_energies(nm, nx, seed + 7, E=0.1, mu=0.0, scale=0.02) if scatter else (None,) * 3
+    return Hii, H1i, s
+
+
+@pytest.mark.parametrize("nm,nx", [(32, 4), (64, 3), (96, 3), (128, 3), (192, 2), (256, 2), (20, 3), (40, 3), (1, 2)])
+@pytest.mark.parametrize("mode", ["compat", "physical"])
+def test_rgf_rs_matches_oracle(nm, nx, mode):
+    rgf = _rgf()
+    Hii, H1i, (sL, sG, sR) = case(nm, nx, 100 + nm + nx, uniform=(mode == "compat"))
+    temp, mu, E = (300.0, 250.0), (0.1, -0.1), 0.13
+    if mode == "compat":
+        p, po = rgf.params(rgf.MODE_COMPAT), orc.params(orc.COMPAT)
+    else:
+        p = rgf.params(rgf.MODE_PHYSICAL, eta_dev=1e-4, eta_lead=1e-4, dec_tol=1e-12)
+        po = orc.params(orc.PHYSICAL, eta_dev=1e-4, eta_lead=1e-4, dec_tol=1e-12)
+    g = rgf.rgf_rs(temp, E, mu, Hii, H1i, sL, sG, sR, p=p)
+    r = orc.rgf(po, temp, E, mu, Hii, H1i, sL, sG, sR)
+    compare(g, r, H1i, mode, f"nm={nm} nx={nx} {mode}")
+
+
+def test_rgf_rs_default_entry_is_the_reference_configuration():
+    rgf = _rgf()
+    Hii, H1i, (sL, sG, sR) = case(32, 3, 5, uniform=False)
+    g = rgf.rgf_rs((300.0, 300.0), -0.2, (0.05, 0.0), Hii, H1i, sL, sG, sR)
+    r = orc.rgf(orc.params(orc.COMPAT), (300.0, 300.0), -0.2, (0.05, 0.0), Hii, H1i, sL, sG, sR)
+    compare(g, r, H1i, "compat", "default")
+    # untouched block nx of the optional off-diagonal outputs, zero cur block nx
+    assert np.all(g["GRi1"][..., -1] == 0) and np.all(g["cur"][..., -1] == 0)
+
+
+def test_energy_loop_batched_matches_oracle():
+    rgf = _rgf()
+    nm, nx, nE = 64, 5, 37
+    Hii, H1i = configs.random_device(nm, nx, 9)
+    E = configs.energy_grid(-1.5, 1.5, nE)
+    sL, sG, sR = configs.scattering_self_energies(nm, nx, 10, E=E, mu=0.0, scale=0.01)
+    p = rgf.params(rgf.MODE_PHYSICAL, eta_dev=1e-5, eta_lead=1e-5, dec_tol=1e-12)
+    po = orc.params(orc.PHYSICAL, eta_dev=1e-5, eta_lead=1e-5, dec_tol=1e-12)
+    temp, mu = (300.0, 300.0), (0.1, -0.1)
+    g = rgf.rgf_rs_energies(p, temp, E, mu, Hii, H1i, sL, sG, sR)
+    r = orc.rgf_energies(po, temp, E, mu, Hii, H1i, sL, sG, sR)
+    for e in range(nE):
+        compare(parity.energy_slice(g, e), parity.energy_slice(r, e), H1i, "physical", f"E[{e}]")
+        assert tuple(g["info"][e, :2]) == tuple(r["diag"][e][:2])
+
+
+def test_device_batched_persistent_loop_and_reductions():
+    """More energies than workgroups (max_slots=5): every energy still solved;
+    dens_n/dens_p are Im Tr G^< and -Im Tr G^> per slab."""
+    rgf = _rgf()
+    import torch
+    nm, nx, nE = 32, 6, 23
+    Hii, H1i = configs.random_device(nm, nx, 21)
+    E = configs.energy_grid(-1.0, 1.0, nE)
+    p = rgf.params(rgf.MODE_PHYSICAL, eta_dev=1e-4, eta_lead=1e-4, dec_tol=1e-12)
+    po = orc.params(orc.PHYSICAL, eta_dev=1e-4, eta_lead=1e-4, dec_tol=1e-12)
+    temp, mu = (300.0, 300.0), (0.05, -0.05)
+    dev = torch.device("cuda:0")
+    Et = torch.tensor(E, dtype=torch.float64, device=dev)
+    out = {k: torch.zeros(nE, dtype=torch.float64, device=dev) for k in ("tr", "tre")}
+    out["dens_n"] = torch.zeros(nE, nx, dtype=torch.float64, device=dev)
+    out["dens_p"] = torch.zeros(nE, nx, dtype=torch.float64, device=dev)
+    out["ldos"] = torch.zeros(nE, nx, nm, nm, dtype=torch.complex128, device=dev)
+    out["info"] = torch.zeros(nE, 4, dtype=torch.int32, device=dev)
+    rgf.solve_batched(p, Et, temp, mu, rgf.to_device_blocks(Hii), rgf.to_device_blocks(H1i), out=out, max_slots=5)
+    torch.cuda.synchronize()
+    r = orc.rgf_energies(po, temp, E, mu, Hii, H1i, outputs=("ndens", "pdens", "ldos"))
+    ldos = rgf.from_device_blocks(out["ldos"])
+    for e in range(nE):
+        for l in range(nx):
+            assert relerr(ldos[..., l, e], r["ldos"][..., l, e]) <= TOL
+        dn = np.imag(np.trace(r["ndens"][..., e], axis1=0, axis2=1))
+        dp = -np.imag(np.trace(r["pdens"][..., e], axis1=0, axis2=1))
+        assert np.allclose(out["dens_n"][e].cpu().numpy(), dn, rtol=1e-10, atol=1e-12)
+        assert np.allclose(out["dens_p"][e].cpu().numpy(), dp, rtol=1e-10, atol=1e-12)
+    assert np.allclose(out["tr"].cpu().numpy(), r["tr"], rtol=1e-10, atol=1e-12)
+    assert np.allclose(out["tre"].cpu().numpy(), r["tre"], rtol=1e-10, atol=1e-12)
+
+
+def test_singular_slab_zeroed_and_counted():
+    rgf = _rgf()
+    nm, nx = 32, 3
+    Hii = np.zeros((nm, nm, nx), np.complex128, order="F")
+    H1i = np.zeros((nm, nm, nx), np.complex128, order="F")
+    g = rgf.rgf_rs_energies(rgf.params(), (300.0, 300.0), np.array([0.0]), (0.0, 0.0), Hii, H1i)
+    assert g["info"][0, 3] >= nx
+    assert np.all(g["ldos"] == 0)
+
+
+def test_decimation_cap_fallback_matches_oracle():
+    rgf = _rgf()
+    Hii, H1i, _ = case(32, 3, 77, uniform=True, scatter=False)
+    p = rgf.params(rgf.MODE_COMPAT, dec_nmax=2)
+    po = orc.params(orc.COMPAT, dec_nmax=2)
+    E = np.array([0.3])
+    g = rgf.rgf_rs_energies(p, (300.0, 300.0), E, (0.0, 0.0), Hii, H1i)
+    r = orc.rgf_energies(po, (300.0, 300.0), E, (0.0, 0.0), Hii, H1i)
+    assert g["info"][0, 2] == 3 and r["diag"][0][2] == 3
+    for l in range(3):
+        assert relerr(g["ldos"][..., l, 0], r["ldos"][..., l, 0]) <= TOL
+
+
+def test_cnt_ballistic_subset_matches_oracle():
+    """(8,0) CNT, 32 orbitals, 100 cells, potential drop, eta 1e-4, tol 1e-12: T(E) and diag G^R."""
+    rgf = _rgf()
+    Hii, H1i = configs.cnt_80_ballistic(L=100)
+    E = configs.energy_grid(-3.0, 3.0, 1000)[::125]
+    p = rgf.params(rgf.MODE_COMPAT, eta_dev=1e-4, eta_lead=1e-4, dec_tol=1e-12)
+    po = orc.params(orc.COMPAT, eta_dev=1e-4, eta_lead=1e-4, dec_tol=1e-12)
+    temp, mu = (300.0, 300.0), (100.0, -100.0)
+    g = rgf.rgf_rs_energies(p, temp, E, mu, Hii, H1i, outputs=("ldos",))
+    r = orc.rgf_energies(po, temp, E, mu, Hii, H1i, outputs=("ldos",))
+    for e in range(len(E)):
+        for l in range(100):
+            assert relerr(g["ldos"][..., l, e], r["ldos"][..., l, e]) <= TOL
+        assert abs(g["tr"][e] - r["tr"][e]) <= TOL * max(1, abs(r["tr"][e]))
