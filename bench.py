@@ -1,0 +1,199 @@
+"""Benchmark of the batched RGF engine on one MI355X.
+
+Workload (one step): the random-device RGF solve with nm = 256 orbitals per
+slab, nx = 200 slabs and 2048 energies in [-2, 2] eV (eta 1e-5 eV in the
+device and the leads, decimation tolerance 1e-12, Fermi leads mu_L = 0.1,
+mu_R = -0.1 eV, kT = 0.0259 eV), complex128, all energies in one persistent
+launch.  Every step computes per energy the lead decimations, the forward
+sweep and the backward sweep for G^R_ll, G^<_ll, G^R_{l,l+1} and
+G^<_{l,l+1}; it retains T(E) spectra (tr, tre) and the per-slab electron and
+hole densities Im Tr G^<_ll, -Im Tr G^>_ll for every energy.
+
+Prints one JSON line: energy-points/s (value), the FP64-MFMA roofline of the
+sweep kernel measured with HIP events on its stream, and the CPU oracle timed
+on this host on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_MFMA = 78.6  # TFLOP/s, MI355X dense FP64 matrix
+PEAK_HBM = 8000.0      # GB/s
+
+CONFIGS = {
+    "random256": dict(nm=256, nx=200, nE=2048, emin=-2.0, emax=2.0, eta=1e-5, seed=2,
+                      workload="random device nm=256 nx=200, 2048 energies in [-2,2] eV, eta 1e-5, Fermi leads"),
+    "cnt32": dict(nm=32, nx=100, nE=1000, emin=-3.0, emax=3.0, eta=1e-4, seed=None,
+                  workload="(8,0) CNT pz TB nm=32 nx=100, 1000 energies in [-3,3] eV, eta 1e-4, ballistic"),
+}
+
+
+def build_inputs(cfg):
+    from liteqt_amd import configs
+    if cfg["seed"] is None:
+        Hii, H1i = configs.cnt_80_ballistic(L=cfg["nx"])
+    else:
+        Hii, H1i = configs.random_device(cfg["nm"], cfg["nx"], cfg["seed"])
+    E = configs.energy_grid(cfg["emin"], cfg["emax"], cfg["nE"])
+    return Hii, H1i, E
+
+
+def algorithmic_flops(nm, nx, iters):
+    """Flops of the step's output set per energy: 8 nm^3 [17 nx + 7 (kL + kR)]
+    (forward 6 GEMM + 1 inverse, backward 10 GEMM per slab; 1 inverse + 6 GEMM
+    per decimation iteration), summed over energies."""
+    k = iters[:, 0].astype(np.float64) + iters[:, 1].astype(np.float64)
+    return float(np.sum(8.0 * nm ** 3 * (17.0 * nx + 7.0 * k)))
+
+
+def cpu_baseline(cfg, params_kw, temp, mu, threads):
+    """Time the CPU oracle (OpenMP over energies, one per thread) on `threads`
+    energies of the workload at nx = 2 and nx = 6 slabs; the cost is linear in
+    nx, so the per-energy time at the full nx is extrapolated from the two."""
+    from liteqt_amd import configs
+    from oracle import pyoracle as orc
+    p = orc.params(orc.COMPAT, **params_kw)
+    E = configs.energy_grid(cfg["emin"], cfg["emax"], cfg["nE"])[:threads]
+    ts = {}
+    for nx_s in (2, 6):
+        if cfg["seed"] is None:
+            Hii, H1i = configs.cnt_80_ballistic(L=nx_s)
+        else:
+            Hii, H1i = configs.random_device(cfg["nm"], nx_s, cfg["seed"])
+        t0 = time.perf_counter()
+        orc.rgf_energies(p, temp, E, mu, Hii, H1i, outputs=(), nthreads=threads)
+        ts[nx_s] = time.perf_counter() - t0
+    per_slab = (ts[6] - ts[2]) / 4.0
+    t_full = ts[2] + per_slab * (cfg["nx"] - 2)  # wall time for `threads` energies at full nx
+    return {"value": threads / t_full, "unit": "energy-points/s", "cores": threads, "kind": "port",
+            "sample": f"{threads} energies x nx in {{2,6}} slabs at nm={cfg['nm']} "
+                      f"(one energy per thread, {ts[2]:.1f}s + {ts[6]:.1f}s wall), linear in nx "
+                      f"extrapolated to nx={cfg['nx']}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="random256", choices=sorted(CONFIGS))
+    ap.add_argument("--energies", type=int, default=0, help="override the number of energies")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (counter calibration)")
+    args = ap.parse_args()
+    if args.copy_calibrate:
+        import torch
+        nbytes = args.copy_calibrate << 20
+        src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": nbytes, "dispatches": 2, "note": "fill + copy; the copy reads and writes copy_bytes"}))
+        return
+    if args.gpus != 1:
+        print(json.dumps({"error": "this RGF build runs on one GPU; multi-GPU energy sharding is a design note only",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import torch
+    from liteqt_amd import rgf
+
+    cfg = dict(CONFIGS[args.config])
+    if args.energies:
+        cfg["nE"] = args.energies
+    nm, nx, nE = cfg["nm"], cfg["nx"], cfg["nE"]
+    kT = 0.0259
+    if args.config == "cnt32":
+        temp, mu = (300.0, 300.0), (100.0, -100.0)
+    else:
+        T = kT / 8.61734e-05
+        temp, mu = (T, T), (0.1, -0.1)
+    pkw = dict(eta_dev=cfg["eta"], eta_lead=cfg["eta"], dec_tol=1e-12, dec_nmax=100)
+    p = rgf.params(rgf.MODE_COMPAT, **pkw)
+
+    Hii, H1i, E = build_inputs(cfg)
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    Hd, Cd = rgf.to_device_blocks(Hii, dev), rgf.to_device_blocks(H1i, dev)
+    Ed = torch.tensor(E, dtype=torch.float64, device=dev)
+    out = {
+        "tr": torch.zeros(nE, dtype=torch.float64, device=dev),
+        "tre": torch.zeros(nE, dtype=torch.float64, device=dev),
+        "dens_n": torch.zeros(nE, nx, dtype=torch.float64, device=dev),
+        "dens_p": torch.zeros(nE, nx, dtype=torch.float64, device=dev),
+        "info": torch.zeros(nE, 4, dtype=torch.int32, device=dev),
+    }
+    stream = torch.cuda.current_stream()
+
+    def step():
+        rgf.solve_batched(p, Ed, temp, mu, Hd, Cd, out=out, stream=stream)
+
+    for i in range(args.warmup):
+        t = time.perf_counter()
+        step()
+        torch.cuda.synchronize()
+        print(f"bench: warmup step {i} {time.perf_counter() - t:.2f}s", file=sys.stderr, flush=True)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        step()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    print(f"bench: {args.steps} timed steps {wall:.2f}s", file=sys.stderr, flush=True)
+    kern_ms = ev0.elapsed_time(ev1) / args.steps
+    info = out["info"].cpu().numpy()
+    flops = algorithmic_flops(nm, nx, info)
+    achieved = flops / (kern_ms * 1e-3) / 1e12
+    ms_per_step = wall * 1e3 / args.steps
+    value = nE / (ms_per_step * 1e-3)
+    tr = out["tr"].cpu().numpy()
+    ok = bool(np.all(np.isfinite(tr)) and np.all(info[:, 3] == 0))
+
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tfile):
+        tinfo = json.load(open(tfile)).get(args.config)
+        if tinfo:
+            traffic = tinfo["bytes_per_energy"] * nE  # HBM bytes per launch, from the PMC profile
+    res = {
+        "metric": "batched RGF energy-pts/s (complex FP64, err<=1e-10 vs CPU oracle)",
+        "value": value,
+        "unit": "energy-points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128 (complex fp64)",
+        "data": "synthetic",
+        "config": {"workload": cfg["workload"], "nm": nm, "nx": nx, "energies": nE, "mode": "compat",
+                   "outputs": "tr, tre, dens_n, dens_p per energy; G^R/G^< diag + offdiag per slab on chip"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_MFMA, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_MFMA, "traffic": traffic,
+                     "kernel": f"rgf_sweep (nm={nm})", "kernel_ms": kern_ms,
+                     "flops_per_launch": flops},
+        "decimation_iters_mean": float(info[:, :2].mean()),
+        "sane": ok,
+    }
+    if not args.no_cpu:
+        threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count())
+        res["cpu_baseline"] = cpu_baseline(cfg, pkw, temp, mu, threads)
+        res["gpu_over_cpu"] = value / res["cpu_baseline"]["value"]
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()
