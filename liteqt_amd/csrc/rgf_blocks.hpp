@@ -12,16 +12,68 @@
 // holds D[4r + l/16][l%16] in accumulator element r.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <stdint.h>
 
 namespace rgf {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
-constexpr int NT = 256;   // threads per workgroup
 constexpr int PAD = 16;   // LDS row padding (doubles): row stride = 16 mod 32 doubles
 constexpr int KP = 16;    // K panel depth (complex)
 
 enum { OP_N = 0, OP_C = 1 };
+
+// Explicit address spaces: global (1) and LDS (3).  Pointers that reach a
+// non-inlined function as generic pointers would be accessed with FLAT
+// instructions, which count against both the vector-memory and the LDS wait
+// counters and so serialise the global prefetch with the LDS fragment reads.
+// The host pass only parses device code, so it sees the unqualified types.
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef __attribute__((address_space(3))) double ldsd;
+typedef __attribute__((address_space(3))) int ldsi;
+typedef __attribute__((address_space(1))) double2 gd2;
+typedef __attribute__((address_space(1))) const double2 gcd2;
+typedef __attribute__((address_space(1))) double gd;
+#else
+typedef double ldsd;
+typedef int ldsi;
+typedef double2 gd2;
+typedef const double2 gcd2;
+typedef double gd;
+#endif
+
+__device__ __forceinline__ gcd2* gc(const double2* p) { return (gcd2*)p; }
+__device__ __forceinline__ gd2* gw(double2* p) { return (gd2*)p; }
+
+// 16-byte complex load through a native vector type (one global_load_dwordx4).
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef __attribute__((address_space(1))) const dvec2 gcv2;
+#else
+typedef const dvec2 gcv2;
+#endif
+__device__ __forceinline__ double2 ldg(const double2* p, size_t i) {
+  const dvec2 v = ((gcv2*)p)[i];
+  return make_double2(v.x, v.y);
+}
+
+// Uniform values arrive in VGPRs through the calling convention of a
+// non-inlined function; moving them to SGPRs frees those VGPRs.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ double uni(double v) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+template <typename P>
+__device__ __forceinline__ P* uni(P* p) {
+  const unsigned long long b = (unsigned long long)(uintptr_t)p;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
+  return (P*)(uintptr_t)(((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ double2 uni(double2 v) { return make_double2(uni(v.x), uni(v.y)); }
 
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
@@ -69,9 +121,9 @@ __device__ __forceinline__ Epi epi_store() {
 __device__ __forceinline__ double2 epi_value(const Epi& ep, int n, int i, int j, double ar, double ai) {
   double2 v = make_double2(ep.sacc * ar, ep.sacc * ai);
   if (i == j) v = cadd(v, ep.diag);
-  if (ep.D1) v = cadd(v, cmul(ep.s1, ep.D1[i + (size_t)j * n]));
-  if (ep.D2) v = cadd(v, cmul(ep.s2, ep.D2[i + (size_t)j * n]));
-  if (ep.D3) v = cadd(v, cmul(ep.s3, cconj(ep.D3[j + (size_t)i * n])));
+  if (ep.D1) v = cadd(v, cmul(ep.s1, gc(ep.D1)[i + (size_t)j * n]));
+  if (ep.D2) v = cadd(v, cmul(ep.s2, gc(ep.D2)[i + (size_t)j * n]));
+  if (ep.D3) v = cadd(v, cmul(ep.s3, cconj(gc(ep.D3)[j + (size_t)i * n])));
   return v;
 }
 
@@ -81,19 +133,20 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Sum over the workgroup; result valid in every thread.  red: >= 4 doubles of LDS.
-__device__ __forceinline__ double block_sum(double v, double* red) {
+// Sum over the workgroup; result valid in every thread.  red: >= 8 doubles of LDS.
+__device__ __forceinline__ double block_sum(double v, ldsd* red) {
   v = wave_sum(v);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (lane == 0) red[wave] = v;
   __syncthreads();
-  double s = red[0] + red[1] + red[2] + red[3];
+  double s = 0.0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += red[w];
   __syncthreads();
   return s;
 }
 
 // Argmax of v over the workgroup, ties to the smallest index.
-__device__ __forceinline__ void block_argmax(double& v, int& idx, double* redv, int* redi) {
+__device__ __forceinline__ void block_argmax(double& v, int& idx, ldsd* redv, ldsi* redi) {
   for (int off = 32; off > 0; off >>= 1) {
     double ov = __shfl_xor(v, off, 64);
     int oi = __shfl_xor(idx, off, 64);
@@ -103,210 +156,250 @@ __device__ __forceinline__ void block_argmax(double& v, int& idx, double* redv, 
   if (lane == 0) { redv[wave] = v; redi[wave] = idx; }
   __syncthreads();
   v = redv[0]; idx = redi[0];
-  for (int w = 1; w < NT / 64; ++w) {
+  for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
     double ov = redv[w]; int oi = redi[w];
     if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
   }
   __syncthreads();
 }
 
+// ------------------------------------------------------------ tile configs
+// Per block-size variant T: a pass computes an MR x MC macro tile of C with NW
+// waves in a WR x WC grid, each wave owning TA x TB MFMA tiles of 16x16
+// complex (2*TA*TB accumulators of 4 doubles).  T = 4 (n = 128, 256) uses a
+// 128 x 64 macro tile: 64 x 32 per wave keeps a wave at 128 accumulator
+// registers, so nothing spills with one wave per SIMD.  Smaller T use square
+// (32T)^2 tiles.
+template <int T>
+struct Cfg {
+  static constexpr int MR = 32 * T;
+  static constexpr int MC = (T == 4) ? 64 : 32 * T;
+  static constexpr int NW = 4;
+  static constexpr int WR = 2;
+  static constexpr int WC = 2;
+  static constexpr int TA = MR / (16 * WR);
+  static constexpr int TB = MC / (16 * WC);
+  static constexpr int NTH = 64 * NW;
+  static constexpr int LDA = MR + PAD;  // LDS row stride of the op(A) panel
+  static constexpr int LDB = MC + PAD;  // LDS row stride of the op(B) panel
+  static constexpr int STAGE = 2 * KP * LDA + 2 * KP * LDB;  // re/im planes of both panels
+  static constexpr int PERA = (MR * KP) / NTH;
+  static constexpr int PERB = (MC * KP) / NTH;
+  static_assert(TA * 16 * WR == MR && TB * 16 * WC == MC, "wave grid must tile the macro tile");
+  static_assert((MR * KP) % NTH == 0 && (MC * KP) % NTH == 0, "panels must split evenly");
+};
+
 // ----------------------------------------------------------- MFMA micro-tile
-// Accumulate nks k-steps (4 complex k each) of a wave's T x T grid of 16x16
+// Accumulate NKS k-steps (4 complex k each) of a wave's TA x TB grid of 16x16
 // complex tiles.  A values at Ar/Ai[k*lda + arow + m], B at Br/Bi[k*ldb + bcol + m].
 // ROWMAJ=0: lane holds C[arow + ti*16 + l%16][bcol + tj*16 + 4r + l/16];
 // ROWMAJ=1: lane holds C[arow + ti*16 + 4r + l/16][bcol + tj*16 + l%16].
-template <int T, int ROWMAJ>
-__device__ __forceinline__ void mma_steps(d4 (&cr)[T][T], d4 (&ci)[T][T],
-                                          const double* Ar, const double* Ai, int lda, int arow,
-                                          const double* Br, const double* Bi, int ldb, int bcol,
-                                          int nks) {
+template <int TA, int TB, int ROWMAJ, int NKS>
+__device__ __forceinline__ void mma_steps(d4 (&cr)[TA][TB], d4 (&ci)[TA][TB],
+                                          const ldsd* Ar, const ldsd* Ai, int lda, int arow,
+                                          const ldsd* Br, const ldsd* Bi, int ldb, int bcol) {
   const int lane = threadIdx.x & 63;
   const int m = lane & 15, kq = lane >> 4;
-  for (int ks = 0; ks < nks; ++ks) {
+  // not unrolled: unrolling lets the scheduler hoist every k-step's fragment
+  // loads, which overflows the 256 registers a wave has at 2 waves per SIMD
+#pragma unroll 1
+  for (int ks = 0; ks < NKS; ++ks) {
     const int k = ks * 4 + kq;
-    double ar[T], ai[T], nai[T], br[T], bi[T];
+    double ar[TA], ai[TA], nai[TA], br[TB], bi[TB];
 #pragma unroll
-    for (int t = 0; t < T; ++t) {
+    for (int t = 0; t < TA; ++t) {
       ar[t] = Ar[k * lda + arow + t * 16 + m];
       ai[t] = Ai[k * lda + arow + t * 16 + m];
       nai[t] = -ai[t];
+    }
+#pragma unroll
+    for (int t = 0; t < TB; ++t) {
       br[t] = Br[k * ldb + bcol + t * 16 + m];
       bi[t] = Bi[k * ldb + bcol + t * 16 + m];
     }
-    if (ROWMAJ) {
 #pragma unroll
-      for (int ti = 0; ti < T; ++ti)
+    for (int ti = 0; ti < TA; ++ti)
 #pragma unroll
-        for (int tj = 0; tj < T; ++tj) cr[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[ti], br[tj], cr[ti][tj], 0, 0, 0);
+      for (int tj = 0; tj < TB; ++tj)
+        cr[ti][tj] = ROWMAJ ? __builtin_amdgcn_mfma_f64_16x16x4f64(ar[ti], br[tj], cr[ti][tj], 0, 0, 0)
+                            : __builtin_amdgcn_mfma_f64_16x16x4f64(br[tj], ar[ti], cr[ti][tj], 0, 0, 0);
 #pragma unroll
-      for (int ti = 0; ti < T; ++ti)
+    for (int ti = 0; ti < TA; ++ti)
 #pragma unroll
-        for (int tj = 0; tj < T; ++tj) ci[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[ti], bi[tj], ci[ti][tj], 0, 0, 0);
+      for (int tj = 0; tj < TB; ++tj)
+        ci[ti][tj] = ROWMAJ ? __builtin_amdgcn_mfma_f64_16x16x4f64(ar[ti], bi[tj], ci[ti][tj], 0, 0, 0)
+                            : __builtin_amdgcn_mfma_f64_16x16x4f64(bi[tj], ar[ti], ci[ti][tj], 0, 0, 0);
 #pragma unroll
-      for (int ti = 0; ti < T; ++ti)
+    for (int ti = 0; ti < TA; ++ti)
 #pragma unroll
-        for (int tj = 0; tj < T; ++tj) cr[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai[ti], bi[tj], cr[ti][tj], 0, 0, 0);
+      for (int tj = 0; tj < TB; ++tj)
+        cr[ti][tj] = ROWMAJ ? __builtin_amdgcn_mfma_f64_16x16x4f64(nai[ti], bi[tj], cr[ti][tj], 0, 0, 0)
+                            : __builtin_amdgcn_mfma_f64_16x16x4f64(bi[tj], nai[ti], cr[ti][tj], 0, 0, 0);
 #pragma unroll
-      for (int ti = 0; ti < T; ++ti)
+    for (int ti = 0; ti < TA; ++ti)
 #pragma unroll
-        for (int tj = 0; tj < T; ++tj) ci[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[ti], br[tj], ci[ti][tj], 0, 0, 0);
-    } else {
-#pragma unroll
-      for (int ti = 0; ti < T; ++ti)
-#pragma unroll
-        for (int tj = 0; tj < T; ++tj) cr[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(br[tj], ar[ti], cr[ti][tj], 0, 0, 0);
-#pragma unroll
-      for (int ti = 0; ti < T; ++ti)
-#pragma unroll
-        for (int tj = 0; tj < T; ++tj) ci[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(bi[tj], ar[ti], ci[ti][tj], 0, 0, 0);
-#pragma unroll
-      for (int ti = 0; ti < T; ++ti)
-#pragma unroll
-        for (int tj = 0; tj < T; ++tj) cr[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(bi[tj], nai[ti], cr[ti][tj], 0, 0, 0);
-#pragma unroll
-      for (int ti = 0; ti < T; ++ti)
-#pragma unroll
-        for (int tj = 0; tj < T; ++tj) ci[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(br[tj], ai[ti], ci[ti][tj], 0, 0, 0);
-    }
+      for (int tj = 0; tj < TB; ++tj)
+        ci[ti][tj] = ROWMAJ ? __builtin_amdgcn_mfma_f64_16x16x4f64(ai[ti], br[tj], ci[ti][tj], 0, 0, 0)
+                            : __builtin_amdgcn_mfma_f64_16x16x4f64(br[tj], ai[ti], ci[ti][tj], 0, 0, 0);
   }
 }
 
 // ------------------------------------------------------------------- GEMM
 // C = op(A) op(B) (+ epilogue), n x n column-major (or row-major C when
-// c_rowmajor), n a multiple of 32*T.  Macro tile (32T)^2 per pass, 4 waves in
-// a 2x2 grid of (16T)^2 wave tiles, K panels of 16 staged in LDS with the next
-// panel prefetched into registers while the current one is multiplied.
-// Returns sum |C_ij| when ep.abs_sum (same value in every thread).
+// c_rowmajor), n a multiple of MR and MC.  K panels of 16 are double-buffered in LDS;
+// the next panel is fetched into registers while the current one is
+// multiplied, then stored to the other stage before the single barrier of the
+// panel.  Returns sum |C_ij| when ep.abs_sum (same value in every thread).
 template <int T>
 struct GemmLds {
-  static constexpr int MT = 32 * T;
-  static constexpr int LD = MT + PAD;
-  static constexpr int DOUBLES = 4 * KP * LD + 8;
+  static constexpr int DOUBLES = 2 * Cfg<T>::STAGE + 8;  // two stages + reduction scratch
 };
 
-template <int T>
-__device__ __forceinline__ void gemm_load_panel(int n, int opA, const double2* __restrict__ A, int opB,
-                                                const double2* __restrict__ B, int row0, int col0, int k0,
-                                                double2 (&ra)[(32 * T * KP) / NT], double2 (&rb)[(32 * T * KP) / NT]) {
-  constexpr int MT = 32 * T;
-  constexpr int PER = (MT * KP) / NT;
+template <int T, int OPA, int OPB>
+__device__ __forceinline__ void gemm_load_panel(int n, const double2* __restrict__ A, const double2* __restrict__ B,
+                                                int row0, int col0, int k0,
+                                                double2 (&ra)[Cfg<T>::PERA], double2 (&rb)[Cfg<T>::PERB]) {
+  constexpr int MR = Cfg<T>::MR, MC = Cfg<T>::MC, NTH = Cfg<T>::NTH;
   const int tid = threadIdx.x;
 #pragma unroll
-  for (int q = 0; q < PER; ++q) {
-    const int idx = q * NT + tid;
-    if (opA == OP_N) {
-      const int i = idx % MT, k = idx / MT;
-      ra[q] = A[(size_t)(row0 + i) + (size_t)(k0 + k) * n];
+  for (int q = 0; q < Cfg<T>::PERA; ++q) {
+    const int idx = q * NTH + tid;
+    if (OPA == OP_N) {
+      const int i = idx % MR, k = idx / MR;
+      ra[q] = ldg(A, (size_t)(row0 + i) + (size_t)(k0 + k) * n);
     } else {
       const int k = idx % KP, i = idx / KP;
-      ra[q] = cconj(A[(size_t)(k0 + k) + (size_t)(row0 + i) * n]);
+      ra[q] = cconj(ldg(A, (size_t)(k0 + k) + (size_t)(row0 + i) * n));
     }
-    if (opB == OP_N) {
+  }
+#pragma unroll
+  for (int q = 0; q < Cfg<T>::PERB; ++q) {
+    const int idx = q * NTH + tid;
+    if (OPB == OP_N) {
       const int k = idx % KP, j = idx / KP;
-      rb[q] = B[(size_t)(k0 + k) + (size_t)(col0 + j) * n];
+      rb[q] = ldg(B, (size_t)(k0 + k) + (size_t)(col0 + j) * n);
     } else {
-      const int j = idx % MT, k = idx / MT;
-      rb[q] = cconj(B[(size_t)(col0 + j) + (size_t)(k0 + k) * n]);
+      const int j = idx % MC, k = idx / MC;
+      rb[q] = cconj(ldg(B, (size_t)(col0 + j) + (size_t)(k0 + k) * n));
     }
   }
 }
 
-template <int T>
-__device__ __forceinline__ void gemm_store_panel(int opA, int opB, double* lds,
-                                                 const double2 (&ra)[(32 * T * KP) / NT],
-                                                 const double2 (&rb)[(32 * T * KP) / NT]) {
-  constexpr int MT = 32 * T;
-  constexpr int LD = MT + PAD;
-  constexpr int PER = (MT * KP) / NT;
-  double* Ar = lds;
-  double* Ai = Ar + KP * LD;
-  double* Br = Ai + KP * LD;
-  double* Bi = Br + KP * LD;
+template <int T, int OPA, int OPB>
+__device__ __forceinline__ void gemm_store_panel(ldsd* st, const double2 (&ra)[Cfg<T>::PERA],
+                                                 const double2 (&rb)[Cfg<T>::PERB]) {
+  constexpr int MR = Cfg<T>::MR, MC = Cfg<T>::MC, LDA = Cfg<T>::LDA, LDB = Cfg<T>::LDB, NTH = Cfg<T>::NTH;
+  ldsd* Ar = st;
+  ldsd* Ai = Ar + KP * LDA;
+  ldsd* Br = Ai + KP * LDA;
+  ldsd* Bi = Br + KP * LDB;
   const int tid = threadIdx.x;
 #pragma unroll
-  for (int q = 0; q < PER; ++q) {
-    const int idx = q * NT + tid;
+  for (int q = 0; q < Cfg<T>::PERA; ++q) {
+    const int idx = q * NTH + tid;
     int i, k;
-    if (opA == OP_N) { i = idx % MT; k = idx / MT; } else { k = idx % KP; i = idx / KP; }
-    Ar[k * LD + i] = ra[q].x;
-    Ai[k * LD + i] = ra[q].y;
-    int j, kb;
-    if (opB == OP_N) { kb = idx % KP; j = idx / KP; } else { j = idx % MT; kb = idx / MT; }
-    Br[kb * LD + j] = rb[q].x;
-    Bi[kb * LD + j] = rb[q].y;
+    if (OPA == OP_N) { i = idx % MR; k = idx / MR; } else { k = idx % KP; i = idx / KP; }
+    Ar[k * LDA + i] = ra[q].x;
+    Ai[k * LDA + i] = ra[q].y;
+  }
+#pragma unroll
+  for (int q = 0; q < Cfg<T>::PERB; ++q) {
+    const int idx = q * NTH + tid;
+    int j, k;
+    if (OPB == OP_N) { k = idx % KP; j = idx / KP; } else { j = idx % MC; k = idx / MC; }
+    Br[k * LDB + j] = rb[q].x;
+    Bi[k * LDB + j] = rb[q].y;
   }
 }
 
-template <int T>
-__device__ __noinline__ double wg_gemm(int n, int opA, const double2* __restrict__ A, int opB,
-                                       const double2* __restrict__ B, void* Cout, int c_rowmajor,
-                                       Epi ep, double* lds) {
-  constexpr int MT = 32 * T;
-  constexpr int LD = MT + PAD;
-  constexpr int PER = (MT * KP) / NT;
-  static_assert((MT * KP) % NT == 0, "panel must split evenly over the workgroup");
-  double* Ar = lds;
-  double* Ai = Ar + KP * LD;
-  double* Br = Ai + KP * LD;
-  double* Bi = Br + KP * LD;
-  double* red = Bi + KP * LD;
+template <int T, int OPA, int OPB>
+__device__ __noinline__ double wg_gemm_k(int n_, const double2* __restrict__ A_, const double2* __restrict__ B_,
+                                         void* Cout_, int c_rowmajor_, Epi ep_, ldsd* lds) {
+  using C = Cfg<T>;
+  constexpr int MR = C::MR, MC = C::MC, LDA = C::LDA, LDB = C::LDB, STAGE = C::STAGE, TA = C::TA, TB = C::TB;
+  const int n = uni(n_);
+  const double2* A = uni(A_);
+  const double2* B = uni(B_);
+  void* Cout = uni(Cout_);
+  const int c_rowmajor = uni(c_rowmajor_);
+  Epi ep;
+  ep.diag = uni(ep_.diag); ep.sacc = uni(ep_.sacc);
+  ep.D1 = uni(ep_.D1); ep.s1 = uni(ep_.s1);
+  ep.D2 = uni(ep_.D2); ep.s2 = uni(ep_.s2);
+  ep.D3 = uni(ep_.D3); ep.s3 = uni(ep_.s3);
+  ep.real_out = uni(ep_.real_out); ep.abs_sum = uni(ep_.abs_sum);
+  ldsd* red = lds + 2 * STAGE;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wr = wave & 1, wc = wave >> 1;
-  const int nt = n / MT, npan = n / KP;
+  const int arow = (wave % C::WR) * 16 * TA, bcol = (wave / C::WR) * 16 * TB;
+  const int ntr = n / MR, ntc = n / MC, npan = n / KP;
   double asum = 0.0;
-  for (int tile = 0; tile < nt * nt; ++tile) {
-    const int row0 = (tile % nt) * MT, col0 = (tile / nt) * MT;
-    d4 cr[T][T], ci[T][T];
+  for (int tile = 0; tile < ntr * ntc; ++tile) {
+    const int row0 = (tile % ntr) * MR, col0 = (tile / ntr) * MC;
+    d4 cr[TA][TB], ci[TA][TB];
 #pragma unroll
-    for (int a = 0; a < T; ++a)
+    for (int a = 0; a < TA; ++a)
 #pragma unroll
-      for (int b = 0; b < T; ++b) { cr[a][b] = d4{0, 0, 0, 0}; ci[a][b] = d4{0, 0, 0, 0}; }
-    double2 ra[PER], rb[PER];
-    gemm_load_panel<T>(n, opA, A, opB, B, row0, col0, 0, ra, rb);
-    gemm_store_panel<T>(opA, opB, lds, ra, rb);
+      for (int b = 0; b < TB; ++b) { cr[a][b] = d4{0, 0, 0, 0}; ci[a][b] = d4{0, 0, 0, 0}; }
+    double2 ra[C::PERA], rb[C::PERB];
+    gemm_load_panel<T, OPA, OPB>(n, A, B, row0, col0, 0, ra, rb);
+    gemm_store_panel<T, OPA, OPB>(lds, ra, rb);
     __syncthreads();
     for (int p = 0; p < npan; ++p) {
-      const bool more = (p + 1 < npan);
-      if (more) gemm_load_panel<T>(n, opA, A, opB, B, row0, col0, (p + 1) * KP, ra, rb);
+      // branch-free pipeline: the last pass refetches its own panel into the idle stage
+      const int kn = (p + 1 < npan ? p + 1 : p) * KP;
+      gemm_load_panel<T, OPA, OPB>(n, A, B, row0, col0, kn, ra, rb);
+      const ldsd* st = lds + (p & 1) * STAGE;
+      const ldsd* sAr = st;
+      const ldsd* sAi = sAr + KP * LDA;
+      const ldsd* sBr = sAi + KP * LDA;
+      const ldsd* sBi = sBr + KP * LDB;
       if (c_rowmajor)
-        mma_steps<T, 1>(cr, ci, Ar, Ai, LD, wr * 16 * T, Br, Bi, LD, wc * 16 * T, KP / 4);
+        mma_steps<TA, TB, 1, KP / 4>(cr, ci, sAr, sAi, LDA, arow, sBr, sBi, LDB, bcol);
       else
-        mma_steps<T, 0>(cr, ci, Ar, Ai, LD, wr * 16 * T, Br, Bi, LD, wc * 16 * T, KP / 4);
+        mma_steps<TA, TB, 0, KP / 4>(cr, ci, sAr, sAi, LDA, arow, sBr, sBi, LDB, bcol);
+      gemm_store_panel<T, OPA, OPB>(lds + ((p + 1) & 1) * STAGE, ra, rb);
       __syncthreads();
-      if (more) {
-        gemm_store_panel<T>(opA, opB, lds, ra, rb);
-        __syncthreads();
-      }
     }
     // epilogue
     const int m = lane & 15, kq = lane >> 4;
 #pragma unroll
-    for (int ti = 0; ti < T; ++ti)
+    for (int ti = 0; ti < TA; ++ti)
 #pragma unroll
-      for (int tj = 0; tj < T; ++tj)
+      for (int tj = 0; tj < TB; ++tj)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           int i, j;
           if (c_rowmajor) {
-            i = row0 + wr * 16 * T + ti * 16 + 4 * r + kq;
-            j = col0 + wc * 16 * T + tj * 16 + m;
+            i = row0 + arow + ti * 16 + 4 * r + kq;
+            j = col0 + bcol + tj * 16 + m;
           } else {
-            i = row0 + wr * 16 * T + ti * 16 + m;
-            j = col0 + wc * 16 * T + tj * 16 + 4 * r + kq;
+            i = row0 + arow + ti * 16 + m;
+            j = col0 + bcol + tj * 16 + 4 * r + kq;
           }
           double2 v = epi_value(ep, n, i, j, cr[ti][tj][r], ci[ti][tj][r]);
           if (ep.abs_sum) asum += sqrt(v.y * v.y + v.x * v.x);
           size_t off = c_rowmajor ? (size_t)i * n + j : (size_t)i + (size_t)j * n;
           if (ep.real_out)
-            ((double*)Cout)[off] = v.x;
+            ((gd*)Cout)[off] = v.x;
           else
-            ((double2*)Cout)[off] = v;
+            ((gd2*)Cout)[off] = v;
         }
   }
   double s = 0.0;
   if (ep.abs_sum) s = block_sum(asum, red);
   __syncthreads();
   return s;
+}
+
+template <int T>
+__device__ __forceinline__ double wg_gemm(int n, int opA, const double2* A, int opB, const double2* B, void* Cout,
+                                          int c_rowmajor, Epi ep, ldsd* lds) {
+  if (opA == OP_N) {
+    if (opB == OP_N) return wg_gemm_k<T, OP_N, OP_N>(n, A, B, Cout, c_rowmajor, ep, lds);
+    return wg_gemm_k<T, OP_N, OP_C>(n, A, B, Cout, c_rowmajor, ep, lds);
+  }
+  if (opB == OP_N) return wg_gemm_k<T, OP_C, OP_N>(n, A, B, Cout, c_rowmajor, ep, lds);
+  return wg_gemm_k<T, OP_C, OP_C>(n, A, B, Cout, c_rowmajor, ep, lds);
 }
 
 // --------------------------------------------------------------- elementwise
@@ -316,34 +409,34 @@ __device__ __noinline__ void wg_ew(int n, double2* out, int out_rowmajor, double
                                    const double2* D1, double2 s1, const double2* D2, double2 s2,
                                    const double2* D3, double2 s3) {
   const int nn = n * n;
-  for (int idx = threadIdx.x; idx < nn; idx += NT) {
+  for (int idx = threadIdx.x; idx < nn; idx += blockDim.x) {
     int i, j;
     if (out_rowmajor) { j = idx % n; i = idx / n; } else { i = idx % n; j = idx / n; }
     double2 v = (i == j) ? diag : make_double2(0, 0);
-    if (D1) v = cadd(v, cmul(s1, D1[i + (size_t)j * n]));
-    if (D2) v = cadd(v, cmul(s2, D2[i + (size_t)j * n]));
-    if (D3) v = cadd(v, cmul(s3, cconj(D3[j + (size_t)i * n])));
-    out[out_rowmajor ? (size_t)i * n + j : (size_t)i + (size_t)j * n] = v;
+    if (D1) v = cadd(v, cmul(s1, gc(D1)[i + (size_t)j * n]));
+    if (D2) v = cadd(v, cmul(s2, gc(D2)[i + (size_t)j * n]));
+    if (D3) v = cadd(v, cmul(s3, cconj(gc(D3)[j + (size_t)i * n])));
+    gw(out)[out_rowmajor ? (size_t)i * n + j : (size_t)i + (size_t)j * n] = v;
   }
   __syncthreads();
 }
 
 // Re sum_ij X(i,j) Y(j,i) = Re Tr(X Y).
-__device__ __noinline__ double wg_retrace_prod(int n, const double2* X, const double2* Y, double* red) {
+__device__ __noinline__ double wg_retrace_prod(int n, const double2* X, const double2* Y, ldsd* red) {
   double s = 0.0;
   const int nn = n * n;
-  for (int idx = threadIdx.x; idx < nn; idx += NT) {
+  for (int idx = threadIdx.x; idx < nn; idx += blockDim.x) {
     const int i = idx % n, j = idx / n;
-    double2 x = X[i + (size_t)j * n], y = Y[j + (size_t)i * n];
+    double2 x = gc(X)[i + (size_t)j * n], y = gc(Y)[j + (size_t)i * n];
     s += x.x * y.x - x.y * y.y;
   }
   return block_sum(s, red);
 }
 
 // Im Tr(X).
-__device__ __forceinline__ double wg_imtrace(int n, const double2* X, double* red) {
+__device__ __forceinline__ double wg_imtrace(int n, const double2* X, ldsd* red) {
   double s = 0.0;
-  for (int i = threadIdx.x; i < n; i += NT) s += X[i + (size_t)i * n].y;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += gc(X)[i + (size_t)i * n].y;
   return block_sum(s, red);
 }
 
@@ -365,24 +458,27 @@ struct InvLds {
 };
 
 template <int T, int NMAX>
-__device__ __noinline__ int wg_inverse(int n, double2* W, double2* out, double* lds) {
+__device__ __noinline__ int wg_inverse(int n, double2* W_, double2* out_, ldsd* lds) {
+  using C = Cfg<T>;
   constexpr int LDI = NMAX + PAD;
-  constexpr int MT = 32 * T;
-  double* Mr = lds;
-  double* Mi = Mr + KP * LDI;
-  double* Br = Mi + KP * LDI;
-  double* Bi = Br + KP * LDI;
-  int* piv = (int*)(Bi + KP * LDI);
-  int* used = piv + NMAX;
-  double* redv = (double*)(used + NMAX);
-  int* redi = (int*)(redv + 4);
+  constexpr int MR = C::MR, MC = C::MC, NTH = C::NTH, TA = C::TA, TB = C::TB;
+  gd2* W = gw(W_);
+  gd2* out = gw(out_);
+  ldsd* Mr = lds;
+  ldsd* Mi = Mr + KP * LDI;
+  ldsd* Br = Mi + KP * LDI;
+  ldsd* Bi = Br + KP * LDI;
+  ldsi* piv = (ldsi*)(Bi + KP * LDI);
+  ldsi* used = piv + NMAX;
+  ldsd* redv = (ldsd*)(used + NMAX);
+  ldsi* redi = (ldsi*)(redv + 4);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wr = wave & 1, wc = wave >> 1;
-  for (int i = tid; i < n; i += NT) used[i] = 0;
+  const int arow = (wave % C::WR) * 16 * TA, bcol = (wave / C::WR) * 16 * TB;
+  for (int i = tid; i < n; i += NTH) used[i] = 0;
   __syncthreads();
   int info = 0;
   for (int k0 = 0; k0 < n && !info; k0 += KP) {
-    for (int idx = tid; idx < n * KP; idx += NT) {
+    for (int idx = tid; idx < n * KP; idx += NTH) {
       const int c = idx % KP, i = idx / KP;
       const double2 v = W[(size_t)i * n + k0 + c];
       Mr[c * LDI + i] = v.x;
@@ -392,7 +488,7 @@ __device__ __noinline__ int wg_inverse(int n, double2* W, double2* out, double* 
     for (int c = 0; c < KP; ++c) {
       double best = -1.0;
       int bi = 0x7fffffff;
-      for (int i = tid; i < n; i += NT) {
+      for (int i = tid; i < n; i += NTH) {
         if (!used[i]) {
           const double v = fabs(Mr[c * LDI + i]) + fabs(Mi[c * LDI + i]);
           if (v > best) { best = v; bi = i; }
@@ -409,7 +505,7 @@ __device__ __noinline__ int wg_inverse(int n, double2* W, double2* out, double* 
         rp[c2] = (c2 == c) ? ainv : cmul(make_double2(Mr[c2 * LDI + p], Mi[c2 * LDI + p]), ainv);
       __syncthreads();
       if (tid == 0) { used[p] = 1; piv[k0 + c] = p; }
-      for (int i = tid; i < n; i += NT) {
+      for (int i = tid; i < n; i += NTH) {
         if (i == p) {
 #pragma unroll
           for (int c2 = 0; c2 < KP; ++c2) { Mr[c2 * LDI + i] = rp[c2].x; Mi[c2 * LDI + i] = rp[c2].y; }
@@ -428,7 +524,7 @@ __device__ __noinline__ int wg_inverse(int n, double2* W, double2* out, double* 
     }
     if (info) break;
     // snapshot of the pivot rows (all columns) before they are updated
-    for (int idx = tid; idx < KP * n; idx += NT) {
+    for (int idx = tid; idx < KP * n; idx += NTH) {
       const int c = idx / n, j = idx % n;
       const double2 v = W[(size_t)piv[k0 + c] * n + j];
       Br[c * LDI + j] = v.x;
@@ -437,25 +533,25 @@ __device__ __noinline__ int wg_inverse(int n, double2* W, double2* out, double* 
     __syncthreads();
     if (tid < KP) Mr[tid * LDI + piv[k0 + tid]] -= 1.0;  // M = F - E
     __syncthreads();
-    const int nt = n / MT;
+    const int ntr = n / MR, ntc = n / MC;
     const int m = lane & 15, kq = lane >> 4;
-    for (int tile = 0; tile < nt * nt; ++tile) {
-      const int row0 = (tile % nt) * MT, col0 = (tile / nt) * MT;
-      d4 cr[T][T], ci[T][T];
+    for (int tile = 0; tile < ntr * ntc; ++tile) {
+      const int row0 = (tile % ntr) * MR, col0 = (tile / ntr) * MC;
+      d4 cr[TA][TB], ci[TA][TB];
 #pragma unroll
-      for (int a = 0; a < T; ++a)
+      for (int a = 0; a < TA; ++a)
 #pragma unroll
-        for (int b = 0; b < T; ++b) { cr[a][b] = d4{0, 0, 0, 0}; ci[a][b] = d4{0, 0, 0, 0}; }
-      mma_steps<T, 1>(cr, ci, Mr, Mi, LDI, row0 + wr * 16 * T, Br, Bi, LDI, col0 + wc * 16 * T, KP / 4);
+        for (int b = 0; b < TB; ++b) { cr[a][b] = d4{0, 0, 0, 0}; ci[a][b] = d4{0, 0, 0, 0}; }
+      mma_steps<TA, TB, 1, KP / 4>(cr, ci, Mr, Mi, LDI, row0 + arow, Br, Bi, LDI, col0 + bcol);
 #pragma unroll
-      for (int ti = 0; ti < T; ++ti)
+      for (int ti = 0; ti < TA; ++ti)
 #pragma unroll
-        for (int tj = 0; tj < T; ++tj)
+        for (int tj = 0; tj < TB; ++tj)
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            const int i = row0 + wr * 16 * T + ti * 16 + 4 * r + kq;
-            const int j = col0 + wc * 16 * T + tj * 16 + m;
-            double2* w = &W[(size_t)i * n + j];
+            const int i = row0 + arow + ti * 16 + 4 * r + kq;
+            const int j = col0 + bcol + tj * 16 + m;
+            gd2* w = &W[(size_t)i * n + j];
             if (j >= k0 && j < k0 + KP) {
               const int c = j - k0;
               double2 v = make_double2(Mr[c * LDI + i], Mi[c * LDI + i]);
@@ -470,14 +566,14 @@ __device__ __noinline__ int wg_inverse(int n, double2* W, double2* out, double* 
     __syncthreads();
   }
   if (info) {
-    for (int idx = tid; idx < n * n; idx += NT) out[idx] = make_double2(0, 0);
+    for (int idx = tid; idx < n * n; idx += NTH) out[idx] = make_double2(0, 0);
     __syncthreads();
     return info;
   }
   // inverse permutation into `used`
-  for (int q = tid; q < n; q += NT) used[piv[q]] = q;
+  for (int q = tid; q < n; q += NTH) used[piv[q]] = q;
   __syncthreads();
-  for (int idx = tid; idx < n * n; idx += NT) {
+  for (int idx = tid; idx < n * n; idx += NTH) {
     const int i = idx % n, q = idx / n;
     out[idx] = W[(size_t)piv[i] * n + used[q]];
   }

@@ -97,7 +97,7 @@ typedef void (*SweepLaunch)(const rgf::SweepArgs&, int grid, hipStream_t);
 
 template <int T, int NMAX>
 void launch_sweep(const rgf::SweepArgs& a, int grid, hipStream_t s) {
-  hipLaunchKernelGGL((rgf::rgf_sweep<T, NMAX>), dim3(grid), dim3(rgf::NT), 0, s, a);
+  hipLaunchKernelGGL((rgf::rgf_sweep<T, NMAX>), dim3(grid), dim3(rgf::Cfg<T>::NTH), 0, s, a);
 }
 
 template <int T, int NMAX>
@@ -108,7 +108,7 @@ const void* sweep_fn() {
 struct Variant {
   SweepLaunch launch;
   const void* fn;
-  int T, nmax;
+  int T, nmax, nth;
 };
 
 // Tile variant per block size: wave tile (16T)^2, macro tile (32T)^2; the
@@ -116,13 +116,13 @@ struct Variant {
 bool pick_variant(int n, Variant* v) {
   if (n % 32 != 0 || n < 32 || n > 256) return false;
   switch (n) {
-    case 32: *v = {launch_sweep<1, 32>, sweep_fn<1, 32>(), 1, 32}; return true;
-    case 64: *v = {launch_sweep<2, 64>, sweep_fn<2, 64>(), 2, 64}; return true;
-    case 96: *v = {launch_sweep<3, 96>, sweep_fn<3, 96>(), 3, 96}; return true;
-    case 128: *v = {launch_sweep<4, 128>, sweep_fn<4, 128>(), 4, 128}; return true;
-    case 192: *v = {launch_sweep<3, 192>, sweep_fn<3, 192>(), 3, 192}; return true;
-    case 256: *v = {launch_sweep<4, 256>, sweep_fn<4, 256>(), 4, 256}; return true;
-    default: *v = {launch_sweep<1, 256>, sweep_fn<1, 256>(), 1, 256}; return true;
+    case 32: *v = {launch_sweep<1, 32>, sweep_fn<1, 32>(), 1, 32, rgf::Cfg<1>::NTH}; return true;
+    case 64: *v = {launch_sweep<2, 64>, sweep_fn<2, 64>(), 2, 64, rgf::Cfg<2>::NTH}; return true;
+    case 96: *v = {launch_sweep<3, 96>, sweep_fn<3, 96>(), 3, 96, rgf::Cfg<3>::NTH}; return true;
+    case 128: *v = {launch_sweep<4, 128>, sweep_fn<4, 128>(), 4, 128, rgf::Cfg<4>::NTH}; return true;
+    case 192: *v = {launch_sweep<3, 192>, sweep_fn<3, 192>(), 3, 192, rgf::Cfg<3>::NTH}; return true;
+    case 256: *v = {launch_sweep<4, 256>, sweep_fn<4, 256>(), 4, 256, rgf::Cfg<4>::NTH}; return true;
+    default: *v = {launch_sweep<1, 256>, sweep_fn<1, 256>(), 1, 256, rgf::Cfg<1>::NTH}; return true;
   }
 }
 
@@ -173,7 +173,7 @@ int rgf_solve_batched(const rgf_params* p, const rgf_batch* b, void* stream) {
   hipDeviceProp_t prop;
   HIPCHK(hipGetDeviceProperties(&prop, dev));
   int per_cu = 0;
-  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, v.fn, rgf::NT, 0));
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, v.fn, v.nth, 0));
   if (per_cu < 1) return fail(RGF_EHIP, "sweep kernel does not fit on a CU");
   const long long slot_bytes = slot_elems(b->nm, b->nx) * 16LL;
   long long slots = std::min<long long>(b->nE, (long long)prop.multiProcessorCount * per_cu);
@@ -416,10 +416,10 @@ int rgf_zgemm_batched(int n, int nbatch, int opA, const double* A, int opB, cons
   const double2* b = reinterpret_cast<const double2*>(B);
   double2* c = reinterpret_cast<double2*>(C);
   switch (v.T) {
-    case 1: hipLaunchKernelGGL(rgf::k_gemm_batched<1>, dim3(grid), dim3(rgf::NT), 0, s, n, nbatch, opA, a, opB, b, c, c_rowmajor); break;
-    case 2: hipLaunchKernelGGL(rgf::k_gemm_batched<2>, dim3(grid), dim3(rgf::NT), 0, s, n, nbatch, opA, a, opB, b, c, c_rowmajor); break;
-    case 3: hipLaunchKernelGGL(rgf::k_gemm_batched<3>, dim3(grid), dim3(rgf::NT), 0, s, n, nbatch, opA, a, opB, b, c, c_rowmajor); break;
-    default: hipLaunchKernelGGL(rgf::k_gemm_batched<4>, dim3(grid), dim3(rgf::NT), 0, s, n, nbatch, opA, a, opB, b, c, c_rowmajor); break;
+    case 1: hipLaunchKernelGGL(rgf::k_gemm_batched<1>, dim3(grid), dim3(rgf::Cfg<1>::NTH), 0, s, n, nbatch, opA, a, opB, b, c, c_rowmajor); break;
+    case 2: hipLaunchKernelGGL(rgf::k_gemm_batched<2>, dim3(grid), dim3(rgf::Cfg<2>::NTH), 0, s, n, nbatch, opA, a, opB, b, c, c_rowmajor); break;
+    case 3: hipLaunchKernelGGL(rgf::k_gemm_batched<3>, dim3(grid), dim3(rgf::Cfg<3>::NTH), 0, s, n, nbatch, opA, a, opB, b, c, c_rowmajor); break;
+    default: hipLaunchKernelGGL(rgf::k_gemm_batched<4>, dim3(grid), dim3(rgf::Cfg<4>::NTH), 0, s, n, nbatch, opA, a, opB, b, c, c_rowmajor); break;
   }
   HIPCHK(hipGetLastError());
   return 0;
@@ -443,13 +443,13 @@ int rgf_zinv_batched(int n, int nbatch, const double* A, double* Ainv, int* info
   double2* o = reinterpret_cast<double2*>(Ainv);
   double2* w = reinterpret_cast<double2*>(g_ws[dev].p);
   switch (n) {
-    case 32: hipLaunchKernelGGL((rgf::k_inv_batched<1, 32>), dim3(grid), dim3(rgf::NT), 0, s, n, nbatch, a, o, w, info); break;
-    case 64: hipLaunchKernelGGL((rgf::k_inv_batched<2, 64>), dim3(grid), dim3(rgf::NT), 0, s, n, nbatch, a, o, w, info); break;
-    case 96: hipLaunchKernelGGL((rgf::k_inv_batched<3, 96>), dim3(grid), dim3(rgf::NT), 0, s, n, nbatch, a, o, w, info); break;
-    case 128: hipLaunchKernelGGL((rgf::k_inv_batched<4, 128>), dim3(grid), dim3(rgf::NT), 0, s, n, nbatch, a, o, w, info); break;
-    case 192: hipLaunchKernelGGL((rgf::k_inv_batched<3, 192>), dim3(grid), dim3(rgf::NT), 0, s, n, nbatch, a, o, w, info); break;
-    case 256: hipLaunchKernelGGL((rgf::k_inv_batched<4, 256>), dim3(grid), dim3(rgf::NT), 0, s, n, nbatch, a, o, w, info); break;
-    default: hipLaunchKernelGGL((rgf::k_inv_batched<1, 256>), dim3(grid), dim3(rgf::NT), 0, s, n, nbatch, a, o, w, info); break;
+    case 32: hipLaunchKernelGGL((rgf::k_inv_batched<1, 32>), dim3(grid), dim3(rgf::Cfg<1>::NTH), 0, s, n, nbatch, a, o, w, info); break;
+    case 64: hipLaunchKernelGGL((rgf::k_inv_batched<2, 64>), dim3(grid), dim3(rgf::Cfg<2>::NTH), 0, s, n, nbatch, a, o, w, info); break;
+    case 96: hipLaunchKernelGGL((rgf::k_inv_batched<3, 96>), dim3(grid), dim3(rgf::Cfg<3>::NTH), 0, s, n, nbatch, a, o, w, info); break;
+    case 128: hipLaunchKernelGGL((rgf::k_inv_batched<4, 128>), dim3(grid), dim3(rgf::Cfg<4>::NTH), 0, s, n, nbatch, a, o, w, info); break;
+    case 192: hipLaunchKernelGGL((rgf::k_inv_batched<3, 192>), dim3(grid), dim3(rgf::Cfg<3>::NTH), 0, s, n, nbatch, a, o, w, info); break;
+    case 256: hipLaunchKernelGGL((rgf::k_inv_batched<4, 256>), dim3(grid), dim3(rgf::Cfg<4>::NTH), 0, s, n, nbatch, a, o, w, info); break;
+    default: hipLaunchKernelGGL((rgf::k_inv_batched<1, 256>), dim3(grid), dim3(rgf::Cfg<1>::NTH), 0, s, n, nbatch, a, o, w, info); break;
   }
   HIPCHK(hipGetLastError());
   return 0;

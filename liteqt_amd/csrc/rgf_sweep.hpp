@@ -52,8 +52,8 @@ __device__ __forceinline__ double ferm_dev(double a) { return 1.0 / (1.0 + exp(a
 // function G00 = (z - H_SS)^-1 and returns the iterations run.
 template <int T, int NMAX>
 __device__ int wg_sancho(int n, double E, double eta, double tol, int nmax, const double2* H00,
-                         const double2* Hc, int left, double2* G00, double2* tb, double* lds,
-                         double* red, int& hit, int& fails) {
+                         const double2* Hc, int left, double2* G00, double2* tb, ldsd* lds,
+                         ldsd* red, int& hit, int& fails) {
   const size_t nn = (size_t)n * n;
   const double2 one = make_double2(1, 0), mone = make_double2(-1, 0), zero = make_double2(0, 0);
   const double2 zl = make_double2(E, eta);
@@ -111,7 +111,7 @@ __device__ int wg_sancho(int n, double E, double eta, double tol, int nmax, cons
 }
 
 template <int T, int NMAX>
-__device__ void solve_energy(const SweepArgs& a, int e, double2* ws, double* lds, double* red) {
+__device__ void solve_energy(const SweepArgs& a, int e, double2* ws, ldsd* lds, ldsd* red) {
   const int n = a.n, nx = a.nx;
   const size_t nn = (size_t)n * n;
   const bool phys = (a.mode == 1);
@@ -305,28 +305,28 @@ struct SweepLds {
 };
 
 template <int T, int NMAX>
-__global__ __launch_bounds__(NT, 1) void rgf_sweep(SweepArgs a) {
+__global__ __launch_bounds__(Cfg<T>::NTH, 1) void rgf_sweep(SweepArgs a) {
   __shared__ double lds[SweepLds<T, NMAX>::DOUBLES];
-  __shared__ double red[8];
+  __shared__ double red[16];
   double2* ws = a.ws + (size_t)blockIdx.x * (size_t)a.ws_slot;
-  for (int e = blockIdx.x; e < a.nE; e += gridDim.x) solve_energy<T, NMAX>(a, e, ws, lds, red);
+  for (int e = blockIdx.x; e < a.nE; e += gridDim.x) solve_energy<T, NMAX>(a, e, ws, (ldsd*)lds, (ldsd*)red);
 }
 
 // Batched building blocks: one workgroup per matrix (used by the block-level
 // entry points rgf_zgemm_batched / rgf_zinv_batched).
 template <int T>
-__global__ __launch_bounds__(NT, 1) void k_gemm_batched(int n, int nb, int opA, const double2* A, int opB,
+__global__ __launch_bounds__(Cfg<T>::NTH, 1) void k_gemm_batched(int n, int nb, int opA, const double2* A, int opB,
                                                         const double2* B, double2* C, int c_rowmajor) {
   __shared__ double lds[GemmLds<T>::DOUBLES];
   const size_t nn = (size_t)n * n;
   for (int b = blockIdx.x; b < nb; b += gridDim.x)
-    wg_gemm<T>(n, opA, A + b * nn, opB, B + b * nn, C + b * nn, c_rowmajor, epi_store(), lds);
+    wg_gemm<T>(n, opA, A + b * nn, opB, B + b * nn, C + b * nn, c_rowmajor, epi_store(), (ldsd*)lds);
 }
 
 // Inverse of column-major matrices: transposed copy into the row-major work
 // buffer W, then the Gauss-Jordan sweep.
 template <int T, int NMAX>
-__global__ __launch_bounds__(NT, 1) void k_inv_batched(int n, int nb, const double2* A, double2* Ainv, double2* W,
+__global__ __launch_bounds__(Cfg<T>::NTH, 1) void k_inv_batched(int n, int nb, const double2* A, double2* Ainv, double2* W,
                                                        int* info) {
   __shared__ double lds[SweepLds<T, NMAX>::DOUBLES];
   const size_t nn = (size_t)n * n;
@@ -334,7 +334,7 @@ __global__ __launch_bounds__(NT, 1) void k_inv_batched(int n, int nb, const doub
   for (int b = blockIdx.x; b < nb; b += gridDim.x) {
     double2* w = W + (size_t)blockIdx.x * nn;
     wg_ew(n, w, 1, zero, A + b * nn, one, nullptr, one, nullptr, one);
-    const int r = wg_inverse<T, NMAX>(n, w, Ainv + b * nn, lds);
+    const int r = wg_inverse<T, NMAX>(n, w, Ainv + b * nn, (ldsd*)lds);
     if (info && threadIdx.x == 0) info[b] = r;
   }
 }
