@@ -45,6 +45,18 @@ def build_inputs(cfg):
     return Hii, H1i, E
 
 
+def kernel_source_hash():
+    """Hash of the HIP engine sources: a PMC traffic figure is reported only for the
+    kernel it was measured on."""
+    import hashlib
+    h = hashlib.sha256()
+    csrc = os.path.join(ROOT, "liteqt_amd", "csrc")
+    for f in sorted(os.listdir(csrc)):
+        if f.endswith((".hpp", ".hip")) and f != "rgf_plain.hip":
+            h.update(open(os.path.join(csrc, f), "rb").read())
+    return h.hexdigest()[:16]
+
+
 def algorithmic_flops(nm, nx, iters):
     """Flops of the step's output set per energy: 8 nm^3 [17 nx + 7 (kL + kR)]
     (forward 6 GEMM + 1 inverse, backward 10 GEMM per slab; 1 inverse + 6 GEMM
@@ -87,6 +99,8 @@ def main():
     ap.add_argument("--energies", type=int, default=0, help="override the number of energies")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--plain-energies", type=int, default=4,
+                    help="energies timed on the plain per-energy GPU path (0: skip)")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -160,11 +174,29 @@ def main():
     tr = out["tr"].cpu().numpy()
     ok = bool(np.all(np.isfinite(tr)) and np.all(info[:, 3] == 0))
 
+    plain = None
+    if args.plain_energies > 0:
+        npl = min(args.plain_energies, nE)
+        outp = {"tr": torch.zeros(npl, dtype=torch.float64, device=dev),
+                "tre": torch.zeros(npl, dtype=torch.float64, device=dev),
+                "info": torch.zeros(npl, 4, dtype=torch.int32, device=dev)}
+        rgf.solve_batched(p, Ed[:1], temp, mu, Hd, Cd, out={"tr": outp["tr"][:1]}, stream=stream, plain=True)
+        torch.cuda.synchronize()
+        t0p = time.perf_counter()
+        rgf.solve_batched(p, Ed[:npl], temp, mu, Hd, Cd, out=outp, stream=stream, plain=True)
+        torch.cuda.synchronize()
+        tp = time.perf_counter() - t0p
+        print(f"bench: plain path {npl} energies {tp:.2f}s", file=sys.stderr, flush=True)
+        dtr = float(np.max(np.abs(outp["tr"].cpu().numpy() - tr[:npl])))
+        plain = {"value": npl / tp, "unit": "energy-points/s", "energies": npl,
+                 "kind": "one energy per launch chain: rocBLAS zgemm + rocSOLVER zgetrf/zgetri in the "
+                         "reference's call order, host-checked decimation",
+                 "max_abs_tr_diff_vs_batched": dtr}
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tfile):
         tinfo = json.load(open(tfile)).get(args.config)
-        if tinfo:
+        if tinfo and tinfo.get("kernel_hash") == kernel_source_hash():
             traffic = tinfo["bytes_per_energy"] * nE  # HBM bytes per launch, from the PMC profile
     res = {
         "metric": "batched RGF energy-pts/s (complex FP64, err<=1e-10 vs CPU oracle)",
@@ -185,6 +217,8 @@ def main():
                      "frac": achieved / PEAK_FP64_MFMA, "traffic": traffic,
                      "kernel": f"rgf_sweep (nm={nm})", "kernel_ms": kern_ms,
                      "flops_per_launch": flops},
+        "plain_gpu": plain,
+        "batched_over_plain": (value / plain["value"]) if plain else None,
         "decimation_iters_mean": float(info[:, :2].mean()),
         "sane": ok,
     }

@@ -19,6 +19,7 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "librgf_hip.so")
+PLAIN_LIB_PATH = os.path.join(_HERE, "librgf_plain.so")
 
 MODE_COMPAT = 0
 MODE_PHYSICAL = 1
@@ -177,8 +178,26 @@ def _tp(t):
     return None if t is None else t.data_ptr()
 
 
-def solve_batched(p, E, temp, mu, Hii, H1i, sigL=None, sigR=None, out=None, max_slots=0, stream=None):
-    """Device-resident batched solve.  E, Hii, H1i, sig*, out[...] are torch
+_plain = None
+
+
+def plain_lib():
+    """The plain per-energy comparison path (rocBLAS/rocSOLVER call chain)."""
+    global _plain
+    if _plain is None:
+        if not os.path.exists(PLAIN_LIB_PATH):
+            raise RgfError(f"plain-path library missing: {PLAIN_LIB_PATH} (run __graft_entry__.build())")
+        L = ctypes.CDLL(PLAIN_LIB_PATH)
+        L.rgf_plain_solve.argtypes = [ctypes.POINTER(Params), ctypes.POINTER(Batch), ctypes.c_void_p]
+        L.rgf_plain_last_error.restype = ctypes.c_char_p
+        _plain = L
+    return _plain
+
+
+def solve_batched(p, E, temp, mu, Hii, H1i, sigL=None, sigR=None, out=None, max_slots=0, stream=None,
+                  plain=False):
+    """Device-resident batched solve (plain=True: the per-energy rocBLAS/rocSOLVER
+    comparison path with the same arguments).  E, Hii, H1i, sig*, out[...] are torch
     tensors on the HIP device: complex blocks as complex128 tensors of shape
     (nE, nx, nm, nm) holding column-major blocks (i.e. the memory of a Fortran
     (nm,nm,nx,nE) array; use to_device_blocks()).  out maps names in
@@ -205,6 +224,11 @@ def solve_batched(p, E, temp, mu, Hii, H1i, sigL=None, sigR=None, out=None, max_
         setattr(b, k, _tp(out.get(k)))
     if stream is None:
         stream = torch.cuda.current_stream()
+    if plain:
+        rc = plain_lib().rgf_plain_solve(ctypes.byref(p), ctypes.byref(b), ctypes.c_void_p(stream.cuda_stream))
+        if rc != 0:
+            raise RgfError(f"plain path error {rc}: {plain_lib().rgf_plain_last_error().decode()}")
+        return out
     rc = lib().rgf_solve_batched(ctypes.byref(p), ctypes.byref(b), ctypes.c_void_p(stream.cuda_stream))
     _check(rc)
     return out

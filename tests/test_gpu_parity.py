@@ -154,3 +154,35 @@ def test_cnt_ballistic_subset_matches_oracle():
         for l in range(100):
             assert relerr(g["ldos"][..., l, e], r["ldos"][..., l, e]) <= TOL
         assert abs(g["tr"][e] - r["tr"][e]) <= TOL * max(1, abs(r["tr"][e]))
+
+
+@pytest.mark.parametrize("mode", ["compat", "physical"])
+def test_plain_per_energy_path_matches_oracle(mode):
+    """The plain per-energy comparison path (vendor zgemm/zgetrf/zgetri chain) solves the same problem."""
+    rgf = _rgf()
+    import torch
+    nm, nx, nE = 32, 4, 3
+    Hii, H1i, (sL, sG, sR) = case(nm, nx, 61, uniform=(mode == "compat"))
+    E = configs.energy_grid(-0.8, 0.8, nE)
+    m = rgf.MODE_COMPAT if mode == "compat" else rgf.MODE_PHYSICAL
+    kw = dict(eta_dev=1e-4, eta_lead=1e-4, dec_tol=1e-12)
+    p = rgf.params(m, **kw)
+    po = orc.params(orc.COMPAT if mode == "compat" else orc.PHYSICAL, **kw)
+    temp, mu = (300.0, 300.0), (0.1, -0.1)
+    dev = torch.device("cuda:0")
+    keys = ("ldos", "ndens", "pdens", "GRi1", "GLi1", "GGi1")
+    out = {k: torch.zeros(nE, nx, nm, nm, dtype=torch.complex128, device=dev) for k in keys}
+    out["cur"] = torch.zeros(nE, nx, nm, nm, dtype=torch.float64, device=dev)
+    out["tr"] = torch.zeros(nE, dtype=torch.float64, device=dev)
+    out["tre"] = torch.zeros(nE, dtype=torch.float64, device=dev)
+    out["info"] = torch.zeros(nE, 4, dtype=torch.int32, device=dev)
+    rgf.solve_batched(p, torch.tensor(E, device=dev), temp, mu, rgf.to_device_blocks(Hii, dev),
+                      rgf.to_device_blocks(H1i, dev), sigL=rgf.to_device_blocks(sL, dev),
+                      sigR=rgf.to_device_blocks(sR, dev), out=out, plain=True)
+    torch.cuda.synchronize()
+    r = orc.rgf_energies(po, temp, E, mu, Hii, H1i, sL, sG, sR)
+    g = {k: rgf.from_device_blocks(out[k]) for k in keys}
+    g["cur"] = np.asfortranarray(out["cur"].cpu().numpy().transpose(3, 2, 1, 0))
+    g["tr"], g["tre"] = out["tr"].cpu().numpy(), out["tre"].cpu().numpy()
+    for e in range(nE):
+        compare(parity.energy_slice(g, e), parity.energy_slice(r, e), H1i, mode, f"plain E[{e}]")
