@@ -90,6 +90,53 @@ def cpu_baseline(cfg, params_kw, temp, mu, threads):
                       f"extrapolated to nx={cfg['nx']}"}
 
 
+def adjoint_cost(rgf, torch, dev, stream, nE):
+    """Adjoint vs forward cost on the differentiation workload: random device
+    nm = 128, nx = 300, nE energies in [-2, 2] eV, eta 1e-5, Fermi leads
+    mu = +-0.1 eV, kT = 0.0259 eV.  Forward: the batched solve with tr and the
+    densities.  Adjoint: J = Landauer current sum_E tr(E) dE (units e/h) and its
+    gradients dJ/dU_l (all slabs) and dJ/dH1i_l (device-internal couplings), one
+    reverse sweep per energy with all left-connected and full blocks kept in HBM."""
+    from liteqt_amd import configs
+    rgf.lib().rgf_release()
+    nm, nx = 128, 300
+    Hii, H1i = configs.random_device(nm, nx, 4)
+    E = configs.energy_grid(-2.0, 2.0, nE)
+    T = 0.0259 / 8.61734e-05
+    temp, mu = (T, T), (0.1, -0.1)
+    p = rgf.params(rgf.MODE_PHYSICAL, eta_dev=1e-5, eta_lead=1e-5, dec_tol=1e-12, dec_nmax=100)
+    Hd, Cd = rgf.to_device_blocks(Hii, dev), rgf.to_device_blocks(H1i, dev)
+    Ed = torch.tensor(E, dtype=torch.float64, device=dev)
+    out = {"tr": torch.zeros(nE, dtype=torch.float64, device=dev),
+           "dens_n": torch.zeros(nE, nx, dtype=torch.float64, device=dev)}
+    w_tr = torch.full((nE,), E[1] - E[0], dtype=torch.float64, device=dev)
+
+    def fwd():
+        rgf.solve_batched(p, Ed, temp, mu, Hd, Cd, out=out, stream=stream)
+
+    def adj():
+        return rgf.adjoint_batched(p, Ed, temp, mu, Hd, Cd, w_tr=w_tr, stream=stream)
+
+    res = {}
+    for name, fn in (("forward", fwd), ("adjoint", adj)):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        res[name] = time.perf_counter() - t0
+        print(f"bench: {name} (nm={nm} nx={nx} nE={nE}) {res[name]:.2f}s", file=sys.stderr, flush=True)
+        if name == "adjoint":
+            J, dU, dH = r
+            res["current_e_over_h"] = float(J.sum().item())
+            res["max_abs_dI_dU"] = float(dU.sum(0).abs().max().item())
+    rgf.lib().rgf_release()
+    return {"workload": f"random device nm={nm} nx={nx}, {nE} energies, eta 1e-5, Fermi leads; "
+                        "J = Landauer current, gradients wrt U_l and H1i_l",
+            "forward_s": res["forward"], "adjoint_s": res["adjoint"], "ratio": res["adjoint"] / res["forward"],
+            "current_e_over_h": res["current_e_over_h"], "max_abs_dI_dU": res["max_abs_dI_dU"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,6 +146,8 @@ def main():
     ap.add_argument("--energies", type=int, default=0, help="override the number of energies")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward cost measurement")
+    ap.add_argument("--adjoint-energies", type=int, default=1024)
     ap.add_argument("--plain-energies", type=int, default=4,
                     help="energies timed on the plain per-energy GPU path (0: skip)")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
@@ -192,6 +241,9 @@ def main():
                  "kind": "one energy per launch chain: rocBLAS zgemm + rocSOLVER zgetrf/zgetri in the "
                          "reference's call order, host-checked decimation",
                  "max_abs_tr_diff_vs_batched": dtr}
+    adjoint = None
+    if not args.no_adjoint:
+        adjoint = adjoint_cost(rgf, torch, dev, stream, args.adjoint_energies)
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tfile):
@@ -219,6 +271,8 @@ def main():
                      "flops_per_launch": flops},
         "plain_gpu": plain,
         "batched_over_plain": (value / plain["value"]) if plain else None,
+        "adjoint": adjoint,
+        "adjoint_over_forward": adjoint["ratio"] if adjoint else None,
         "decimation_iters_mean": float(info[:, :2].mean()),
         "sane": ok,
     }

@@ -128,6 +128,31 @@ int rgf_solve_batched(const rgf_params *p, const rgf_batch *b, void *stream);
  * flight at block size nm and nx slabs. */
 long long rgf_workspace_bytes(int nm, int nx, int slots);
 
+/* Adjoint (reverse-mode) RGF of the physical-mode device with the contact
+ * self-energies held fixed.  Per energy e the objective is
+ *   J_e = w_tr[e] * tr_e + sum_l w_n[e*nx + l] * Im Tr G^<_ll
+ * (tr: current spectrum into the right contact, src/green_rgf.f90:4058-4066;
+ * Im Tr G^<_ll: the density integrand of src/green_rgf.f90:3919-3945); w_tr
+ * or w_n may be NULL (zero).  Outputs (device arrays): J[e]; dU[e*nx + l] =
+ * dJ_e/dU_l for a potential U_l added to the diagonal of slab l; dH1i (nx
+ * blocks, may be NULL) = sum_e (dJ_e/dRe H1i_l + i dJ_e/dIm H1i_l) for the
+ * device-internal couplings l = 2..nx-1 (1-based), zero for l = 1 and nx,
+ * whose blocks also define the leads.  lead_H00 (2 blocks, may be NULL) gives
+ * the onsite blocks of the left and right lead; NULL takes them from the edge
+ * slabs, Hii + sigR at l = 1 and nx, as the reference does
+ * (src/green_rgf.f90:3995, 4035), so that a potential on an edge slab then
+ * moves only the device slab, not the lead.  b supplies E, Hii, H1i, sigL,
+ * sigR, sig_stride, temp, mu, nE, nm (multiple of 32, <= 256), nx, max_slots
+ * and optionally tr and info.  Asynchronous on `stream`.  No counterpart in
+ * the reference (which has no derivatives); it serves optimisation and
+ * inverse-design drivers around the energy loop. */
+int rgf_adjoint_batched(const rgf_params *p, const rgf_batch *b, const double *w_tr,
+                        const double *w_n, const double *lead_H00, double *J, double *dU,
+                        double *dH1i, void *stream);
+
+/* Device workspace of rgf_adjoint_batched for `slots` energies in flight. */
+long long rgf_adjoint_workspace_bytes(int nm, int nx, int slots);
+
 /* Batched block operations of the engine, one workgroup per matrix, device
  * arrays of nbatch contiguous n x n column-major complex matrices, asynchronous
  * on `stream`; n a multiple of 32 in [32, 256].  They are the building blocks

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/rgf.h"
+#include "rgf_adjoint.hpp"
 #include "rgf_sweep.hpp"
 
 namespace {
@@ -92,6 +93,32 @@ __global__ void unpad_blocks(const V* __restrict__ src, int np, V* __restrict__ 
 }
 
 Buf g_stage[kMaxDev];
+Buf g_adj_dh[kMaxDev];
+
+typedef void (*AdjLaunch)(const rgf::AdjArgs&, int grid, hipStream_t);
+template <int T, int NMAX>
+void launch_adj(const rgf::AdjArgs& a, int grid, hipStream_t s) {
+  hipLaunchKernelGGL((rgf::rgf_adjoint<T, NMAX>), dim3(grid), dim3(rgf::Cfg<T>::NTH), 0, s, a);
+}
+struct AdjVariant {
+  AdjLaunch launch;
+  const void* fn;
+  int nth;
+};
+bool pick_adj(int n, AdjVariant* v) {
+  if (n % 32 != 0 || n < 32 || n > 256) return false;
+#define ADJV(T, NM) {launch_adj<T, NM>, reinterpret_cast<const void*>(&rgf::rgf_adjoint<T, NM>), rgf::Cfg<T>::NTH}
+  switch (n) {
+    case 32: *v = ADJV(1, 32); return true;
+    case 64: *v = ADJV(2, 64); return true;
+    case 96: *v = ADJV(3, 96); return true;
+    case 128: *v = ADJV(4, 128); return true;
+    case 192: *v = ADJV(3, 192); return true;
+    case 256: *v = ADJV(4, 256); return true;
+    default: *v = ADJV(1, 256); return true;
+  }
+#undef ADJV
+}
 
 typedef void (*SweepLaunch)(const rgf::SweepArgs&, int grid, hipStream_t);
 
@@ -402,6 +429,76 @@ int rgf_rs(const double temp[2], int nm, int nx, double E, const double mu[2], c
                    GLi1, GGi1);
 }
 
+long long rgf_adjoint_workspace_bytes(int nm, int nx, int slots) {
+  return (long long)(7 * nx + rgf::kAdjTmp) * nm * nm * 16LL * slots + (long long)nx * nm * nm * 16LL * slots;
+}
+
+int rgf_adjoint_batched(const rgf_params* p, const rgf_batch* b, const double* w_tr, const double* w_n,
+                        const double* lead_H00, double* J, double* dU, double* dH1i, void* stream) {
+  g_err.clear();
+  if (int rc = check_params(p)) return rc;
+  if (p->mode != RGF_MODE_PHYSICAL) return fail(RGF_EINVAL, "the adjoint differentiates the physical mode");
+  if (!b || !b->E || !b->Hii || !b->H1i || !J || !dU) return fail(RGF_EINVAL, "missing arguments");
+  if (b->nx < 2 || b->nE < 0) return fail(RGF_EINVAL, "need nx >= 2, nE >= 0");
+  if (b->nE == 0) return 0;
+  AdjVariant v;
+  if (!pick_adj(b->nm, &v)) return fail(RGF_EUNSUP, "rgf_adjoint_batched: nm must be a multiple of 32 in [32, 256]");
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDev) return fail(RGF_EINVAL, "device index out of range");
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, dev));
+  int per_cu = 0;
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, v.fn, v.nth, 0));
+  if (per_cu < 1) return fail(RGF_EHIP, "adjoint kernel does not fit on a CU");
+  const long long nn = (long long)b->nm * b->nm;
+  const long long slot_bytes = (7LL * b->nx + rgf::kAdjTmp) * nn * 16LL;
+  const long long dh_bytes = (long long)b->nx * nn * 16LL;
+  long long slots = std::min<long long>(b->nE, (long long)prop.multiProcessorCount * per_cu);
+  if (b->max_slots > 0) slots = std::min<long long>(slots, b->max_slots);
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  Buf& ws = g_ws[dev];
+  Buf& dh = g_adj_dh[dev];
+  if ((size_t)(slots * slot_bytes) > ws.bytes || (size_t)(slots * dh_bytes) > dh.bytes) {
+    size_t fr = 0, tot = 0;
+    HIPCHK(hipMemGetInfo(&fr, &tot));
+    const size_t avail = (size_t)((double)(fr + ws.bytes + dh.bytes) * 0.9);
+    const long long fit = (long long)(avail / (size_t)(slot_bytes + dh_bytes));
+    if (fit < 1) return fail(RGF_ENOMEM, "not enough device memory for one adjoint slot");
+    slots = std::min(slots, fit);
+    ws.release();
+    dh.release();
+    if (int rc = ws.ensure((size_t)(slots * slot_bytes))) return rc;
+    if (int rc = dh.ensure((size_t)(slots * dh_bytes))) return rc;
+  }
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  HIPCHK(hipMemsetAsync(dh.p, 0, (size_t)(slots * dh_bytes), s));
+  rgf::AdjArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.n = b->nm; a.nx = b->nx; a.nE = b->nE; a.dec_nmax = p->dec_nmax;
+  a.eta_dev = p->eta_dev; a.eta_lead = p->eta_lead; a.dec_tol = p->dec_tol; a.boltz = p->boltz;
+  a.temp[0] = b->temp[0]; a.temp[1] = b->temp[1]; a.mu[0] = b->mu[0]; a.mu[1] = b->mu[1];
+  a.E = b->E;
+  a.Hii = reinterpret_cast<const double2*>(b->Hii);
+  a.H1i = reinterpret_cast<const double2*>(b->H1i);
+  a.sigL = reinterpret_cast<const double2*>(b->sigL);
+  a.sigR = reinterpret_cast<const double2*>(b->sigR);
+  a.sig_stride = b->sig_stride;
+  a.w_tr = w_tr; a.w_n = w_n; a.lead = reinterpret_cast<const double2*>(lead_H00); a.J = J; a.tr = b->tr; a.dU = dU;
+  a.dH = reinterpret_cast<double2*>(dh.p);
+  a.info = b->info;
+  a.ws = reinterpret_cast<double2*>(ws.p);
+  a.ws_slot = (7LL * b->nx + rgf::kAdjTmp) * nn;
+  v.launch(a, (int)slots, s);
+  HIPCHK(hipGetLastError());
+  if (dH1i) {
+    hipLaunchKernelGGL(rgf::k_sum_slots, dim3(1024), dim3(256), 0, s, reinterpret_cast<const double2*>(dh.p),
+                       (int)slots, b->nx, b->nm, reinterpret_cast<double2*>(dH1i));
+    HIPCHK(hipGetLastError());
+  }
+  return 0;
+}
+
 int rgf_zgemm_batched(int n, int nbatch, int opA, const double* A, int opB, const double* B, double* C,
                       int c_rowmajor, void* stream) {
   g_err.clear();
@@ -461,6 +558,7 @@ void rgf_release(void) {
   std::lock_guard<std::mutex> lk(g_ws_mu);
   g_ws[dev].release();
   g_stage[dev].release();
+  g_adj_dh[dev].release();
 }
 
 }  // extern "C"

@@ -53,7 +53,7 @@ class Batch(ctypes.Structure):
 # every function declared in include/rgf.h
 EXPORTS = ("rgf_params_reference", "rgf_last_error", "rgf_version", "rgf_rs", "rgf_rs_ex",
            "rgf_rs_energies", "rgf_solve_batched", "rgf_workspace_bytes", "rgf_release",
-           "rgf_zgemm_batched", "rgf_zinv_batched")
+           "rgf_zgemm_batched", "rgf_zinv_batched", "rgf_adjoint_batched", "rgf_adjoint_workspace_bytes")
 
 _lib = None
 
@@ -78,6 +78,9 @@ def lib():
     L.rgf_zgemm_batched.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, vp, vp,
                                     ctypes.c_int, vp]
     L.rgf_zinv_batched.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp, vp, vp]
+    L.rgf_adjoint_batched.argtypes = [ctypes.POINTER(Params), ctypes.POINTER(Batch), vp, vp, vp, vp, vp, vp, vp]
+    L.rgf_adjoint_workspace_bytes.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.rgf_adjoint_workspace_bytes.restype = ctypes.c_longlong
     L.rgf_workspace_bytes.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.rgf_workspace_bytes.restype = ctypes.c_longlong
     _lib = L
@@ -178,6 +181,26 @@ def _tp(t):
     return None if t is None else t.data_ptr()
 
 
+def _req(t, name, dtype, numel, device):
+    """Validate a tensor handed to the C ABI as a raw pointer: right dtype,
+    contiguous, on `device`, with at least `numel` elements.  None passes."""
+    if t is None:
+        return None
+    if t.dtype != dtype:
+        raise RgfError(f"{name}: expected {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise RgfError(f"{name}: tensor must be contiguous")
+    if t.device != device:
+        raise RgfError(f"{name}: tensor on {t.device}, expected {device}")
+    if t.numel() < numel:
+        raise RgfError(f"{name}: {t.numel()} elements, need {numel}")
+    return t.data_ptr()
+
+
+_OUT_KINDS = {"ldos": "c", "ndens": "c", "pdens": "c", "GRi1": "c", "GLi1": "c", "GGi1": "c", "cur": "r",
+              "tr": "e", "tre": "e", "dens_n": "x", "dens_p": "x", "info": "i"}
+
+
 _plain = None
 
 
@@ -206,22 +229,14 @@ def solve_batched(p, E, temp, mu, Hii, H1i, sigL=None, sigR=None, out=None, max_
     stream)."""
     import torch
     out = out or {}
-    nE = int(E.shape[0])
-    nx, nm = int(Hii.shape[-3]), int(Hii.shape[-1])
-    b = Batch()
-    b.nm, b.nx, b.nE, b.max_slots = nm, nx, nE, int(max_slots)
-    b.E = _tp(E)
-    b.temp[0], b.temp[1] = float(temp[0]), float(temp[1])
-    b.mu[0], b.mu[1] = float(mu[0]), float(mu[1])
-    b.Hii, b.H1i = _tp(Hii), _tp(H1i)
-    b.sigL, b.sigR = _tp(sigL), _tp(sigR)
-    stride = 0
-    for s in (sigL, sigR):
-        if s is not None and s.dim() == 4:
-            stride = nm * nm * nx
-    b.sig_stride = stride
-    for k in ("ldos", "ndens", "pdens", "GRi1", "GLi1", "GGi1", "cur", "tr", "tre", "dens_n", "dens_p", "info"):
-        setattr(b, k, _tp(out.get(k)))
+    b = _batch(torch, E, temp, mu, Hii, H1i, sigL, sigR, max_slots)
+    nE, nx, nm, dev = b.nE, b.nx, b.nm, E.device
+    blk = nE * nx * nm * nm
+    sizes = {"c": (torch.complex128, blk), "r": (torch.float64, blk), "e": (torch.float64, nE),
+             "x": (torch.float64, nE * nx), "i": (torch.int32, 4 * nE)}
+    for k, kind in _OUT_KINDS.items():
+        dt, cnt = sizes[kind]
+        setattr(b, k, _req(out.get(k), k, dt, cnt, dev))
     if stream is None:
         stream = torch.cuda.current_stream()
     if plain:
@@ -232,6 +247,30 @@ def solve_batched(p, E, temp, mu, Hii, H1i, sigL=None, sigR=None, out=None, max_
     rc = lib().rgf_solve_batched(ctypes.byref(p), ctypes.byref(b), ctypes.c_void_p(stream.cuda_stream))
     _check(rc)
     return out
+
+
+def _batch(torch, E, temp, mu, Hii, H1i, sigL, sigR, max_slots):
+    """rgf_batch with the inputs checked: E float64 (nE,), Hii/H1i complex128
+    (nx, nm, nm) column-major blocks, sig* (nx, nm, nm) shared or (nE, nx, nm, nm)."""
+    dev = E.device
+    nE = int(E.shape[0])
+    nx, nm = int(Hii.shape[-3]), int(Hii.shape[-1])
+    b = Batch()
+    b.nm, b.nx, b.nE, b.max_slots = nm, nx, nE, int(max_slots)
+    b.E = _req(E, "E", torch.float64, nE, dev)
+    b.temp[0], b.temp[1] = float(temp[0]), float(temp[1])
+    b.mu[0], b.mu[1] = float(mu[0]), float(mu[1])
+    blk = nx * nm * nm
+    b.Hii = _req(Hii, "Hii", torch.complex128, blk, dev)
+    b.H1i = _req(H1i, "H1i", torch.complex128, blk, dev)
+    stride = 0
+    for s in (sigL, sigR):
+        if s is not None and s.dim() == 4:
+            stride = blk
+    b.sigL = _req(sigL, "sigL", torch.complex128, blk + stride * (nE - 1), dev)
+    b.sigR = _req(sigR, "sigR", torch.complex128, blk + stride * (nE - 1), dev)
+    b.sig_stride = stride
+    return b
 
 
 def to_device_blocks(a, device="cuda"):
@@ -270,3 +309,28 @@ def zinv_batched(A, stream=None):
     _check(lib().rgf_zinv_batched(int(A.shape[-1]), int(A.shape[0]), A.data_ptr(), C.data_ptr(), info.data_ptr(),
                                   ctypes.c_void_p(s.cuda_stream)))
     return C, info
+
+
+def adjoint_batched(p, E, temp, mu, Hii, H1i, sigL=None, sigR=None, w_tr=None, w_n=None, max_slots=0,
+                    stream=None, want_dH=True, tr=None, info=None, lead_H00=None):
+    """Adjoint RGF (physical mode, contacts fixed).  Tensors as in solve_batched;
+    w_tr float64 (nE,), w_n float64 (nE, nx); lead_H00 optional complex128
+    (2, nm, nm): onsite blocks of the left and right lead (default: the edge
+    slabs plus sigR there).  Returns (J[nE], dU[nE, nx], dH1i[nx, nm, nm]
+    column-major blocks or None)."""
+    import torch
+    b = _batch(torch, E, temp, mu, Hii, H1i, sigL, sigR, max_slots)
+    nE, nx, nm, dev = b.nE, b.nx, b.nm, E.device
+    J = torch.zeros(nE, dtype=torch.float64, device=dev)
+    dU = torch.zeros(nE, nx, dtype=torch.float64, device=dev)
+    dH = torch.zeros(nx, nm, nm, dtype=torch.complex128, device=dev) if want_dH else None
+    b.tr = _req(tr, "tr", torch.float64, nE, dev)
+    b.info = _req(info, "info", torch.int32, 4 * nE, dev)
+    pw = _req(w_tr, "w_tr", torch.float64, nE, dev)
+    pn = _req(w_n, "w_n", torch.float64, nE * nx, dev)
+    pl = _req(lead_H00, "lead_H00", torch.complex128, 2 * nm * nm, dev)
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    _check(lib().rgf_adjoint_batched(ctypes.byref(p), ctypes.byref(b), pw, pn, pl, _tp(J), _tp(dU),
+                                     _tp(dH), ctypes.c_void_p(stream.cuda_stream)))
+    return J, dU, dH

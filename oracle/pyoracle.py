@@ -189,3 +189,34 @@ def rgf_energies(p, temp, E, mu, Hii, H1i, sigL=None, sigG=None, sigR=None,
     out["tre"] = tre
     out["diag"] = [(d.iters_left, d.iters_right, d.nmax_hit, d.inv_failures) for d in diag]
     return {k: v for k, v in out.items() if v is not None}
+
+
+def rgf_fixed_leads(p, temp, E, mu, Hii, H1i, sigL=None, sigR=None, leadL_H00=None, leadR_H00=None):
+    """orc_rgf_fixed_leads: the lead decimations use the given onsite blocks
+    (default: the unperturbed edge blocks Hii[...,0] + sigR[...,0], Hii[...,-1] + sigR[...,-1])."""
+    L = lib()
+    if not hasattr(L, "_fl"):
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.orc_rgf_fixed_leads.argtypes = [ctypes.POINTER(OrcParams), dp, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_double, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp,
+                                          ctypes.POINTER(OrcDiag)]
+        L._fl = True
+    Hii, H1i = _cf(Hii), _cf(H1i)
+    nm, _, nx = Hii.shape
+    sigL = None if sigL is None else _cf(sigL)
+    sigR = None if sigR is None else _cf(sigR)
+    if leadL_H00 is None:
+        leadL_H00 = Hii[:, :, 0] + (sigR[:, :, 0] if sigR is not None else 0)
+    if leadR_H00 is None:
+        leadR_H00 = Hii[:, :, nx - 1] + (sigR[:, :, nx - 1] if sigR is not None else 0)
+    leadL_H00, leadR_H00 = _cf(leadL_H00), _cf(leadR_H00)
+    ndens = np.zeros((nm, nm, nx), np.complex128, order="F")
+    ldos = np.zeros((nm, nm, nx), np.complex128, order="F")
+    tr, tre = ctypes.c_double(0.0), ctypes.c_double(0.0)
+    t = np.asarray(temp, np.float64)
+    m = np.asarray(mu, np.float64)
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.orc_rgf_fixed_leads(ctypes.byref(p), _p(t), nm, nx, float(E), _p(m), _p(Hii), _p(H1i), _p(sigL), _p(sigR),
+                          _p(leadL_H00), _p(leadR_H00), _p(ndens), _p(ldos), ctypes.cast(ctypes.byref(tr), dp),
+                          ctypes.cast(ctypes.byref(tre), dp), None)
+    return {"ndens": ndens, "ldos": ldos, "tr": tr.value, "tre": tre.value}

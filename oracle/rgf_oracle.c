@@ -333,11 +333,12 @@ int orc_sancho(int n, double E, double eta_lead, double tol, int nmax,
  *     G^>_{l,l+1} = G^<_{l,l+1} + G^R_{l,l+1} - (G^R_{l+1,l})^H;
  *   - compat: the right contact uses TEMP(1) and tre uses TEMP(2)
  *     (green_rgf.f90:4051, 4058, 4152); physical: the natural assignment. */
-void orc_rgf(const orc_params *p, const double temp[2], int nm, int nx, double E,
-             const double mu[2], const double *Hii, const double *H1i,
-             const double *sigL, const double *sigG, const double *sigR,
-             double *ndens, double *pdens, double *ldos, double *tr, double *tre,
-             double *cur, double *GRi1, double *GLi1, double *GGi1, orc_diag *diag) {
+static void rgf_impl(const orc_params *p, const double temp[2], int nm, int nx, double E,
+                     const double mu[2], const double *Hii, const double *H1i,
+                     const double *sigL, const double *sigG, const double *sigR,
+                     double *ndens, double *pdens, double *ldos, double *tr, double *tre,
+                     double *cur, double *GRi1, double *GLi1, double *GGi1, orc_diag *diag,
+                     const double *leadL_H00, const double *leadR_H00) {
   (void)sigG; /* accepted but unused by the reference (green_rgf.f90:3953) */
   const int n = nm;
   const size_t bs = 2 * (size_t)n * n;
@@ -362,6 +363,9 @@ void orc_rgf(const orc_params *p, const double temp[2], int nm, int nx, double E
 
   /* left contact, green_rgf.f90:3992-4012 */
   if (sigR) madd(n, H00, blk(Hii, n, 0), blk(sigR, n, 0)); else mcopy(n, H00, blk(Hii, n, 0));
+  double *H00dev = mnew(n);
+  mcopy(n, H00dev, H00);
+  if (leadL_H00) mcopy(n, H00, leadL_H00);
   mcopy(n, H10, blk(H1i, n, 0));
   for (int j = 0; j < n; ++j)
     for (int i = 0; i < n; ++i) {
@@ -375,6 +379,7 @@ void orc_rgf(const orc_params *p, const double temp[2], int nm, int nx, double E
                                &hit, &dg.inv_failures);
     if (hit) dg.nmax_hit |= 1;
   }
+  mcopy(n, H00, H00dev);
   orc_zgemm('n', 'n', n, H10, G00, A);
   orc_zgemm('n', 'c', n, A, H10, sigmal);
   {
@@ -412,8 +417,9 @@ void orc_rgf(const orc_params *p, const double temp[2], int nm, int nx, double E
   mcopy(n, H10, blk(H1i, n, nx - 1));
   {
     int hit = 0;
-    dg.iters_right = orc_sancho(n, E, p->eta_lead, p->dec_tol, p->dec_nmax, H00, H10, G00, GBB,
-                                &hit, &dg.inv_failures);
+    mcopy(n, H00dev, H00);
+    dg.iters_right = orc_sancho(n, E, p->eta_lead, p->dec_tol, p->dec_nmax, leadR_H00 ? leadR_H00 : H00, H10,
+                                G00, GBB, &hit, &dg.inv_failures);
     if (hit) dg.nmax_hit |= 2;
   }
   orc_zgemm('c', 'n', n, H10, G00, A);
@@ -546,7 +552,26 @@ void orc_rgf(const orc_params *p, const double temp[2], int nm, int nx, double E
   free(Gl); free(Gln);
   free(H00); free(H10); free(A); free(B); free(C); free(D); free(G00); free(GBB);
   free(GN0); free(Gn); free(Gp); free(sig); free(sigmal); free(sigmar); free(H10h); free(T1);
+  free(H00dev);
   (void)mzero;
+}
+
+void orc_rgf(const orc_params *p, const double temp[2], int nm, int nx, double E,
+             const double mu[2], const double *Hii, const double *H1i,
+             const double *sigL, const double *sigG, const double *sigR,
+             double *ndens, double *pdens, double *ldos, double *tr, double *tre,
+             double *cur, double *GRi1, double *GLi1, double *GGi1, orc_diag *diag) {
+  rgf_impl(p, temp, nm, nx, E, mu, Hii, H1i, sigL, sigG, sigR, ndens, pdens, ldos, tr, tre, cur,
+           GRi1, GLi1, GGi1, diag, NULL, NULL);
+}
+
+void orc_rgf_fixed_leads(const orc_params *p, const double temp[2], int nm, int nx, double E,
+                         const double mu[2], const double *Hii, const double *H1i,
+                         const double *sigL, const double *sigR, const double *leadL_H00,
+                         const double *leadR_H00, double *ndens, double *ldos, double *tr,
+                         double *tre, orc_diag *diag) {
+  rgf_impl(p, temp, nm, nx, E, mu, Hii, H1i, sigL, NULL, sigR, ndens, NULL, ldos, tr, tre, NULL,
+           NULL, NULL, NULL, diag, leadL_H00, leadR_H00);
 }
 
 void orc_rgf_energies(const orc_params *p, const double temp[2], int nm, int nx,

@@ -78,6 +78,16 @@ void orc_rgf(const orc_params *p, const double temp[2], int nm, int nx, double E
              double *ndens, double *pdens, double *ldos, double *tr, double *tre,
              double *cur, double *GRi1, double *GLi1, double *GGi1, orc_diag *diag);
 
+/* orc_rgf with the lead decimations fed by explicit onsite blocks (the
+ * reference feeds them Hii(:,:,1)+sigR(:,:,1) and Hii(:,:,nx)+sigR(:,:,nx),
+ * green_rgf.f90:3992, 4032); used to differentiate the device with the
+ * contacts held fixed. */
+void orc_rgf_fixed_leads(const orc_params *p, const double temp[2], int nm, int nx, double E,
+                         const double mu[2], const double *Hii, const double *H1i,
+                         const double *sigL, const double *sigR, const double *leadL_H00,
+                         const double *leadR_H00, double *ndens, double *ldos, double *tr,
+                         double *tre, orc_diag *diag);
+
 /* Energy loop of the reference drivers (green_rgf.f90:1305-1316): OpenMP over
  * energies, one energy per thread.  Per-energy arrays have a trailing energy
  * index (stride nm*nm*nx complex); sig_stride_e is the energy stride of the

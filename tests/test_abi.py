@@ -40,3 +40,19 @@ def test_workspace_bytes():
     b1 = rgf.lib().rgf_workspace_bytes(256, 200, 1)
     assert b1 % (256 * 256 * 16) == 0 and b1 // (256 * 256 * 16) >= 400
     assert rgf.lib().rgf_workspace_bytes(256, 200, 10) == 10 * b1
+
+
+def test_tensor_checks_reject_wrong_dtype_layout_and_size():
+    """Raw pointers cross the C ABI: the binding refuses tensors whose dtype,
+    layout or size the engine would misread (no GPU call is made)."""
+    import torch
+    from liteqt_amd import rgf
+    cpu = torch.device("cpu")
+    ok = torch.zeros(4, dtype=torch.float64)
+    assert rgf._req(ok, "w", torch.float64, 4, cpu) == ok.data_ptr()
+    assert rgf._req(None, "w", torch.float64, 4, cpu) is None
+    for bad, why in ((torch.zeros(4, dtype=torch.float32), "expected"),
+                     (torch.zeros(4, 2, dtype=torch.float64).t(), "contiguous"),
+                     (torch.zeros(3, dtype=torch.float64), "elements")):
+        with pytest.raises(rgf.RgfError, match=why):
+            rgf._req(bad, "w", torch.float64, 4, cpu)

@@ -1,0 +1,123 @@
+"""GPU adjoint RGF against (1) the numpy reverse pass of tests/adjoint_ref.py
+(itself pinned to finite differences of the oracle), (2) central finite
+differences of the CPU oracle with h = 1e-6 eV on random slabs, (3)
+torch.autograd.gradcheck through liteqt_amd.autograd.RGFObjective."""
+import numpy as np
+import pytest
+
+from liteqt_amd import configs
+from oracle import pyoracle as orc
+from tests import adjoint_ref
+from tests.dense_ref import fermi, lead_self_energies
+
+pytestmark = pytest.mark.gpu
+
+
+def _env():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.fail("no HIP device visible")
+    from liteqt_amd import rgf
+    return torch, rgf
+
+
+def _case(nm, nx, seed, scatter=True):
+    Hii, H1i = configs.random_device(nm, nx, seed, identical_leads=False)
+    if scatter:
+        sL, _, sR = configs.scattering_self_energies(nm, nx, seed + 1, E=0.1, mu=0.0, scale=0.02)
+    else:
+        sL = sR = None
+    return Hii, H1i, sL, sR
+
+
+@pytest.mark.parametrize("nm,nx,scatter", [(32, 4, True), (64, 5, False), (32, 2, True), (128, 3, True)])
+def test_adjoint_kernel_matches_reference_reverse_pass(nm, nx, scatter):
+    torch, rgf = _env()
+    dev = torch.device("cuda:0")
+    Hii, H1i, sL, sR = _case(nm, nx, 40 + nm + nx, scatter)
+    kw = dict(eta_dev=1e-4, eta_lead=1e-4, dec_tol=1e-12)
+    p = rgf.params(rgf.MODE_PHYSICAL, **kw)
+    po = orc.params(orc.PHYSICAL, **kw)
+    E = np.array([-0.3, 0.05, 0.4])
+    temp, mu = (300.0, 250.0), (0.1, -0.1)
+    rng = np.random.default_rng(nm + nx)
+    w_tr = rng.standard_normal(len(E))
+    w_n = rng.standard_normal((len(E), nx))
+    Jg, dUg, dHg = rgf.adjoint_batched(
+        p, torch.tensor(E, device=dev), temp, mu, rgf.to_device_blocks(Hii, dev), rgf.to_device_blocks(H1i, dev),
+        sigL=None if sL is None else rgf.to_device_blocks(sL, dev),
+        sigR=None if sR is None else rgf.to_device_blocks(sR, dev),
+        w_tr=torch.tensor(w_tr, device=dev), w_n=torch.tensor(w_n, device=dev))
+    torch.cuda.synchronize()
+    Jg, dUg = Jg.cpu().numpy(), dUg.cpu().numpy()
+    dHg = rgf.from_device_blocks(dHg)
+    dH_ref = np.zeros((nm, nm, nx), complex)
+    for e, En in enumerate(E):
+        SigL, SigR = lead_self_energies(orc.sancho, En, Hii, H1i, sR, po.eta_lead, po.dec_tol, po.dec_nmax)
+        fL, fR = fermi(En, mu[0], po.boltz, temp[0]), fermi(En, mu[1], po.boltz, temp[1])
+        J, dU, dH = adjoint_ref.adjoint(En, po.eta_dev, Hii, H1i, sL, sR, SigL, SigR, fL, fR, w_tr[e], w_n[e])
+        assert abs(Jg[e] - J) <= 1e-10 * max(1.0, abs(J)), (e, Jg[e], J)
+        assert np.linalg.norm(dUg[e] - dU) <= 1e-9 * max(1.0, np.linalg.norm(dU)), (e, dUg[e], dU)
+        dH_ref += dH
+    assert np.linalg.norm(dHg - dH_ref) <= 1e-9 * max(1.0, np.linalg.norm(dH_ref))
+
+
+def test_adjoint_kernel_matches_oracle_central_fd():
+    """dJ/dU_l and directional dJ/dH1i_l vs central differences (h = 1e-6) of the
+    CPU oracle with the contacts held fixed, on random slabs."""
+    torch, rgf = _env()
+    dev = torch.device("cuda:0")
+    nm, nx = 32, 12
+    Hii, H1i, sL, sR = _case(nm, nx, 77, scatter=False)
+    kw = dict(eta_dev=1e-4, eta_lead=1e-4, dec_tol=1e-12)
+    p, po = rgf.params(rgf.MODE_PHYSICAL, **kw), orc.params(orc.PHYSICAL, **kw)
+    E0 = 0.21
+    temp, mu = (300.0, 300.0), (0.1, -0.1)
+    w_n = np.linspace(0.5, 1.5, nx)
+    J, dU, dH = rgf.adjoint_batched(p, torch.tensor([E0], dtype=torch.float64, device=dev), temp, mu, rgf.to_device_blocks(Hii, dev),
+                                    rgf.to_device_blocks(H1i, dev),
+                                    w_tr=torch.tensor([1.0], dtype=torch.float64, device=dev),
+                                    w_n=torch.tensor(w_n[None, :], device=dev))
+    torch.cuda.synchronize()
+    dU = dU.cpu().numpy()[0]
+    dH = rgf.from_device_blocks(dH)
+    leadL, leadR = Hii[:, :, 0].copy(), Hii[:, :, nx - 1].copy()
+
+    def Jo(Hm, Cm):
+        o = orc.rgf_fixed_leads(po, temp, E0, mu, Hm, Cm, None, None, leadL, leadR)
+        return o["tr"] + sum(w_n[l] * np.imag(np.trace(o["ndens"][:, :, l])) for l in range(nx))
+
+    h = 1e-6
+    rng = np.random.default_rng(5)
+    for l in rng.choice(nx, 8, replace=False):
+        Hp, Hm = Hii.copy(order="F"), Hii.copy(order="F")
+        Hp[:, :, l] += h * np.eye(nm)
+        Hm[:, :, l] -= h * np.eye(nm)
+        fd = (Jo(Hp, H1i) - Jo(Hm, H1i)) / (2 * h)
+        assert abs(fd - dU[l]) <= 1e-6 * max(1.0, abs(fd)), (l, fd, dU[l])
+    for l in rng.choice(np.arange(1, nx - 1), 4, replace=False):
+        D = rng.standard_normal((nm, nm)) + 1j * rng.standard_normal((nm, nm))
+        Cp, Cm = H1i.copy(order="F"), H1i.copy(order="F")
+        Cp[:, :, l] += h * D
+        Cm[:, :, l] -= h * D
+        fd = (Jo(Hii, Cp) - Jo(Hii, Cm)) / (2 * h)
+        ad = np.real(np.vdot(dH[:, :, l], D))
+        assert abs(fd - ad) <= 1e-6 * max(1.0, abs(fd)), (l, fd, ad)
+
+
+def test_autograd_function_gradcheck():
+    torch, rgf = _env()
+    from liteqt_amd.autograd import RGFObjective
+    dev = torch.device("cuda:0")
+    nm, nx = 32, 4
+    Hii, H1i, sL, sR = _case(nm, nx, 91, scatter=True)
+    p = rgf.params(rgf.MODE_PHYSICAL, eta_dev=1e-3, eta_lead=1e-3, dec_tol=1e-12)
+    E = torch.tensor([0.0, 0.3], dtype=torch.float64, device=dev)
+    Hd, Cd = rgf.to_device_blocks(Hii, dev), rgf.to_device_blocks(H1i, dev)
+    sLd, sRd = rgf.to_device_blocks(sL, dev), rgf.to_device_blocks(sR, dev)
+    w_tr = torch.tensor([1.0, 0.5], dtype=torch.float64, device=dev)
+    w_n = torch.ones(2, nx, dtype=torch.float64, device=dev) * 0.3
+    U = torch.zeros(nx, dtype=torch.float64, device=dev, requires_grad=True)
+    Tint = Cd[1:nx - 1].clone().requires_grad_(True)
+    f = lambda U, Tint: RGFObjective.apply(U, Tint, p, E, (300.0, 300.0), (0.1, -0.1), Hd, Cd, sLd, sRd, w_tr, w_n)
+    assert torch.autograd.gradcheck(f, (U, Tint), eps=1e-6, atol=1e-6, rtol=1e-5, fast_mode=True)
