@@ -120,9 +120,47 @@ typedef struct rgf_batch {
   double *tr, *tre;
   double *dens_n, *dens_p;
   int *info;
+  /* First off-diagonal blocks of a block-tridiagonal scattering self-energy
+   * (an extension: the reference's sigma arrays are diagonal blocks only,
+   * src/green_rgf.f90:3952).  Block l (0-based, l = 1..nx-1; block 0 unused)
+   * of sigR_lo / sigR_up / sigL_up is Sigma^R_{l,l-1} / Sigma^R_{l-1,l} /
+   * Sigma^<_{l-1,l}; Sigma^<_{l,l-1} = -(Sigma^<_{l-1,l})^H.  Same energy
+   * stride as sigL/sigR.  Any may be NULL (zero); a non-NULL one needs
+   * RGF_MODE_PHYSICAL.  Zero-initialise the struct so unused fields are NULL. */
+  const double *sigR_lo, *sigR_up, *sigL_up;
 } rgf_batch;
 
 int rgf_solve_batched(const rgf_params *p, const rgf_batch *b, void *stream);
+
+/* One elastic self-consistent Born (SCBA) iteration: the batched solve of b
+ * (RGF_MODE_PHYSICAL, per-energy Sigma with sig_stride = nm*nm*nx) that also
+ * refreshes, in place and per energy, every Sigma block it was given:
+ *   Sigma <- (1 - mix) * Sigma + mix * D * G
+ * Sigma^R (sigR, sigR_lo, sigR_up) from G^R, Sigma^< (sigL, sigL_up) from
+ * G^< on the same blocks.  sigL/sigR are written through despite their const
+ * qualifier.  The outputs of b are those of the solve with the old Sigma.
+ * No counterpart in the reference (whose SCBA loops build Sigma on the host,
+ * src/green_rgf.f90:3174-3290); it keeps the self-energy in HBM between
+ * iterations. */
+typedef struct rgf_scba {
+  double D;   /* scattering strength (eV^2 for deformation-potential models) */
+  double mix; /* linear mixing of the new self-energy, 0 < mix <= 1 */
+} rgf_scba;
+
+int rgf_scba_step(const rgf_params *p, const rgf_batch *b, const rgf_scba *scba, void *stream);
+
+/* HIP-graph form of rgf_solve_batched / rgf_scba_step: rgf_graph_create
+ * captures one launch on `stream` (a created stream, not NULL) into an
+ * executable graph that owns its workspace; rgf_graph_launch replays it
+ * `replays` times on `stream` (with scba given, each replay is one SCBA
+ * iteration on the same device arrays).  The graph keeps the pointers of b.
+ * No counterpart in the reference; for drivers that repeat the solve on
+ * refreshed inputs without host launch overhead. */
+typedef struct rgf_graph_s *rgf_graph;
+int rgf_graph_create(const rgf_params *p, const rgf_batch *b, const rgf_scba *scba, void *stream,
+                     rgf_graph *out);
+int rgf_graph_launch(rgf_graph g, int replays, void *stream);
+void rgf_graph_destroy(rgf_graph g);
 
 /* Bytes of device workspace rgf_solve_batched uses for `slots` energies in
  * flight at block size nm and nx slabs. */

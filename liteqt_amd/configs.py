@@ -122,3 +122,44 @@ def scattering_self_energies(nm, L, seed, E=0.0, mu=0.0, kT=0.0259, scale=1e-3, 
     if np.ndim(E) == 0:
         return sigL[..., 0].copy(order="F"), sigG[..., 0].copy(order="F"), sigR[..., 0].copy(order="F")
     return sigL, sigG, sigR
+
+
+def scattering_blocks(nm, nx, seed, scale=1e-3):
+    """Block-tridiagonal scattering matrices of a synthetic SCBA step: a
+    Hermitian positive semidefinite Gamma = B B^H (B block lower bidiagonal,
+    random) and a Hermitian Lambda, both of Frobenius size ~scale per block.
+    Returns (Gam_d, Gam_lo, Lam_d, Lam_lo), (nm, nm, nx) each, with
+    Gam_lo[..., l] = Gamma_{l,l-1} (block 0 zero)."""
+    rng = np.random.default_rng(seed)
+
+    def cn(*shape):
+        return (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) / np.sqrt(2.0)
+
+    Bd = cn(nx, nm, nm) / np.sqrt(2 * nm)
+    Bl = cn(nx, nm, nm) / np.sqrt(2 * nm)
+    Bl[0] = 0
+    H = lambda a: np.conj(np.swapaxes(a, -1, -2))
+    Gd = Bd @ H(Bd) + Bl @ H(Bl)
+    Glo = np.zeros_like(Gd)
+    Glo[1:] = Bl[1:] @ H(Bd[:-1])  # (B B^H)_{l,l-1} = B_{l,l-1} B_{l-1,l-1}^H
+    A = cn(nx, nm, nm) / np.sqrt(nm)
+    Ld = (A + H(A)) / 2
+    Llo = cn(nx, nm, nm) / np.sqrt(nm)
+    Llo[0] = 0
+    tof = lambda a: np.asfortranarray(np.moveaxis(a, 0, -1)) * scale
+    return tof(Gd), tof(Glo), tof(Ld), tof(Llo)
+
+
+def offdiag_self_energies(blocks, f):
+    """(sigL, sigR, sigR_lo, sigR_up, sigL_up) from scattering_blocks at
+    occupation f: Sigma^R = Lambda - i Gamma / 2, Sigma^< = i f Gamma (so that
+    Sigma^> - Sigma^< = Sigma^R - Sigma^A and Sigma^< = -f (Sigma^R - Sigma^A))."""
+    Gd, Glo, Ld, Llo = blocks
+    H = lambda a: np.conj(np.transpose(a, (1, 0, 2)))
+    sigR = Ld - 0.5j * Gd
+    sigR_lo = Llo - 0.5j * Glo
+    sigR_up = H(Llo) - 0.5j * H(Glo)
+    sigL = 1j * f * Gd
+    sigL_up = 1j * f * H(Glo)
+    c = lambda a: np.asfortranarray(a, dtype=np.complex128)
+    return c(sigL), c(sigR), c(sigR_lo), c(sigR_up), c(sigL_up)

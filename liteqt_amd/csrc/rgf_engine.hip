@@ -12,6 +12,7 @@
 
 #include "../../include/rgf.h"
 #include "rgf_adjoint.hpp"
+#include "rgf_offdiag.hpp"
 #include "rgf_sweep.hpp"
 
 namespace {
@@ -132,6 +133,16 @@ const void* sweep_fn() {
   return reinterpret_cast<const void*>(&rgf::rgf_sweep<T, NMAX>);
 }
 
+template <int T, int NMAX>
+void launch_sweep_od(const rgf::SweepArgs& a, int grid, hipStream_t s) {
+  hipLaunchKernelGGL((rgf::rgf_sweep_od<T, NMAX>), dim3(grid), dim3(rgf::Cfg<T>::NTH), 0, s, a);
+}
+
+template <int T, int NMAX>
+const void* sweep_od_fn() {
+  return reinterpret_cast<const void*>(&rgf::rgf_sweep_od<T, NMAX>);
+}
+
 struct Variant {
   SweepLaunch launch;
   const void* fn;
@@ -153,7 +164,24 @@ bool pick_variant(int n, Variant* v) {
   }
 }
 
+// The same tile variants for the block-tridiagonal-Sigma kernel.
+bool pick_variant_od(int n, Variant* v) {
+  if (n % 32 != 0 || n < 32 || n > 256) return false;
+  switch (n) {
+    case 32: *v = {launch_sweep_od<1, 32>, sweep_od_fn<1, 32>(), 1, 32, rgf::Cfg<1>::NTH}; return true;
+    case 64: *v = {launch_sweep_od<2, 64>, sweep_od_fn<2, 64>(), 2, 64, rgf::Cfg<2>::NTH}; return true;
+    case 96: *v = {launch_sweep_od<3, 96>, sweep_od_fn<3, 96>(), 3, 96, rgf::Cfg<3>::NTH}; return true;
+    case 128: *v = {launch_sweep_od<4, 128>, sweep_od_fn<4, 128>(), 4, 128, rgf::Cfg<4>::NTH}; return true;
+    case 192: *v = {launch_sweep_od<3, 192>, sweep_od_fn<3, 192>(), 3, 192, rgf::Cfg<3>::NTH}; return true;
+    case 256: *v = {launch_sweep_od<4, 256>, sweep_od_fn<4, 256>(), 4, 256, rgf::Cfg<4>::NTH}; return true;
+    default: *v = {launch_sweep_od<1, 256>, sweep_od_fn<1, 256>(), 1, 256, rgf::Cfg<1>::NTH}; return true;
+  }
+}
+
 long long slot_elems(int n, int nx) { return (long long)(2 * nx + rgf::kTmpBlocks) * n * n; }
+long long slot_elems_od(int n, int nx) { return (long long)(2 * nx + rgf::kTmpBlocksOd) * n * n; }
+
+bool has_offdiag(const rgf_batch* b) { return b->sigR_lo || b->sigR_up || b->sigL_up; }
 
 int check_params(const rgf_params* p) {
   if (!p) return fail(RGF_EINVAL, "params is NULL");
@@ -162,37 +190,37 @@ int check_params(const rgf_params* p) {
   return 0;
 }
 
-}  // namespace
+// Everything a sweep launch needs: variant, slot count, kernel arguments.
+struct Prepared {
+  Variant v;
+  rgf::SweepArgs a;
+  long long slots = 0, slot_elems = 0;
+  int dev = 0;
+  bool nothing = false;
+};
 
-extern "C" {
-
-void rgf_params_reference(rgf_params* p) {
-  p->eta_dev = 0.0;
-  p->eta_lead = 1.0e-3;
-  p->dec_tol = 1.0e-100;
-  p->dec_nmax = 100;
-  p->boltz = 8.61734e-05;
-  p->mode = RGF_MODE_COMPAT;
-}
-
-const char* rgf_last_error(void) { return g_err.c_str(); }
-
-const char* rgf_version(void) { return "liteqt_amd rgf 0.1 (gfx950, complex FP64 MFMA)"; }
-
-long long rgf_workspace_bytes(int nm, int nx, int slots) {
-  return slot_elems(nm, nx) * 16LL * (long long)slots;
-}
-
-int rgf_solve_batched(const rgf_params* p, const rgf_batch* b, void* stream) {
-  g_err.clear();
+// Validate a batch, pick the kernel and size the slots (HBM admission).  The
+// workspace pointer is left null; the caller binds the shared or an owned one.
+int prepare(const rgf_params* p, const rgf_batch* b, const rgf_scba* scba, Prepared* out) {
   if (int rc = check_params(p)) return rc;
   if (!b) return fail(RGF_EINVAL, "batch is NULL");
   if (b->nx < 2) return fail(RGF_EINVAL, "nx must be >= 2 (the reference reads Gl(:,:,nx-1))");
   if (b->nE < 0) return fail(RGF_EINVAL, "nE < 0");
-  if (b->nE == 0) return 0;
+  if (b->nE == 0) {
+    out->nothing = true;
+    return 0;
+  }
   if (!b->E || !b->Hii || !b->H1i) return fail(RGF_EINVAL, "E, Hii and H1i are required");
-  Variant v;
-  if (!pick_variant(b->nm, &v))
+  const bool od = has_offdiag(b) || scba;
+  if (od && p->mode != RGF_MODE_PHYSICAL)
+    return fail(RGF_EINVAL, "off-diagonal Sigma and the SCBA refresh need RGF_MODE_PHYSICAL");
+  if (scba) {
+    if (!b->sigR || !b->sigL) return fail(RGF_EINVAL, "SCBA refresh: sigR and sigL are required");
+    if (b->sig_stride != (long long)b->nm * b->nm * b->nx)
+      return fail(RGF_EINVAL, "SCBA refresh: Sigma must be per energy (sig_stride = nm*nm*nx)");
+  }
+  Variant& v = out->v;
+  if (!(od ? pick_variant_od(b->nm, &v) : pick_variant(b->nm, &v)))
     return fail(RGF_EUNSUP, "rgf_solve_batched: nm must be a multiple of 32 in [32, 256]");
   int dev = 0;
   HIPCHK(hipGetDevice(&dev));
@@ -202,23 +230,13 @@ int rgf_solve_batched(const rgf_params* p, const rgf_batch* b, void* stream) {
   int per_cu = 0;
   HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, v.fn, v.nth, 0));
   if (per_cu < 1) return fail(RGF_EHIP, "sweep kernel does not fit on a CU");
-  const long long slot_bytes = slot_elems(b->nm, b->nx) * 16LL;
+  const long long se = od ? slot_elems_od(b->nm, b->nx) : slot_elems(b->nm, b->nx);
   long long slots = std::min<long long>(b->nE, (long long)prop.multiProcessorCount * per_cu);
   if (b->max_slots > 0) slots = std::min<long long>(slots, b->max_slots);
-  std::lock_guard<std::mutex> lk(g_ws_mu);
-  Buf& ws = g_ws[dev];
-  if ((size_t)(slots * slot_bytes) > ws.bytes) {
-    // HBM admission: fit the stored left-connected blocks into free memory
-    size_t fr = 0, tot = 0;
-    HIPCHK(hipMemGetInfo(&fr, &tot));
-    const size_t avail = (size_t)((double)(fr + ws.bytes) * 0.9);
-    const long long fit = (long long)(avail / (size_t)slot_bytes);
-    if (fit < 1) return fail(RGF_ENOMEM, "not enough device memory for one energy slot");
-    slots = std::min(slots, fit);
-    ws.release();
-    if (int rc = ws.ensure((size_t)(slots * slot_bytes))) return rc;
-  }
-  rgf::SweepArgs a;
+  out->slots = slots;
+  out->slot_elems = se;
+  out->dev = dev;
+  rgf::SweepArgs& a = out->a;
   std::memset(&a, 0, sizeof(a));
   a.n = b->nm;
   a.nx = b->nx;
@@ -251,11 +269,135 @@ int rgf_solve_batched(const rgf_params* p, const rgf_batch* b, void* stream) {
   a.dens_n = b->dens_n;
   a.dens_p = b->dens_p;
   a.info = b->info;
-  a.ws = reinterpret_cast<double2*>(ws.p);
-  a.ws_slot = slot_elems(b->nm, b->nx);
-  v.launch(a, (int)slots, reinterpret_cast<hipStream_t>(stream));
+  a.ws_slot = se;
+  a.sigR_lo = reinterpret_cast<const double2*>(b->sigR_lo);
+  a.sigR_up = reinterpret_cast<const double2*>(b->sigR_up);
+  a.sigL_up = reinterpret_cast<const double2*>(b->sigL_up);
+  if (scba) {
+    a.scba = 1;
+    a.scba_D = scba->D;
+    a.scba_mix = scba->mix;
+  }
+  return 0;
+}
+
+int solve_impl(const rgf_params* p, const rgf_batch* b, const rgf_scba* scba, void* stream) {
+  g_err.clear();
+  Prepared pr;
+  if (int rc = prepare(p, b, scba, &pr)) return rc;
+  if (pr.nothing) return 0;
+  const long long slot_bytes = pr.slot_elems * 16LL;
+  long long slots = pr.slots;
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  Buf& ws = g_ws[pr.dev];
+  if ((size_t)(slots * slot_bytes) > ws.bytes) {
+    // HBM admission: fit the stored left-connected blocks into free memory
+    size_t fr = 0, tot = 0;
+    HIPCHK(hipMemGetInfo(&fr, &tot));
+    const size_t avail = (size_t)((double)(fr + ws.bytes) * 0.9);
+    const long long fit = (long long)(avail / (size_t)slot_bytes);
+    if (fit < 1) return fail(RGF_ENOMEM, "not enough device memory for one energy slot");
+    slots = std::min(slots, fit);
+    ws.release();
+    if (int rc = ws.ensure((size_t)(slots * slot_bytes))) return rc;
+  }
+  pr.a.ws = reinterpret_cast<double2*>(ws.p);
+  pr.v.launch(pr.a, (int)slots, reinterpret_cast<hipStream_t>(stream));
   HIPCHK(hipGetLastError());
   return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+void rgf_params_reference(rgf_params* p) {
+  p->eta_dev = 0.0;
+  p->eta_lead = 1.0e-3;
+  p->dec_tol = 1.0e-100;
+  p->dec_nmax = 100;
+  p->boltz = 8.61734e-05;
+  p->mode = RGF_MODE_COMPAT;
+}
+
+const char* rgf_last_error(void) { return g_err.c_str(); }
+
+const char* rgf_version(void) { return "liteqt_amd rgf 0.1 (gfx950, complex FP64 MFMA)"; }
+
+long long rgf_workspace_bytes(int nm, int nx, int slots) {
+  return slot_elems(nm, nx) * 16LL * (long long)slots;
+}
+
+int rgf_solve_batched(const rgf_params* p, const rgf_batch* b, void* stream) {
+  return solve_impl(p, b, nullptr, stream);
+}
+
+int rgf_scba_step(const rgf_params* p, const rgf_batch* b, const rgf_scba* scba, void* stream) {
+  if (!scba) {
+    g_err.clear();
+    return fail(RGF_EINVAL, "rgf_scba_step: scba is NULL");
+  }
+  return solve_impl(p, b, scba, stream);
+}
+
+struct rgf_graph_s {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  void* ws = nullptr;
+  int dev = 0;
+};
+
+int rgf_graph_create(const rgf_params* p, const rgf_batch* b, const rgf_scba* scba, void* stream,
+                     rgf_graph* out) {
+  g_err.clear();
+  if (!out) return fail(RGF_EINVAL, "rgf_graph_create: out is NULL");
+  *out = nullptr;
+  if (!stream) return fail(RGF_EINVAL, "rgf_graph_create: capture needs a created stream (not the NULL stream)");
+  Prepared pr;
+  if (int rc = prepare(p, b, scba, &pr)) return rc;
+  if (pr.nothing) return fail(RGF_EINVAL, "rgf_graph_create: nothing to solve (nE == 0)");
+  // the graph owns its workspace, so later calls that grow the shared one cannot move it
+  rgf_graph_s* g = new rgf_graph_s();
+  g->dev = pr.dev;
+  hipError_t e = hipMalloc(&g->ws, (size_t)(pr.slots * pr.slot_elems * 16LL));
+  if (e != hipSuccess) {
+    delete g;
+    return fail(e == hipErrorOutOfMemory ? RGF_ENOMEM : RGF_EHIP, std::string("hipMalloc graph workspace: ") +
+                                                                     hipGetErrorString(e));
+  }
+  pr.a.ws = reinterpret_cast<double2*>(g->ws);
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    pr.v.launch(pr.a, (int)pr.slots, s);
+    hipError_t le = hipGetLastError();
+    e = hipStreamEndCapture(s, &g->graph);
+    if (e == hipSuccess && le != hipSuccess) e = le;
+  }
+  if (e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    rgf_graph_destroy(g);
+    return fail(RGF_EHIP, std::string("rgf_graph_create: ") + hipGetErrorString(e));
+  }
+  *out = g;
+  return 0;
+}
+
+int rgf_graph_launch(rgf_graph g, int replays, void* stream) {
+  g_err.clear();
+  if (!g || !g->exec) return fail(RGF_EINVAL, "rgf_graph_launch: invalid graph");
+  if (replays < 0) return fail(RGF_EINVAL, "rgf_graph_launch: replays < 0");
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  for (int i = 0; i < replays; ++i) HIPCHK(hipGraphLaunch(g->exec, s));
+  return 0;
+}
+
+void rgf_graph_destroy(rgf_graph g) {
+  if (!g) return;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  if (g->ws) (void)hipFree(g->ws);
+  delete g;
 }
 
 // Host-array solve of nE energies: stage (and pad) inputs in HBM, run the

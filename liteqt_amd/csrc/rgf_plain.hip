@@ -185,6 +185,8 @@ int rgf_plain_solve(const rgf_params* p, const rgf_batch* b, void* stream) {
   g_err.clear();
   if (!p || !b || !b->E || !b->Hii || !b->H1i) return fail(RGF_EINVAL, "missing arguments");
   if (b->nx < 2 || b->nm < 1) return fail(RGF_EINVAL, "need nm >= 1, nx >= 2");
+  if (b->sigR_lo || b->sigR_up || b->sigL_up)
+    return fail(RGF_EUNSUP, "the plain path takes diagonal Sigma blocks only");
   std::lock_guard<std::mutex> lk(g_mu);
   Ctx& c = g_ctx;
   c.s = reinterpret_cast<hipStream_t>(stream);

@@ -42,6 +42,16 @@ struct SweepArgs {
   int* info;
   double2* ws;
   long long ws_slot;  // complex elements of workspace per slot
+  // block-tridiagonal scattering self-energy (rgf_offdiag.hpp only): block l
+  // of sigR_lo / sigR_up / sigL_up is Sigma^R_{l,l-1} / Sigma^R_{l-1,l} /
+  // Sigma^<_{l-1,l}, l = 1..nx-1; any may be null
+  const double2* sigR_lo;
+  const double2* sigR_up;
+  const double2* sigL_up;
+  // in-place elastic SCBA refresh (rgf_offdiag.hpp only):
+  // Sigma <- (1 - scba_mix) Sigma + scba_mix * scba_D * G on every block given
+  int scba;
+  double scba_D, scba_mix;
 };
 
 __device__ __forceinline__ double ferm_dev(double a) { return 1.0 / (1.0 + exp(a)); }

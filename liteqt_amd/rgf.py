@@ -47,13 +47,19 @@ class Batch(ctypes.Structure):
                 ("ldos", ctypes.c_void_p), ("ndens", ctypes.c_void_p), ("pdens", ctypes.c_void_p),
                 ("GRi1", ctypes.c_void_p), ("GLi1", ctypes.c_void_p), ("GGi1", ctypes.c_void_p),
                 ("cur", ctypes.c_void_p), ("tr", ctypes.c_void_p), ("tre", ctypes.c_void_p),
-                ("dens_n", ctypes.c_void_p), ("dens_p", ctypes.c_void_p), ("info", ctypes.c_void_p)]
+                ("dens_n", ctypes.c_void_p), ("dens_p", ctypes.c_void_p), ("info", ctypes.c_void_p),
+                ("sigR_lo", ctypes.c_void_p), ("sigR_up", ctypes.c_void_p), ("sigL_up", ctypes.c_void_p)]
+
+
+class Scba(ctypes.Structure):
+    _fields_ = [("D", ctypes.c_double), ("mix", ctypes.c_double)]
 
 
 # every function declared in include/rgf.h
 EXPORTS = ("rgf_params_reference", "rgf_last_error", "rgf_version", "rgf_rs", "rgf_rs_ex",
            "rgf_rs_energies", "rgf_solve_batched", "rgf_workspace_bytes", "rgf_release",
-           "rgf_zgemm_batched", "rgf_zinv_batched", "rgf_adjoint_batched", "rgf_adjoint_workspace_bytes")
+           "rgf_zgemm_batched", "rgf_zinv_batched", "rgf_adjoint_batched", "rgf_adjoint_workspace_bytes",
+           "rgf_scba_step", "rgf_graph_create", "rgf_graph_launch", "rgf_graph_destroy")
 
 _lib = None
 
@@ -75,6 +81,12 @@ def lib():
     L.rgf_rs_energies.argtypes = ([ctypes.POINTER(Params), vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp,
                                    vp, vp, vp, vp, vp, ctypes.c_int] + [vp] * 10)
     L.rgf_solve_batched.argtypes = [ctypes.POINTER(Params), ctypes.POINTER(Batch), vp]
+    L.rgf_scba_step.argtypes = [ctypes.POINTER(Params), ctypes.POINTER(Batch), ctypes.POINTER(Scba), vp]
+    L.rgf_graph_create.argtypes = [ctypes.POINTER(Params), ctypes.POINTER(Batch), ctypes.POINTER(Scba), vp,
+                                   ctypes.POINTER(vp)]
+    L.rgf_graph_launch.argtypes = [vp, ctypes.c_int, vp]
+    L.rgf_graph_destroy.argtypes = [vp]
+    L.rgf_graph_destroy.restype = None
     L.rgf_zgemm_batched.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, vp, vp,
                                     ctypes.c_int, vp]
     L.rgf_zinv_batched.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp, vp, vp]
@@ -218,25 +230,21 @@ def plain_lib():
 
 
 def solve_batched(p, E, temp, mu, Hii, H1i, sigL=None, sigR=None, out=None, max_slots=0, stream=None,
-                  plain=False):
+                  plain=False, sigR_lo=None, sigR_up=None, sigL_up=None, scba=None):
     """Device-resident batched solve (plain=True: the per-energy rocBLAS/rocSOLVER
     comparison path with the same arguments).  E, Hii, H1i, sig*, out[...] are torch
     tensors on the HIP device: complex blocks as complex128 tensors of shape
     (nE, nx, nm, nm) holding column-major blocks (i.e. the memory of a Fortran
     (nm,nm,nx,nE) array; use to_device_blocks()).  out maps names in
     {ldos, ndens, pdens, GRi1, GLi1, GGi1, cur, tr, tre, dens_n, dens_p, info}
-    to preallocated tensors.  Asynchronous on `stream` (default: torch's current
-    stream)."""
+    to preallocated tensors.  sigR_lo / sigR_up / sigL_up: first off-diagonal
+    Sigma blocks (physical mode).  scba=(D, mix): one SCBA iteration that also
+    refreshes the given Sigma tensors in place (rgf_scba_step).  Asynchronous on
+    `stream` (default: torch's current stream)."""
     import torch
     out = out or {}
-    b = _batch(torch, E, temp, mu, Hii, H1i, sigL, sigR, max_slots)
-    nE, nx, nm, dev = b.nE, b.nx, b.nm, E.device
-    blk = nE * nx * nm * nm
-    sizes = {"c": (torch.complex128, blk), "r": (torch.float64, blk), "e": (torch.float64, nE),
-             "x": (torch.float64, nE * nx), "i": (torch.int32, 4 * nE)}
-    for k, kind in _OUT_KINDS.items():
-        dt, cnt = sizes[kind]
-        setattr(b, k, _req(out.get(k), k, dt, cnt, dev))
+    b = _batch(torch, E, temp, mu, Hii, H1i, sigL, sigR, max_slots, sigR_lo, sigR_up, sigL_up)
+    _out_batch(torch, b, out, E.device)
     if stream is None:
         stream = torch.cuda.current_stream()
     if plain:
@@ -244,12 +252,61 @@ def solve_batched(p, E, temp, mu, Hii, H1i, sigL=None, sigR=None, out=None, max_
         if rc != 0:
             raise RgfError(f"plain path error {rc}: {plain_lib().rgf_plain_last_error().decode()}")
         return out
-    rc = lib().rgf_solve_batched(ctypes.byref(p), ctypes.byref(b), ctypes.c_void_p(stream.cuda_stream))
+    if scba is not None:
+        sc = Scba(float(scba[0]), float(scba[1]))
+        rc = lib().rgf_scba_step(ctypes.byref(p), ctypes.byref(b), ctypes.byref(sc), ctypes.c_void_p(stream.cuda_stream))
+    else:
+        rc = lib().rgf_solve_batched(ctypes.byref(p), ctypes.byref(b), ctypes.c_void_p(stream.cuda_stream))
     _check(rc)
     return out
 
 
-def _batch(torch, E, temp, mu, Hii, H1i, sigL, sigR, max_slots):
+def _out_batch(torch, b, out, dev):
+    nE, nx, nm = b.nE, b.nx, b.nm
+    blk = nE * nx * nm * nm
+    sizes = {"c": (torch.complex128, blk), "r": (torch.float64, blk), "e": (torch.float64, nE),
+             "x": (torch.float64, nE * nx), "i": (torch.int32, 4 * nE)}
+    for k, kind in _OUT_KINDS.items():
+        dt, cnt = sizes[kind]
+        setattr(b, k, _req(out.get(k), k, dt, cnt, dev))
+
+
+class Graph:
+    """A captured batched solve (rgf_graph_create) replayed with launch(n).
+    With scba=(D, mix) each replay is one SCBA iteration that refreshes the
+    Sigma tensors in place.  The tensors passed here must stay alive and in
+    place for the graph's lifetime; `stream` must be a created stream
+    (torch.cuda.Stream())."""
+
+    def __init__(self, p, E, temp, mu, Hii, H1i, stream, sigL=None, sigR=None, out=None, max_slots=0,
+                 sigR_lo=None, sigR_up=None, sigL_up=None, scba=None):
+        import torch
+        self._keep = (E, Hii, H1i, sigL, sigR, sigR_lo, sigR_up, sigL_up, out)
+        b = _batch(torch, E, temp, mu, Hii, H1i, sigL, sigR, max_slots, sigR_lo, sigR_up, sigL_up)
+        _out_batch(torch, b, out or {}, E.device)
+        self._stream = stream
+        h = ctypes.c_void_p()
+        sc = None if scba is None else Scba(float(scba[0]), float(scba[1]))
+        _check(lib().rgf_graph_create(ctypes.byref(p), ctypes.byref(b), None if sc is None else ctypes.byref(sc),
+                                      ctypes.c_void_p(stream.cuda_stream), ctypes.byref(h)))
+        self._h = h
+
+    def launch(self, replays=1):
+        _check(lib().rgf_graph_launch(self._h, int(replays), ctypes.c_void_p(self._stream.cuda_stream)))
+
+    def close(self):
+        if self._h:
+            lib().rgf_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _batch(torch, E, temp, mu, Hii, H1i, sigL, sigR, max_slots, sigR_lo=None, sigR_up=None, sigL_up=None):
     """rgf_batch with the inputs checked: E float64 (nE,), Hii/H1i complex128
     (nx, nm, nm) column-major blocks, sig* (nx, nm, nm) shared or (nE, nx, nm, nm)."""
     dev = E.device
@@ -264,11 +321,14 @@ def _batch(torch, E, temp, mu, Hii, H1i, sigL, sigR, max_slots):
     b.Hii = _req(Hii, "Hii", torch.complex128, blk, dev)
     b.H1i = _req(H1i, "H1i", torch.complex128, blk, dev)
     stride = 0
-    for s in (sigL, sigR):
-        if s is not None and s.dim() == 4:
+    sigs = {"sigL": sigL, "sigR": sigR, "sigR_lo": sigR_lo, "sigR_up": sigR_up, "sigL_up": sigL_up}
+    for k, t in sigs.items():
+        if t is not None and t.dim() == 4:
             stride = blk
-    b.sigL = _req(sigL, "sigL", torch.complex128, blk + stride * (nE - 1), dev)
-    b.sigR = _req(sigR, "sigR", torch.complex128, blk + stride * (nE - 1), dev)
+    for k, t in sigs.items():
+        if t is not None and stride and t.dim() != 4:
+            raise RgfError(f"{k}: all Sigma tensors must be per energy (nE, nx, nm, nm) when one is")
+        setattr(b, k, _req(t, k, torch.complex128, blk + stride * (nE - 1), dev))
     b.sig_stride = stride
     return b
 

@@ -220,3 +220,35 @@ def rgf_fixed_leads(p, temp, E, mu, Hii, H1i, sigL=None, sigR=None, leadL_H00=No
                           _p(leadL_H00), _p(leadR_H00), _p(ndens), _p(ldos), ctypes.cast(ctypes.byref(tr), dp),
                           ctypes.cast(ctypes.byref(tre), dp), None)
     return {"ndens": ndens, "ldos": ldos, "tr": tr.value, "tre": tre.value}
+
+
+def rgf_offdiag(p, temp, E, mu, Hii, H1i, sigL=None, sigR=None, sigR_lo=None, sigR_up=None, sigL_up=None):
+    """orc_rgf_offdiag: physical-mode RGF with block-tridiagonal Sigma (one energy).
+    sigR_lo[..., l] = Sigma^R_{l,l-1}, sigR_up[..., l] = Sigma^R_{l-1,l},
+    sigL_up[..., l] = Sigma^<_{l-1,l} (l >= 1).  Returns the physical-mode outputs
+    of rgf() plus GRi1_lo[..., l] = G^R_{l+1,l}."""
+    L = lib()
+    if not hasattr(L, "_od"):
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.orc_rgf_offdiag.argtypes = ([ctypes.POINTER(OrcParams), dp, ctypes.c_int, ctypes.c_int, ctypes.c_double, dp]
+                                      + [dp] * 17 + [ctypes.POINTER(OrcDiag)])
+        L._od = True
+    Hii, H1i = _cf(Hii), _cf(H1i)
+    nm, _, nx = Hii.shape
+    sigs = [None if s is None else _cf(s) for s in (sigL, sigR, sigR_lo, sigR_up, sigL_up)]
+    shp = (nm, nm, nx)
+    out = {k: np.zeros(shp, np.complex128, order="F") for k in ("ndens", "pdens", "ldos", "GRi1", "GLi1", "GGi1",
+                                                               "GRi1_lo")}
+    out["cur"] = np.zeros(shp, np.float64, order="F")
+    tr, tre = ctypes.c_double(0.0), ctypes.c_double(0.0)
+    d = OrcDiag()
+    dp = ctypes.POINTER(ctypes.c_double)
+    t = np.asarray(temp, np.float64)
+    m = np.asarray(mu, np.float64)
+    L.orc_rgf_offdiag(ctypes.byref(p), _p(t), nm, nx, float(E), _p(m), _p(Hii), _p(H1i), *[_p(s) for s in sigs],
+                      _p(out["ndens"]), _p(out["pdens"]), _p(out["ldos"]), ctypes.cast(ctypes.byref(tr), dp),
+                      ctypes.cast(ctypes.byref(tre), dp), _p(out["cur"]), _p(out["GRi1"]), _p(out["GLi1"]),
+                      _p(out["GGi1"]), _p(out["GRi1_lo"]), ctypes.byref(d))
+    out["tr"], out["tre"] = tr.value, tre.value
+    out["diag"] = (d.iters_left, d.iters_right, d.nmax_hit, d.inv_failures)
+    return out

@@ -600,3 +600,184 @@ void orc_rgf_energies(const orc_params *p, const double temp[2], int nm, int nx,
             diag ? diag + e : NULL);
   }
 }
+
+/* --------------------------------------------- block-tridiagonal Sigma */
+/* d = a + s*b^H (b may be NULL) */
+static void maddh(int n, double *d, const double *a, double s, const double *b) {
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) {
+      size_t p = (size_t)i + (size_t)j * n, q = (size_t)j + (size_t)i * n;
+      double br = b ? RE(b, q) : 0.0, bi = b ? -IM(b, q) : 0.0;
+      RE(d, p) = RE(a, p) + s * br;
+      IM(d, p) = IM(a, p) + s * bi;
+    }
+}
+/* d = a + m - m^H (adds the anti-Hermitian part of m twice over; d may alias a) */
+static void maddanti(int n, double *d, const double *a, const double *m) {
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) {
+      size_t p = (size_t)i + (size_t)j * n, q = (size_t)j + (size_t)i * n;
+      RE(d, p) = RE(a, p) + RE(m, p) - RE(m, q);
+      IM(d, p) = IM(a, p) + IM(m, p) + IM(m, q);
+    }
+}
+
+void orc_rgf_offdiag(const orc_params *p, const double temp[2], int nm, int nx, double E,
+                     const double mu[2], const double *Hii, const double *H1i,
+                     const double *sigL, const double *sigR, const double *sigR_lo,
+                     const double *sigR_up, const double *sigL_up, double *ndens,
+                     double *pdens, double *ldos, double *tr, double *tre, double *cur,
+                     double *GRi1, double *GLi1, double *GGi1, double *GRi1_lo, orc_diag *diag) {
+  const int n = nm, N = nx;
+  const size_t bs = 2 * (size_t)n * n, nn = (size_t)n * n;
+  const double zr = E, zi = p->eta_dev;
+  orc_diag dg = {0, 0, 0, 0};
+  double *g = (double *)calloc(bs * (size_t)N, sizeof(double));
+  double *gn = (double *)calloc(bs * (size_t)N, sizeof(double));
+  double *H00 = mnew(n), *Th = mnew(n), *G00 = mnew(n), *A = mnew(n), *B = mnew(n), *C = mnew(n);
+  double *S = mnew(n), *V = mnew(n), *d = mnew(n), *u = mnew(n), *sigmal = mnew(n), *sigmar = mnew(n);
+  double *GamL = mnew(n), *GamR = mnew(n), *G = mnew(n), *Gn = mnew(n), *Q = mnew(n), *R = mnew(n);
+  double *Glo = mnew(n), *M = mnew(n), *gs = mnew(n), *QGn = mnew(n), *Rd = mnew(n), *Gnn = mnew(n);
+  double *Gll = mnew(n), *Gp = mnew(n), *Z0 = mnew(n); /* Z0 stays zero */
+#define SB(a, l) ((a) ? blk((a), n, (l)) : NULL)
+  /* leads: onsite Hii + sigR at the edge slabs, couplings H1i(:,:,1), H1i(:,:,nx) */
+  if (sigR) madd(n, H00, blk(Hii, n, 0), blk(sigR, n, 0)); else mcopy(n, H00, blk(Hii, n, 0));
+  maddh(n, Th, G00, 1.0, blk(H1i, n, 0)); /* G00 is zero here: Th = H1i_1^H */
+  {
+    int hit = 0;
+    dg.iters_left = orc_sancho(n, E, p->eta_lead, p->dec_tol, p->dec_nmax, H00, Th, G00, NULL, &hit,
+                               &dg.inv_failures);
+    if (hit) dg.nmax_hit |= 1;
+  }
+  orc_zgemm('n', 'n', n, blk(H1i, n, 0), G00, A);
+  orc_zgemm('n', 'c', n, A, blk(H1i, n, 0), sigmal);
+  if (sigR) madd(n, H00, blk(Hii, n, N - 1), blk(sigR, n, N - 1)); else mcopy(n, H00, blk(Hii, n, N - 1));
+  {
+    int hit = 0;
+    dg.iters_right = orc_sancho(n, E, p->eta_lead, p->dec_tol, p->dec_nmax, H00, blk(H1i, n, N - 1), G00,
+                                NULL, &hit, &dg.inv_failures);
+    if (hit) dg.nmax_hit |= 2;
+  }
+  orc_zgemm('c', 'n', n, blk(H1i, n, N - 1), G00, A);
+  orc_zgemm('n', 'n', n, A, blk(H1i, n, N - 1), sigmar);
+  const double fL = orc_ferm((E - mu[0]) / (p->boltz * temp[0]));
+  const double fR = orc_ferm((E - mu[1]) / (p->boltz * temp[1]));
+  mantiherm(n, GamL, sigmal);
+  mantiherm(n, GamR, sigmar);
+  for (size_t k = 0; k < bs; ++k) { GamL[k] = -GamL[k]; GamR[k] = -GamR[k]; } /* i Gamma */
+
+  /* forward: g_l = (z - H_l - SR_l - d_l g_{l-1} u_l)^-1,
+   *          g<_l = g_l (S<_l + d_l g<_{l-1} d_l^H + V - V^H) g_l^H, V = d_l g_{l-1} s_l */
+  for (int l = 0; l < N; ++l) {
+    double *gg = (l == N - 1) ? G : blkw(g, n, l);
+    double *gng = (l == N - 1) ? Gn : blkw(gn, n, l);
+    mzminus(n, A, zr, zi, blk(Hii, n, l), SB(sigR, l));
+    for (size_t k = 0; k < bs; ++k) S[k] = sigL ? blk(sigL, n, l)[k] : 0.0;
+    if (l == 0)
+      for (size_t k = 0; k < bs; ++k) { A[k] -= sigmal[k]; S[k] += fL * GamL[k]; }
+    if (l == N - 1)
+      for (size_t k = 0; k < bs; ++k) { A[k] -= sigmar[k]; S[k] += fR * GamR[k]; }
+    if (l > 0) {
+      if (sigR_lo) madd(n, d, blk(H1i, n, l), blk(sigR_lo, n, l)); else mcopy(n, d, blk(H1i, n, l));
+      maddh(n, u, sigR_up ? blk(sigR_up, n, l) : Z0, 1.0, blk(H1i, n, l));
+      orc_zgemm('n', 'n', n, d, blk(g, n, l - 1), B);  /* d g */
+      orc_zgemm('n', 'n', n, B, u, C);
+      for (size_t k = 0; k < bs; ++k) A[k] -= C[k];
+      orc_zgemm('n', 'n', n, d, blk(gn, n, l - 1), C);
+      orc_zgemm('n', 'c', n, C, d, V);
+      madd(n, S, S, V);
+      if (sigL_up) {
+        orc_zgemm('n', 'n', n, B, blk(sigL_up, n, l), V);
+        maddanti(n, S, S, V);
+      }
+    }
+    if (orc_zinv(n, A)) ++dg.inv_failures;
+    mcopy(n, gg, A);
+    orc_zgemm('n', 'n', n, A, S, B);
+    orc_zgemm('n', 'c', n, B, A, gng);
+  }
+  mantiherm(n, Gp, G);
+  madd(n, Gp, Gn, Gp);
+  if (ldos) mcopy(n, blkw(ldos, n, N - 1), G);
+  if (ndens) mcopy(n, blkw(ndens, n, N - 1), Gn);
+  if (pdens) mcopy(n, blkw(pdens, n, N - 1), Gp);
+  if (cur) memset(cur + nn * (size_t)(N - 1), 0, nn * sizeof(double));
+  {
+    /* tr = -Re Tr[iGam_R fR G>_NN - iGam_R (fR - 1) G<_NN] */
+    orc_zgemm('n', 'n', n, GamR, Gp, B);
+    orc_zgemm('n', 'n', n, GamR, Gn, C);
+    double t = 0.0;
+    for (int i = 0; i < n; ++i) {
+      size_t k = (size_t)i + (size_t)i * n;
+      t -= fR * RE(B, k) - (fR - 1.0) * RE(C, k);
+    }
+    if (tr) *tr = t;
+  }
+  /* backward, pair (l, l+1): u = H_{l,l+1} + SR_{l,l+1}, d = H_{l+1,l} + SR_{l+1,l}, s = S<_{l,l+1};
+   * Q = g u, R = Q G = G_{l,l+1}, G_{l+1,l} = G d g,
+   * G<_{l,l+1} = g< d^H G^H + g s G^H + Q G<,
+   * G_ll = g + R d g, G<_ll = g< + Q G< Q^H + (R d g< - h.c.) + (g s R^H - h.c.) */
+  for (int l = N - 2; l >= 0; --l) {
+    const double *gl = blk(g, n, l), *gln = blk(gn, n, l), *T = blk(H1i, n, l + 1);
+    if (sigR_lo) madd(n, d, T, blk(sigR_lo, n, l + 1)); else mcopy(n, d, T);
+    maddh(n, u, sigR_up ? blk(sigR_up, n, l + 1) : Z0, 1.0, T);
+    orc_zgemm('n', 'n', n, gl, u, Q);
+    orc_zgemm('n', 'n', n, Q, G, R);
+    orc_zgemm('n', 'n', n, G, d, B);
+    orc_zgemm('n', 'n', n, B, gl, Glo);
+    orc_zgemm('n', 'n', n, Q, Gn, QGn);
+    if (sigL_up) orc_zgemm('n', 'n', n, gl, blk(sigL_up, n, l + 1), gs); else mzero(n, gs);
+    orc_zgemm('n', 'c', n, gln, d, M);
+    madd(n, M, M, gs);
+    orc_zgemm('n', 'c', n, M, G, B);
+    madd(n, Gnn, B, QGn); /* G<_{l,l+1} */
+    if (GRi1) mcopy(n, blkw(GRi1, n, l), R);
+    if (GRi1_lo) mcopy(n, blkw(GRi1_lo, n, l), Glo);
+    if (GLi1) mcopy(n, blkw(GLi1, n, l), Gnn);
+    if (GGi1) {
+      double *o = blkw(GGi1, n, l);
+      madd(n, o, Gnn, R);
+      maddh(n, o, o, -1.0, Glo);
+    }
+    if (cur) {
+      /* Re(H_{l,l+1} G<_{l+1,l}) - Re(H_{l+1,l} G<_{l,l+1}), G<_{l+1,l} = -(G<_{l,l+1})^H */
+      orc_zgemm('c', 'c', n, T, Gnn, A);
+      orc_zgemm('n', 'n', n, T, Gnn, B);
+      double *cl = cur + nn * (size_t)l;
+      for (size_t k = 0; k < nn; ++k) cl[k] = -RE(A, k) - RE(B, k);
+    }
+    orc_zgemm('n', 'n', n, R, d, Rd);
+    orc_zgemm('n', 'n', n, Rd, gl, A);
+    madd(n, Gll, gl, A);
+    orc_zgemm('n', 'c', n, QGn, Q, A);
+    madd(n, B, gln, A);
+    orc_zgemm('n', 'n', n, Rd, gln, A);
+    maddanti(n, B, B, A);
+    orc_zgemm('n', 'c', n, gs, R, A);
+    maddanti(n, B, B, A);
+    mcopy(n, G, Gll);
+    mcopy(n, Gn, B);
+    mantiherm(n, Gp, G);
+    madd(n, Gp, Gn, Gp);
+    if (ldos) mcopy(n, blkw(ldos, n, l), G);
+    if (ndens) mcopy(n, blkw(ndens, n, l), Gn);
+    if (pdens) mcopy(n, blkw(pdens, n, l), Gp);
+  }
+  {
+    /* tre = Re Tr[iGam_L fL G>_11 - iGam_L (fL - 1) G<_11] */
+    orc_zgemm('n', 'n', n, GamL, Gp, B);
+    orc_zgemm('n', 'n', n, GamL, Gn, C);
+    double t = 0.0;
+    for (int i = 0; i < n; ++i) {
+      size_t k = (size_t)i + (size_t)i * n;
+      t += fL * RE(B, k) - (fL - 1.0) * RE(C, k);
+    }
+    if (tre) *tre = t;
+  }
+#undef SB
+  if (diag) *diag = dg;
+  free(g); free(gn);
+  free(H00); free(Th); free(G00); free(A); free(B); free(C); free(S); free(V); free(d); free(u);
+  free(sigmal); free(sigmar); free(GamL); free(GamR); free(G); free(Gn); free(Q); free(R); free(Glo);
+  free(M); free(gs); free(QGn); free(Rd); free(Gnn); free(Gll); free(Gp); free(Z0);
+}

@@ -88,6 +88,24 @@ void orc_rgf_fixed_leads(const orc_params *p, const double temp[2], int nm, int 
                          const double *leadR_H00, double *ndens, double *ldos, double *tr,
                          double *tre, orc_diag *diag);
 
+/* Physical-mode RGF with block-tridiagonal scattering self-energies (an
+ * extension beyond green_RGF_RS, whose sigma arrays are diagonal blocks only,
+ * src/green_rgf.f90:3952).  sigR, sigL: diagonal blocks as in orc_rgf;
+ * sigR_lo[l] = Sigma^R_{l,l-1}, sigR_up[l] = Sigma^R_{l-1,l},
+ * sigL_up[l] = Sigma^<_{l-1,l} for l = 1..nx-1 (0-based; block 0 unused),
+ * Sigma^<_{l,l-1} = -(Sigma^<_{l-1,l})^H.  Any of them may be NULL (zero).
+ * The leads are built from Hii + sigR at the edge slabs and H1i(:,:,1),
+ * H1i(:,:,nx) as in orc_rgf.  Outputs as orc_rgf (physical mode) plus
+ * GRi1_lo[l] = G^R_{l+1,l}; every pointer may be NULL.  Product order is free
+ * (no compat arithmetic to reproduce).  Pinned by the dense inverse of the
+ * whole open device with the same Sigma. */
+void orc_rgf_offdiag(const orc_params *p, const double temp[2], int nm, int nx, double E,
+                     const double mu[2], const double *Hii, const double *H1i,
+                     const double *sigL, const double *sigR, const double *sigR_lo,
+                     const double *sigR_up, const double *sigL_up, double *ndens,
+                     double *pdens, double *ldos, double *tr, double *tre, double *cur,
+                     double *GRi1, double *GLi1, double *GGi1, double *GRi1_lo, orc_diag *diag);
+
 /* Energy loop of the reference drivers (green_rgf.f90:1305-1316): OpenMP over
  * energies, one energy per thread.  Per-energy arrays have a trailing energy
  * index (stride nm*nm*nx complex); sig_stride_e is the energy stride of the

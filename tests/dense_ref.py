@@ -30,12 +30,16 @@ def lead_self_energies(sancho, E, Hii, H1i, sigR, eta_lead, tol, nmax):
     return tL @ GL @ dag(tL), dag(tR) @ GR @ tR
 
 
-def dense_rgf(E, eta_dev, Hii, H1i, SigL, SigR, fL, fR, sigL=None, sigR=None, couplings=None):
+def dense_rgf(E, eta_dev, Hii, H1i, SigL, SigR, fL, fR, sigL=None, sigR=None, couplings=None, sig_off=None):
     """All blocks of G^R, G^<, G^> for the open device.
 
     couplings[l] (l = 1..nx-1) is H_{l,l-1}; default H1i[:, :, l] (the
-    physically consistent indexing).  Returns a dict with the reference's
-    output names plus 'G', 'Gl', 'Gg' (full matrices) and 'tr', 'tre'."""
+    physically consistent indexing).  sig_off (optional) holds first
+    off-diagonal scattering blocks: 'R_lo'[..., l] = Sigma^R_{l,l-1},
+    'R_up'[..., l] = Sigma^R_{l-1,l}, 'L_up'[..., l] = Sigma^<_{l-1,l}
+    (Sigma^<_{l,l-1} = -(Sigma^<_{l-1,l})^H).  Returns a dict with the
+    reference's output names plus 'GRi1_lo' (G^R_{l+1,l}), 'G', 'Gl', 'Gg'
+    (full matrices) and 'tr', 'tre'."""
     nm, _, nx = Hii.shape
     N = nm * nx
     if couplings is None:
@@ -48,6 +52,10 @@ def dense_rgf(E, eta_dev, Hii, H1i, SigL, SigR, fL, fR, sigL=None, sigR=None, co
             sp = slice((l - 1) * nm, l * nm)
             H[s, sp] = couplings[l]
             H[sp, s] = dag(couplings[l])
+            if sig_off is not None and sig_off.get("R_lo") is not None:
+                H[s, sp] += sig_off["R_lo"][:, :, l]
+            if sig_off is not None and sig_off.get("R_up") is not None:
+                H[sp, s] += sig_off["R_up"][:, :, l]
     A = (E + 1j * eta_dev) * np.eye(N) - H
     A[:nm, :nm] -= SigL
     A[N - nm:, N - nm:] -= SigR
@@ -57,6 +65,11 @@ def dense_rgf(E, eta_dev, Hii, H1i, SigL, SigR, fL, fR, sigL=None, sigR=None, co
         for l in range(nx):
             s = slice(l * nm, (l + 1) * nm)
             Sl[s, s] = sigL[:, :, l]
+    if sig_off is not None and sig_off.get("L_up") is not None:
+        for l in range(1, nx):
+            s, sp = slice(l * nm, (l + 1) * nm), slice((l - 1) * nm, l * nm)
+            Sl[sp, s] = sig_off["L_up"][:, :, l]
+            Sl[s, sp] = -dag(sig_off["L_up"][:, :, l])
     GamL = -(SigL - dag(SigL))  # i*Gamma_L
     GamR = -(SigR - dag(SigR))
     Sl[:nm, :nm] += GamL * fL
@@ -79,6 +92,10 @@ def dense_rgf(E, eta_dev, Hii, H1i, SigL, SigR, fL, fR, sigL=None, sigR=None, co
         for l in range(nx - 1):
             o[:, :, l] = b(M, l, l + 1)
         out[key] = o
+    o = np.zeros(shp, np.complex128, order="F")
+    for l in range(nx - 1):
+        o[:, :, l] = b(G, l + 1, l)
+    out["GRi1_lo"] = o
     cur = np.zeros(shp, np.float64, order="F")
     for l in range(nx - 1):
         Hc = couplings[l + 1]

@@ -56,3 +56,29 @@ def test_tensor_checks_reject_wrong_dtype_layout_and_size():
                      (torch.zeros(3, dtype=torch.float64), "elements")):
         with pytest.raises(rgf.RgfError, match=why):
             rgf._req(bad, "w", torch.float64, 4, cpu)
+
+
+def test_offdiag_and_scba_argument_errors_before_any_device_call():
+    """Argument errors of the block-tridiagonal-Sigma and SCBA entry points are
+    reported before the device is touched (dummy pointers are never read)."""
+    from liteqt_amd import rgf as r
+    L = r.lib()
+    dummy = ctypes.c_void_p(16)
+    b = r.Batch()
+    b.nm, b.nx, b.nE = 32, 4, 2
+    b.E = b.Hii = b.H1i = dummy.value
+    b.sigR_lo = dummy.value
+    p = r.params(r.MODE_COMPAT)
+    assert L.rgf_solve_batched(ctypes.byref(p), ctypes.byref(b), None) == 1  # RGF_EINVAL: needs physical
+    assert b"PHYSICAL" in L.rgf_last_error()
+    p = r.params(r.MODE_PHYSICAL)
+    sc = r.Scba(1e-3, 0.5)
+    b.sigR_lo = None
+    assert L.rgf_scba_step(ctypes.byref(p), ctypes.byref(b), ctypes.byref(sc), None) == 1  # sigR/sigL missing
+    b.sigR = b.sigL = dummy.value
+    b.sig_stride = 0
+    assert L.rgf_scba_step(ctypes.byref(p), ctypes.byref(b), ctypes.byref(sc), None) == 1  # shared Sigma
+    assert b"per energy" in L.rgf_last_error()
+    h = ctypes.c_void_p()
+    assert L.rgf_graph_create(ctypes.byref(p), ctypes.byref(b), None, None, ctypes.byref(h)) == 1
+    assert b"stream" in L.rgf_last_error() and not h.value
