@@ -57,6 +57,12 @@ def kernel_source_hash():
     return h.hexdigest()[:16]
 
 
+def algorithmic_flops_per_energy(nm, nx, info):
+    """8 nm^3 [17 nx + 7 (k_L + k_R)] per energy (see algorithmic_flops)."""
+    info = np.asarray(info)
+    return 8.0 * nm ** 3 * (17 * nx + 7 * (info[:, 0] + info[:, 1]))
+
+
 def algorithmic_flops(nm, nx, iters):
     """Flops of the step's output set per energy: 8 nm^3 [17 nx + 7 (kL + kR)]
     (forward 6 GEMM + 1 inverse, backward 10 GEMM per slab; 1 inverse + 6 GEMM
@@ -137,6 +143,157 @@ def adjoint_cost(rgf, torch, dev, stream, nE):
             "current_e_over_h": res["current_e_over_h"], "max_abs_dI_dU": res["max_abs_dI_dU"]}
 
 
+def scba_cost(rgf, torch, dev, nE, iters):
+    """SCBA step on the block-tridiagonal-Sigma path: random device nm = 64,
+    nx = 100, nE energies in [-3, 3] eV, eta 1e-4, Fermi leads mu = +-0.1 eV,
+    kT = 0.0259 eV; diagonal and first off-diagonal Sigma^R / Sigma^< blocks of
+    size ~1e-3 (Sigma^R = Lambda - i Gamma/2, Sigma^< = i f Gamma), per energy,
+    resident in HBM.  One captured launch (HIP graph) solves every energy and
+    refreshes Sigma in place (elastic, Sigma <- (1-mix) Sigma + mix D G); it is
+    replayed `iters` times."""
+    from liteqt_amd import configs
+    rgf.lib().rgf_release()
+    torch.cuda.empty_cache()
+    nm, nx = 64, 100
+    Hii, H1i = configs.random_device(nm, nx, 5)
+    Gd, Glo, Ld, Llo = [rgf.to_device_blocks(a, dev) for a in configs.scattering_blocks(nm, nx, 6, scale=1e-3)]
+    E = configs.energy_grid(-3.0, 3.0, nE)
+    kT = 0.0259
+    T = kT / 8.61734e-05
+    temp, mu = (T, T), (0.1, -0.1)
+    f = torch.tensor(1.0 / (1.0 + np.exp(E / kT)), dtype=torch.float64, device=dev).to(torch.complex128)
+    H = lambda a: a.conj().transpose(-1, -2)
+    ex = lambda a: a.unsqueeze(0).expand(nE, *a.shape).contiguous()
+    fe = f.view(nE, 1, 1, 1)
+    sig = {"sigR": ex(Ld - 0.5j * Gd), "sigR_lo": ex(Llo - 0.5j * Glo), "sigR_up": ex(H(Llo) - 0.5j * H(Glo))}
+    sig["sigL"] = ex(1j * Gd) * fe
+    sig["sigL_up"] = ex(1j * H(Glo)) * fe
+    del Gd, Glo, Ld, Llo
+    p = rgf.params(rgf.MODE_PHYSICAL, eta_dev=1e-4, eta_lead=1e-4, dec_tol=1e-12, dec_nmax=100)
+    Hd, Cd = rgf.to_device_blocks(Hii, dev), rgf.to_device_blocks(H1i, dev)
+    Ed = torch.tensor(E, dtype=torch.float64, device=dev)
+    out = {"tr": torch.zeros(nE, dtype=torch.float64, device=dev),
+           "dens_n": torch.zeros(nE, nx, dtype=torch.float64, device=dev),
+           "info": torch.zeros(nE, 4, dtype=torch.int32, device=dev)}
+    D, mix = 1e-3, 0.5
+    stream = torch.cuda.Stream()
+    t0 = time.perf_counter()
+    g = rgf.Graph(p, Ed, temp, mu, Hd, Cd, stream, out=out, scba=(D, mix), **sig)
+    t_capture = time.perf_counter() - t0
+    current = []
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    dE = float(E[1] - E[0])
+    with torch.cuda.stream(stream):
+        ev0.record(stream)
+        for _ in range(iters):
+            g.launch(1)
+            current.append((out["tr"].sum() * dE).clone())  # Landauer current per iteration, e/h units
+        ev1.record(stream)
+    stream.synchronize()
+    ms = ev0.elapsed_time(ev1)
+    info = out["info"].cpu().numpy()
+    g.close()
+    cur = [float(c.item()) for c in current]
+    n3 = 8.0 * nm ** 3
+    flops = float(np.sum(n3 * (21 * nx + 7 * (info[:, 0] + info[:, 1]))))
+    achieved = flops / (ms * 1e-3 / iters) / 1e12
+    print(f"bench: scba {iters} graph replays (nm={nm} nx={nx} nE={nE}) {ms / 1e3:.2f}s", file=sys.stderr, flush=True)
+    del sig, out
+    torch.cuda.empty_cache()
+    return {"workload": f"random device nm={nm} nx={nx}, {nE} energies in [-3,3] eV, eta 1e-4, Fermi leads; "
+                        "diagonal + first off-diagonal Sigma^R/Sigma^< (~1e-3) per energy in HBM; elastic SCBA "
+                        f"refresh D={D} mix={mix} in place; one HIP graph replayed per iteration",
+            "iterations": iters, "s_per_iteration": ms / 1e3 / iters, "capture_s": t_capture,
+            "energy_points_per_s": nE * iters / (ms / 1e3), "achieved_tflops": achieved,
+            "frac": achieved / PEAK_FP64_MFMA, "flops_per_iteration": flops,
+            "current_e_over_h_per_iteration": cur, "singular": int(info[:, 3].sum())}
+
+
+def random_device_torch(torch, nm, nx, gen, dev, u_range=0.3, coupling_sigma=0.5):
+    """Random device generated on the GPU with the distribution of
+    liteqt_amd.configs.random_device: H_ll = (A + A^H)/(2 sqrt(nm)) + U_l I with
+    A_ij ~ CN(0,1), U_l ~ U(-u, u); H1i_l ~ CN(0, sigma^2/nm) i.i.d.; the last
+    slab repeats the first (onsite and coupling), so both leads are the same
+    material.  Column-major blocks as to_device_blocks."""
+    def cn(*shape):
+        re = torch.randn(*shape, dtype=torch.float64, device=dev, generator=gen)
+        im = torch.randn(*shape, dtype=torch.float64, device=dev, generator=gen)
+        return torch.complex(re, im) / np.sqrt(2.0)
+    A = cn(nx, nm, nm)
+    H = (A + A.conj().transpose(-1, -2)) / (2.0 * np.sqrt(nm))
+    U = (torch.rand(nx, dtype=torch.float64, device=dev, generator=gen) * 2 - 1) * u_range
+    H = H + U.view(nx, 1, 1) * torch.eye(nm, dtype=torch.complex128, device=dev)
+    C = cn(nx, nm, nm) * (coupling_sigma / np.sqrt(nm))
+    H[nx - 1] = H[0]
+    C[nx - 1] = C[0]
+    return H.contiguous(), C.contiguous()
+
+
+def devices_cost(rgf, torch, dev, ndev, e_div, seed=3):
+    """64 uneven devices in one persistent launch: nm in {64, 96, 128, 192, 256},
+    nx ~ U[50, 400], energy count ~ U[256, 2048] divided by e_div (energies in
+    [-2, 2] eV), eta 1e-5, Fermi leads, compat mode, tr/tre/dens per energy.
+    Timed: the autotuned one-launch solve (rgf_solve_devices) and, for
+    comparison, the same devices one rgf_solve_batched launch after another."""
+    rgf.lib().rgf_release()
+    torch.cuda.empty_cache()
+    rng = np.random.default_rng(seed)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    kT = 0.0259
+    T = kT / 8.61734e-05
+    temp, mu = (T, T), (0.1, -0.1)
+    p = rgf.params(rgf.MODE_COMPAT, eta_dev=1e-5, eta_lead=1e-5, dec_tol=1e-12, dec_nmax=100)
+    devs, shapes = [], []
+    for j in range(ndev):
+        nm = int(rng.choice([64, 96, 128, 192, 256]))
+        nx = int(rng.integers(50, 401))
+        nE = max(1, int(rng.integers(256, 2049)) // e_div)
+        Hd, Cd = random_device_torch(torch, nm, nx, gen, dev)
+        E = torch.tensor(np.linspace(-2.0, 2.0, nE, endpoint=False) + 4.0 / nE, dtype=torch.float64, device=dev)
+        out = {"tr": torch.zeros(nE, dtype=torch.float64, device=dev), "tre": torch.zeros(nE, dtype=torch.float64, device=dev),
+               "dens_n": torch.zeros(nE, nx, dtype=torch.float64, device=dev),
+               "info": torch.zeros(nE, 4, dtype=torch.int32, device=dev)}
+        devs.append({"E": E, "temp": temp, "mu": mu, "Hii": Hd, "H1i": Cd, "out": out})
+        shapes.append((nm, nx, nE))
+    items = sum(s[2] for s in shapes)
+    t0 = time.perf_counter()
+    tiles, rep = rgf.solve_devices(p, devs, autotune=True)  # includes the autotuner's first timing
+    torch.cuda.synchronize()
+    t_first = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    tiles, rep = rgf.solve_devices(p, devs, autotune=True)
+    torch.cuda.synchronize()
+    t_one = time.perf_counter() - t0
+    print(f"bench: {ndev} devices, {items} energies, one launch {t_one:.2f}s (first call {t_first:.2f}s)",
+          file=sys.stderr, flush=True)
+    flops = 0.0
+    for (nm, nx, nE), d in zip(shapes, devs):
+        flops += float(np.sum(algorithmic_flops_per_energy(nm, nx, d["out"]["info"].cpu().numpy())))
+    tr_one = [d["out"]["tr"].clone() for d in devs]
+    rgf.lib().rgf_release()
+    t0 = time.perf_counter()
+    for d in devs:
+        rgf.solve_batched(p, d["E"], temp, mu, d["Hii"], d["H1i"], out=d["out"])
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+    print(f"bench: {ndev} devices one launch each {t_seq:.2f}s", file=sys.stderr, flush=True)
+    same = all(torch.equal(a, d["out"]["tr"]) for a, d in zip(tr_one, devs))
+    rgf.lib().rgf_release()
+    del devs
+    torch.cuda.empty_cache()
+    achieved = flops / t_one / 1e12
+    sizes = sorted(set(s[0] for s in shapes))
+    return {"workload": f"{ndev} random devices, nm in {{64,96,128,192,256}}, nx ~ U[50,400], energies ~ U[256,2048]/{e_div} "
+                        "in [-2,2] eV, eta 1e-5, Fermi leads, compat mode; tr, tre, dens per energy",
+            "devices": ndev, "energy_points": items, "s_one_launch": t_one, "s_first_call_with_autotune": t_first,
+            "energy_points_per_s": items / t_one, "achieved_tflops": achieved, "frac": achieved / PEAK_FP64_MFMA,
+            "s_one_launch_per_device": t_seq, "one_launch_over_per_device": t_seq / t_one,
+            "tr_identical_to_per_device_launches": bool(same), "slots": rep["slots"],
+            "workspace_gib": rep["ws_bytes"] / 2 ** 30,
+            "tile_T_by_nm": {str(nm): int(tiles[[s[0] for s in shapes].index(nm)]) for nm in sizes}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -148,6 +305,13 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward cost measurement")
     ap.add_argument("--adjoint-energies", type=int, default=1024)
+    ap.add_argument("--no-scba", action="store_true", help="skip the SCBA-step (off-diagonal Sigma) measurement")
+    ap.add_argument("--scba-energies", type=int, default=4096)
+    ap.add_argument("--scba-iterations", type=int, default=10)
+    ap.add_argument("--no-devices", action="store_true", help="skip the 64-device one-launch measurement")
+    ap.add_argument("--devices", type=int, default=64)
+    ap.add_argument("--devices-energy-div", type=int, default=32,
+                    help="divide each device's energy count (U[256,2048]) by this")
     ap.add_argument("--plain-energies", type=int, default=4,
                     help="energies timed on the plain per-energy GPU path (0: skip)")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
@@ -244,6 +408,12 @@ def main():
     adjoint = None
     if not args.no_adjoint:
         adjoint = adjoint_cost(rgf, torch, dev, stream, args.adjoint_energies)
+    scba = None
+    if not args.no_scba:
+        scba = scba_cost(rgf, torch, dev, args.scba_energies, args.scba_iterations)
+    devices = None
+    if not args.no_devices:
+        devices = devices_cost(rgf, torch, dev, args.devices, args.devices_energy_div)
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tfile):
@@ -273,6 +443,8 @@ def main():
         "batched_over_plain": (value / plain["value"]) if plain else None,
         "adjoint": adjoint,
         "adjoint_over_forward": adjoint["ratio"] if adjoint else None,
+        "scba_step": scba,
+        "devices_one_launch": devices,
         "decimation_iters_mean": float(info[:, :2].mean()),
         "sane": ok,
     }

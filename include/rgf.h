@@ -162,6 +162,39 @@ int rgf_graph_create(const rgf_params *p, const rgf_batch *b, const rgf_scba *sc
 int rgf_graph_launch(rgf_graph g, int replays, void *stream);
 void rgf_graph_destroy(rgf_graph g);
 
+/* Many independent devices in one persistent launch (the reference's
+ * drivers call green_RGF_RS once per device and energy, e.g. the k-point and
+ * device loops around src/green_rgf.f90:1301-1318).  devs[j] describes device
+ * j exactly as rgf_solve_batched's batch does (its own nm, multiple of 32 and
+ * at most 256; nx; nE and energy grid; Hii, H1i, diagonal sigma and outputs;
+ * max_slots and the off-diagonal sigma fields must be 0/NULL).  All
+ * (device, energy) pairs form one work list ordered by predicted cost,
+ * longest first; the workgroups of one launch take items from it
+ * dynamically, each with the tile variant of its device's block size.
+ * autotune != 0: before the first solve at a block size, time its candidate
+ * tile variants on the device (results kept per process) and use the fastest
+ * and its measured cost for the ordering.  The number of workgroups is
+ * bounded by the CU count and by free HBM for the largest device's stored
+ * blocks.  tile_T (optional, ndev ints) receives the tile parameter chosen
+ * per device; rep (optional) the launch shape.  Synchronises the device
+ * before it starts (its descriptor buffer is reused), then runs
+ * asynchronously on `stream`.  No counterpart in the reference. */
+typedef struct rgf_multi_report {
+  int slots;          /* workgroups of the persistent launch */
+  int tuned;          /* block sizes timed by the autotuner in this call */
+  long long items;    /* (device, energy) pairs */
+  long long ws_bytes; /* device workspace of the launch */
+} rgf_multi_report;
+
+int rgf_solve_devices(const rgf_params *p, const rgf_batch *devs, int ndev, int autotune,
+                      int *tile_T, rgf_multi_report *rep, void *stream);
+
+/* rgf_solve_devices captured as one HIP graph (counter reset + the persistent
+ * launch; the graph owns its workspace and descriptors); replay it with
+ * rgf_graph_launch, release it with rgf_graph_destroy. */
+int rgf_graph_create_devices(const rgf_params *p, const rgf_batch *devs, int ndev, int autotune,
+                             void *stream, rgf_graph *out);
+
 /* Bytes of device workspace rgf_solve_batched uses for `slots` energies in
  * flight at block size nm and nx slabs. */
 long long rgf_workspace_bytes(int nm, int nx, int slots);

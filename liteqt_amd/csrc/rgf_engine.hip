@@ -6,12 +6,14 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
 
 #include "../../include/rgf.h"
 #include "rgf_adjoint.hpp"
+#include "rgf_multi.hpp"
 #include "rgf_offdiag.hpp"
 #include "rgf_sweep.hpp"
 
@@ -199,6 +201,8 @@ struct Prepared {
   bool nothing = false;
 };
 
+void fill_args(const rgf_params* p, const rgf_batch* b, long long se, rgf::SweepArgs* out);
+
 // Validate a batch, pick the kernel and size the slots (HBM admission).  The
 // workspace pointer is left null; the caller binds the shared or an owned one.
 int prepare(const rgf_params* p, const rgf_batch* b, const rgf_scba* scba, Prepared* out) {
@@ -236,7 +240,18 @@ int prepare(const rgf_params* p, const rgf_batch* b, const rgf_scba* scba, Prepa
   out->slots = slots;
   out->slot_elems = se;
   out->dev = dev;
-  rgf::SweepArgs& a = out->a;
+  fill_args(p, b, se, &out->a);
+  if (scba) {
+    out->a.scba = 1;
+    out->a.scba_D = scba->D;
+    out->a.scba_mix = scba->mix;
+  }
+  return 0;
+}
+
+// Kernel arguments of one batch (workspace pointer left null).
+void fill_args(const rgf_params* p, const rgf_batch* b, long long se, rgf::SweepArgs* out) {
+  rgf::SweepArgs& a = *out;
   std::memset(&a, 0, sizeof(a));
   a.n = b->nm;
   a.nx = b->nx;
@@ -273,12 +288,6 @@ int prepare(const rgf_params* p, const rgf_batch* b, const rgf_scba* scba, Prepa
   a.sigR_lo = reinterpret_cast<const double2*>(b->sigR_lo);
   a.sigR_up = reinterpret_cast<const double2*>(b->sigR_up);
   a.sigL_up = reinterpret_cast<const double2*>(b->sigL_up);
-  if (scba) {
-    a.scba = 1;
-    a.scba_D = scba->D;
-    a.scba_mix = scba->mix;
-  }
-  return 0;
 }
 
 int solve_impl(const rgf_params* p, const rgf_batch* b, const rgf_scba* scba, void* stream) {
@@ -307,6 +316,231 @@ int solve_impl(const rgf_params* p, const rgf_batch* b, const rgf_scba* scba, vo
   return 0;
 }
 
+
+// ------------------------------------------------------------ many devices
+int mv_T(int v) {
+  static const int t[rgf::MV_COUNT] = {1, 2, 1, 3, 1, 4, 2, 3, 1, 4, 2, 1};
+  return (v >= 0 && v < rgf::MV_COUNT) ? t[v] : 0;
+}
+
+int default_mv(int n) {
+  switch (n) {
+    case 32: return rgf::MV_1_32;
+    case 64: return rgf::MV_2_64;
+    case 96: return rgf::MV_3_96;
+    case 128: return rgf::MV_4_128;
+    case 192: return rgf::MV_3_192;
+    case 256: return rgf::MV_4_256;
+    default: return rgf::MV_1_256;
+  }
+}
+
+std::vector<int> candidate_mv(int n) {
+  switch (n) {
+    case 64: return {rgf::MV_2_64, rgf::MV_1_64};
+    case 96: return {rgf::MV_3_96, rgf::MV_1_96};
+    case 128: return {rgf::MV_4_128, rgf::MV_2_128};
+    case 192: return {rgf::MV_3_192, rgf::MV_1_192};
+    case 256: return {rgf::MV_4_256, rgf::MV_2_256};
+    default: return {default_mv(n)};
+  }
+}
+
+// Autotuner memory per device: block size -> (variant, seconds per slab-energy
+// measured with the machine full).
+struct Tuned {
+  int mv;
+  double t_slab;
+};
+std::map<int, Tuned> g_tuned[kMaxDev];
+std::mutex g_multi_mu;
+Buf g_multi_desc[kMaxDev];
+int g_multi_cu[kMaxDev];
+
+int multi_per_cu(int* per_cu) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, reinterpret_cast<const void*>(&rgf::rgf_sweep_multi),
+                                                        256, 0) == hipSuccess ? 0 : 1;
+}
+
+// Device buffers of one multi-device launch, laid out in one allocation.
+struct MultiDesc {
+  size_t off_variant, off_items, off_counter, bytes;
+};
+MultiDesc multi_layout(int ndev, long long nitems) {
+  MultiDesc d;
+  auto al = [](size_t x) { return (x + 255) / 256 * 256; };
+  d.off_variant = al(sizeof(rgf::SweepArgs) * (size_t)ndev);
+  d.off_items = d.off_variant + al(sizeof(int) * (size_t)ndev);
+  d.off_counter = d.off_items + al(sizeof(int2) * (size_t)std::max<long long>(nitems, 1));
+  d.bytes = d.off_counter + 256;
+  return d;
+}
+
+// Copy the descriptors of one multi-device launch into `desc` (synchronous).
+int multi_upload(const std::vector<rgf::SweepArgs>& jobs, const std::vector<int>& mv, const std::vector<int2>& items,
+                 void* desc) {
+  const MultiDesc L = multi_layout((int)jobs.size(), (long long)items.size());
+  char* base = reinterpret_cast<char*>(desc);
+  HIPCHK(hipMemcpy(base, jobs.data(), sizeof(rgf::SweepArgs) * jobs.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(base + L.off_variant, mv.data(), sizeof(int) * mv.size(), hipMemcpyHostToDevice));
+  if (!items.empty())
+    HIPCHK(hipMemcpy(base + L.off_items, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice));
+  return 0;
+}
+
+// Counter reset (a fill kernel, so the pair can be captured in a graph) and the
+// persistent launch over the uploaded descriptors.
+int multi_launch(int ndev, long long nitems, void* desc, double2* ws, long long ws_slot, int slots, hipStream_t s) {
+  const MultiDesc L = multi_layout(ndev, nitems);
+  char* base = reinterpret_cast<char*>(desc);
+  rgf::MultiArgs m;
+  m.jobs = reinterpret_cast<const rgf::SweepArgs*>(base);
+  m.variant = reinterpret_cast<const int*>(base + L.off_variant);
+  m.items = reinterpret_cast<const int2*>(base + L.off_items);
+  m.nitems = (int)nitems;
+  m.counter = reinterpret_cast<int*>(base + L.off_counter);
+  m.ws = ws;
+  m.ws_slot = ws_slot;
+  hipLaunchKernelGGL(rgf::k_fill_int, dim3(1), dim3(64), 0, s, m.counter, 1, 0);
+  hipLaunchKernelGGL(rgf::rgf_sweep_multi, dim3(slots), dim3(256), 0, s, m);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// Validate the devices and build their kernel arguments.
+int multi_jobs(const rgf_params* p, const rgf_batch* devs, int ndev, std::vector<rgf::SweepArgs>* jobs) {
+  if (int rc = check_params(p)) return rc;
+  if (!devs || ndev < 1) return fail(RGF_EINVAL, "rgf_solve_devices: no devices");
+  jobs->resize(ndev);
+  for (int j = 0; j < ndev; ++j) {
+    const rgf_batch* b = &devs[j];
+    if (b->nm % 32 != 0 || b->nm < 32 || b->nm > 256)
+      return fail(RGF_EUNSUP, "rgf_solve_devices: device " + std::to_string(j) + ": nm must be a multiple of 32 in [32, 256]");
+    if (b->nx < 2 || b->nE < 0) return fail(RGF_EINVAL, "rgf_solve_devices: device " + std::to_string(j) + ": need nx >= 2, nE >= 0");
+    if (b->nE > 0 && (!b->E || !b->Hii || !b->H1i))
+      return fail(RGF_EINVAL, "rgf_solve_devices: device " + std::to_string(j) + ": E, Hii and H1i are required");
+    if (has_offdiag(b)) return fail(RGF_EUNSUP, "rgf_solve_devices: off-diagonal Sigma is not supported here");
+    fill_args(p, b, 0, &(*jobs)[j]);
+  }
+  return 0;
+}
+
+// Autotuner: for every block size with more than one candidate variant, time
+// each candidate on a slice of the first device of that size (its first
+// energy, at most 12 slabs, 2 decimation iterations), replicated on every
+// workgroup so the machine is as loaded as in the real launch; keep the fastest.
+int multi_autotune(const std::vector<rgf::SweepArgs>& jobs, int dev, int slots, hipStream_t s, int* ntuned) {
+  std::map<int, int> first;
+  for (int j = 0; j < (int)jobs.size(); ++j)
+    if (jobs[j].nE > 0 && !first.count(jobs[j].n)) first[jobs[j].n] = j;
+  *ntuned = 0;
+  for (auto& kv : first) {
+    const int n = kv.first;
+    if (g_tuned[dev].count(n)) continue;
+    rgf::SweepArgs a = jobs[kv.second];
+    a.nx = std::min(a.nx, 12);
+    a.nE = 1;
+    a.dec_nmax = 2;
+    a.ldos = a.ndens = a.pdens = a.GRi1 = a.GLi1 = a.GGi1 = nullptr;
+    a.cur = a.tr = a.tre = a.dens_n = a.dens_p = nullptr;
+    a.info = nullptr;
+    const long long se = (long long)(2 * a.nx + rgf::kTmpBlocks) * n * n;
+    const int cs = std::max(1, std::min(slots, (int)std::min<long long>(slots, (long long)(8LL << 30) / (se * 16))));
+    Buf ws, desc;
+    if (int rc = ws.ensure((size_t)(se * 16 * cs))) return rc;
+    std::vector<rgf::SweepArgs> one(1, a);
+    std::vector<int2> items((size_t)cs, make_int2(0, 0));
+    const MultiDesc L = multi_layout(1, cs);
+    if (int rc = desc.ensure(L.bytes)) return rc;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    Tuned best{default_mv(n), 1e30};
+    for (int v : candidate_mv(n)) {
+      std::vector<int> mv(1, v);
+      if (int rc = multi_upload(one, mv, items, desc.p)) return rc;
+      if (int rc = multi_launch(1, cs, desc.p, reinterpret_cast<double2*>(ws.p), se, cs, s)) return rc;  // warm-up
+      HIPCHK(hipEventRecord(e0, s));
+      if (int rc = multi_launch(1, cs, desc.p, reinterpret_cast<double2*>(ws.p), se, cs, s)) return rc;
+      HIPCHK(hipEventRecord(e1, s));
+      HIPCHK(hipEventSynchronize(e1));
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+      // per slab-energy: 12 slabs plus 2 x 2 decimation iterations (~7/17 slab each)
+      const double t = ms * 1e-3 / (a.nx + 4.0 * 7.0 / 17.0);
+      if (t < best.t_slab) best = Tuned{v, t};
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ws.release();
+    desc.release();
+    g_tuned[dev][n] = best;
+    ++*ntuned;
+  }
+  return 0;
+}
+
+// Plan: variants, work list (longest first), workspace per slot, slot count.
+struct MultiPlan {
+  std::vector<rgf::SweepArgs> jobs;
+  std::vector<int> mv;
+  std::vector<int2> items;
+  long long ws_slot = 0;
+  int slots = 0;
+  int dev = 0;
+  int ntuned = 0;
+};
+
+int multi_plan(const rgf_params* p, const rgf_batch* devs, int ndev, int autotune, int max_slots, hipStream_t s,
+               MultiPlan* pl, size_t reserved) {
+  if (int rc = multi_jobs(p, devs, ndev, &pl->jobs)) return rc;
+  HIPCHK(hipGetDevice(&pl->dev));
+  const int dev = pl->dev;
+  if (dev < 0 || dev >= kMaxDev) return fail(RGF_EINVAL, "device index out of range");
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, dev));
+  int per_cu = 0;
+  if (multi_per_cu(&per_cu) || per_cu < 1) return fail(RGF_EHIP, "multi-device kernel does not fit on a CU");
+  const int full = prop.multiProcessorCount * per_cu;
+  long long nitems = 0, se_max = 0;
+  for (auto& a : pl->jobs) {
+    nitems += a.nE;
+    if (a.nE > 0) se_max = std::max<long long>(se_max, (long long)(2 * a.nx + rgf::kTmpBlocks) * a.n * a.n);
+  }
+  if (autotune) {
+    if (int rc = multi_autotune(pl->jobs, dev, full, s, &pl->ntuned)) return rc;
+  }
+  pl->mv.resize(ndev);
+  std::vector<double> key(ndev);
+  for (int j = 0; j < ndev; ++j) {
+    const int n = pl->jobs[j].n;
+    auto it = g_tuned[dev].find(n);
+    const bool tuned = autotune && it != g_tuned[dev].end();
+    pl->mv[j] = tuned ? it->second.mv : default_mv(n);
+    // predicted seconds per energy: slabs plus ~36 decimation iterations at 7/17 slab each
+    const double per_slab = tuned ? it->second.t_slab : (double)n * n * n;
+    key[j] = per_slab * (pl->jobs[j].nx + 36.0 * 7.0 / 17.0);
+  }
+  pl->items.clear();
+  pl->items.reserve((size_t)nitems);
+  for (int j = 0; j < ndev; ++j)
+    for (int e = 0; e < pl->jobs[j].nE; ++e) pl->items.push_back(make_int2(j, e));
+  std::stable_sort(pl->items.begin(), pl->items.end(),
+                   [&](const int2& x, const int2& y) { return key[x.x] > key[y.x]; });
+  pl->ws_slot = se_max;
+  long long slots = std::min<long long>(nitems, full);
+  if (max_slots > 0) slots = std::min<long long>(slots, max_slots);
+  size_t fr = 0, tot = 0;
+  HIPCHK(hipMemGetInfo(&fr, &tot));
+  const size_t avail = (size_t)((double)(fr + reserved) * 0.9);
+  if (se_max > 0) {
+    const long long fit = (long long)(avail / (size_t)(se_max * 16));
+    if (fit < 1) return fail(RGF_ENOMEM, "not enough device memory for one slot of the largest device");
+    slots = std::min(slots, fit);
+  }
+  pl->slots = (int)std::max<long long>(slots, 1);
+  return 0;
+}
 }  // namespace
 
 extern "C" {
@@ -344,6 +578,7 @@ struct rgf_graph_s {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   void* ws = nullptr;
+  void* desc = nullptr;
   int dev = 0;
 };
 
@@ -397,7 +632,93 @@ void rgf_graph_destroy(rgf_graph g) {
   if (g->exec) (void)hipGraphExecDestroy(g->exec);
   if (g->graph) (void)hipGraphDestroy(g->graph);
   if (g->ws) (void)hipFree(g->ws);
+  if (g->desc) (void)hipFree(g->desc);
   delete g;
+}
+
+int rgf_solve_devices(const rgf_params* p, const rgf_batch* devs, int ndev, int autotune, int* tile_T,
+                      rgf_multi_report* rep, void* stream) {
+  g_err.clear();
+  {
+    std::vector<rgf::SweepArgs> check;  // argument errors before any device call
+    if (int rc = multi_jobs(p, devs, ndev, &check)) return rc;
+  }
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  std::lock_guard<std::mutex> lk(g_multi_mu);
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDev) return fail(RGF_EINVAL, "device index out of range");
+  // the descriptor buffer and workspace are reused: let earlier launches finish
+  HIPCHK(hipDeviceSynchronize());
+  std::lock_guard<std::mutex> lw(g_ws_mu);
+  Buf& ws = g_ws[dev];
+  MultiPlan pl;
+  if (int rc = multi_plan(p, devs, ndev, autotune, 0, s, &pl, ws.bytes)) return rc;
+  if (tile_T)
+    for (int j = 0; j < ndev; ++j) tile_T[j] = mv_T(pl.mv[j]);
+  if (rep) {
+    rep->slots = pl.slots;
+    rep->items = (long long)pl.items.size();
+    rep->ws_bytes = pl.ws_slot * 16LL * pl.slots;
+    rep->tuned = pl.ntuned;
+  }
+  if (pl.items.empty()) return 0;
+  if ((size_t)(pl.ws_slot * 16LL * pl.slots) > ws.bytes) {
+    ws.release();
+    if (int rc = ws.ensure((size_t)(pl.ws_slot * 16LL * pl.slots))) return rc;
+  }
+  Buf& desc = g_multi_desc[dev];
+  const MultiDesc L = multi_layout(ndev, (long long)pl.items.size());
+  if (int rc = desc.ensure(L.bytes)) return rc;
+  if (int rc = multi_upload(pl.jobs, pl.mv, pl.items, desc.p)) return rc;
+  return multi_launch(ndev, (long long)pl.items.size(), desc.p, reinterpret_cast<double2*>(ws.p), pl.ws_slot,
+                      pl.slots, s);
+}
+
+int rgf_graph_create_devices(const rgf_params* p, const rgf_batch* devs, int ndev, int autotune, void* stream,
+                             rgf_graph* out) {
+  g_err.clear();
+  if (!out) return fail(RGF_EINVAL, "rgf_graph_create_devices: out is NULL");
+  *out = nullptr;
+  if (!stream) return fail(RGF_EINVAL, "rgf_graph_create_devices: capture needs a created stream (not the NULL stream)");
+  {
+    std::vector<rgf::SweepArgs> check;
+    if (int rc = multi_jobs(p, devs, ndev, &check)) return rc;
+  }
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  std::lock_guard<std::mutex> lk(g_multi_mu);
+  MultiPlan pl;
+  if (int rc = multi_plan(p, devs, ndev, autotune, 0, s, &pl, 0)) return rc;
+  if (pl.items.empty()) return fail(RGF_EINVAL, "rgf_graph_create_devices: nothing to solve");
+  rgf_graph_s* g = new rgf_graph_s();
+  g->dev = pl.dev;
+  const MultiDesc L = multi_layout(ndev, (long long)pl.items.size());
+  hipError_t e = hipMalloc(&g->ws, (size_t)(pl.ws_slot * 16LL * pl.slots));
+  if (e == hipSuccess) e = hipMalloc(&g->desc, L.bytes);
+  if (e != hipSuccess) {
+    rgf_graph_destroy(g);
+    return fail(e == hipErrorOutOfMemory ? RGF_ENOMEM : RGF_EHIP, std::string("hipMalloc graph buffers: ") +
+                                                                     hipGetErrorString(e));
+  }
+  // upload outside the capture, then capture counter reset + persistent launch
+  if (int rc = multi_upload(pl.jobs, pl.mv, pl.items, g->desc)) {
+    rgf_graph_destroy(g);
+    return rc;
+  }
+  e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    int rc = multi_launch(ndev, (long long)pl.items.size(), g->desc, reinterpret_cast<double2*>(g->ws), pl.ws_slot,
+                          pl.slots, s);
+    e = hipStreamEndCapture(s, &g->graph);
+    if (rc && e == hipSuccess) e = hipErrorLaunchFailure;
+  }
+  if (e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    rgf_graph_destroy(g);
+    return fail(RGF_EHIP, std::string("rgf_graph_create_devices: ") + hipGetErrorString(e));
+  }
+  *out = g;
+  return 0;
 }
 
 // Host-array solve of nE energies: stage (and pad) inputs in HBM, run the

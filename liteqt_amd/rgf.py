@@ -51,6 +51,11 @@ class Batch(ctypes.Structure):
                 ("sigR_lo", ctypes.c_void_p), ("sigR_up", ctypes.c_void_p), ("sigL_up", ctypes.c_void_p)]
 
 
+class MultiReport(ctypes.Structure):
+    _fields_ = [("slots", ctypes.c_int), ("tuned", ctypes.c_int), ("items", ctypes.c_longlong),
+                ("ws_bytes", ctypes.c_longlong)]
+
+
 class Scba(ctypes.Structure):
     _fields_ = [("D", ctypes.c_double), ("mix", ctypes.c_double)]
 
@@ -59,7 +64,8 @@ class Scba(ctypes.Structure):
 EXPORTS = ("rgf_params_reference", "rgf_last_error", "rgf_version", "rgf_rs", "rgf_rs_ex",
            "rgf_rs_energies", "rgf_solve_batched", "rgf_workspace_bytes", "rgf_release",
            "rgf_zgemm_batched", "rgf_zinv_batched", "rgf_adjoint_batched", "rgf_adjoint_workspace_bytes",
-           "rgf_scba_step", "rgf_graph_create", "rgf_graph_launch", "rgf_graph_destroy")
+           "rgf_scba_step", "rgf_graph_create", "rgf_graph_launch", "rgf_graph_destroy", "rgf_solve_devices",
+           "rgf_graph_create_devices")
 
 _lib = None
 
@@ -85,6 +91,10 @@ def lib():
     L.rgf_graph_create.argtypes = [ctypes.POINTER(Params), ctypes.POINTER(Batch), ctypes.POINTER(Scba), vp,
                                    ctypes.POINTER(vp)]
     L.rgf_graph_launch.argtypes = [vp, ctypes.c_int, vp]
+    L.rgf_solve_devices.argtypes = [ctypes.POINTER(Params), ctypes.POINTER(Batch), ctypes.c_int, ctypes.c_int, vp,
+                                    ctypes.POINTER(MultiReport), vp]
+    L.rgf_graph_create_devices.argtypes = [ctypes.POINTER(Params), ctypes.POINTER(Batch), ctypes.c_int,
+                                           ctypes.c_int, vp, ctypes.POINTER(vp)]
     L.rgf_graph_destroy.argtypes = [vp]
     L.rgf_graph_destroy.restype = None
     L.rgf_zgemm_batched.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, vp, vp,
@@ -394,3 +404,42 @@ def adjoint_batched(p, E, temp, mu, Hii, H1i, sigL=None, sigR=None, w_tr=None, w
     _check(lib().rgf_adjoint_batched(ctypes.byref(p), ctypes.byref(b), pw, pn, pl, _tp(J), _tp(dU),
                                      _tp(dH), ctypes.c_void_p(stream.cuda_stream)))
     return J, dU, dH
+
+
+def _device_batches(torch, devices):
+    """rgf_batch array for rgf_solve_devices from a list of dicts with keys
+    E, temp, mu, Hii, H1i and optional sigL, sigR, out (as solve_batched)."""
+    arr = (Batch * len(devices))()
+    for j, d in enumerate(devices):
+        b = _batch(torch, d["E"], d["temp"], d["mu"], d["Hii"], d["H1i"], d.get("sigL"), d.get("sigR"), 0)
+        _out_batch(torch, b, d.get("out") or {}, d["E"].device)
+        arr[j] = b
+    return arr
+
+
+def solve_devices(p, devices, autotune=False, stream=None):
+    """Many independent devices in one persistent launch (rgf_solve_devices).
+    Returns (tile T per device, report dict).  Asynchronous on `stream`."""
+    import torch
+    arr = _device_batches(torch, devices)
+    tiles = (ctypes.c_int * len(devices))()
+    rep = MultiReport()
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    _check(lib().rgf_solve_devices(ctypes.byref(p), arr, len(devices), int(bool(autotune)), tiles, ctypes.byref(rep),
+                                   ctypes.c_void_p(stream.cuda_stream)))
+    return list(tiles), {"slots": rep.slots, "tuned": rep.tuned, "items": rep.items, "ws_bytes": rep.ws_bytes}
+
+
+class DevicesGraph(Graph):
+    """rgf_graph_create_devices: the many-device launch captured as one HIP graph."""
+
+    def __init__(self, p, devices, stream, autotune=False):
+        import torch
+        self._keep = devices
+        arr = _device_batches(torch, devices)
+        self._stream = stream
+        h = ctypes.c_void_p()
+        _check(lib().rgf_graph_create_devices(ctypes.byref(p), arr, len(devices), int(bool(autotune)),
+                                              ctypes.c_void_p(stream.cuda_stream), ctypes.byref(h)))
+        self._h = h

@@ -82,3 +82,24 @@ def test_offdiag_and_scba_argument_errors_before_any_device_call():
     h = ctypes.c_void_p()
     assert L.rgf_graph_create(ctypes.byref(p), ctypes.byref(b), None, None, ctypes.byref(h)) == 1
     assert b"stream" in L.rgf_last_error() and not h.value
+
+
+def test_solve_devices_rejects_bad_devices_before_any_device_call():
+    from liteqt_amd import rgf as r
+    L = r.lib()
+    arr = (r.Batch * 2)()
+    for b in arr:
+        b.nm, b.nx, b.nE = 64, 3, 1
+        b.E = b.Hii = b.H1i = 16
+    arr[1].nm = 40
+    p = r.params(r.MODE_PHYSICAL)
+    assert L.rgf_solve_devices(ctypes.byref(p), arr, 2, 0, None, None, None) == 4  # RGF_EUNSUP
+    assert b"device 1" in L.rgf_last_error()
+    arr[1].nm = 64
+    arr[1].nx = 1
+    assert L.rgf_solve_devices(ctypes.byref(p), arr, 2, 0, None, None, None) == 1
+    arr[1].nx = 3
+    arr[0].sigL_up = 16
+    assert L.rgf_solve_devices(ctypes.byref(p), arr, 2, 0, None, None, None) == 4
+    h = ctypes.c_void_p()
+    assert L.rgf_graph_create_devices(ctypes.byref(p), arr, 2, 0, None, ctypes.byref(h)) == 1  # NULL stream
