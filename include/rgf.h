@@ -80,6 +80,35 @@ int rgf_rs_ex(const rgf_params *p, const double temp[2], int nm, int nx, double 
               double *ndens, double *pdens, double *ldos, double *tr, double *tre,
               double *cur, double *GRi1, double *GLi1, double *GGi1);
 
+/* Drop-in for green_RGF_CMS (src/green_rgf.f90:4167-4175): the RGF with the
+ * overlap blocks Sii (argument before Hii, as in the reference): z S_l,
+ * Sigma^R_l S_l and Sigma^<_l S_l, the lead decimation with S00 = Sii(:,:,1)
+ * and Sii(:,:,nx), and pdens / GGi1 from the reference's G^> recursion whose
+ * left-connected G^> (Glp) is never filled:
+ * pdens_l = g_l H1i_l^H pdens_{l+1} H1i_l g_l^H below the last slab.  Compat
+ * arithmetic and quirks as rgf_rs.  The reference never calls it. */
+int rgf_cms(const double temp[2], int nm, int nx, double E, const double mu[2],
+            const double *Sii, const double *Hii, const double *H1i, const double *sigL,
+            const double *sigG, const double *sigR, double *ndens, double *pdens,
+            double *ldos, double *tr, double *tre, double *cur, double *GRi1,
+            double *GLi1, double *GGi1);
+
+/* rgf_cms with explicit parameters (mode must be RGF_MODE_COMPAT). */
+int rgf_cms_ex(const rgf_params *p, const double temp[2], int nm, int nx, double E,
+               const double mu[2], const double *Sii, const double *Hii, const double *H1i,
+               const double *sigL, const double *sigG, const double *sigR,
+               double *ndens, double *pdens, double *ldos, double *tr, double *tre,
+               double *cur, double *GRi1, double *GLi1, double *GGi1);
+
+/* Energy loop of rgf_cms, as rgf_rs_energies (Sii shared by all energies). */
+int rgf_cms_energies(const rgf_params *p, const double temp[2], int nm, int nx,
+                     int nE, const double *E, const double mu[2], const double *Sii,
+                     const double *Hii, const double *H1i, const double *sigL,
+                     const double *sigG, const double *sigR, int sig_per_energy,
+                     double *ndens, double *pdens, double *ldos, double *tr,
+                     double *tre, double *cur, double *GRi1, double *GLi1,
+                     double *GGi1, int *info);
+
 /* Replacement of the drivers' energy loops `do ie ... call green_RGF_RS(...)`
  * (src/green_rgf.f90:3174-3180, 2914-2927, 1301-1318): all nE energies in one
  * batched GPU solve.  Host arrays with a trailing energy index, i.e. the
@@ -128,6 +157,10 @@ typedef struct rgf_batch {
    * stride as sigL/sigR.  Any may be NULL (zero); a non-NULL one needs
    * RGF_MODE_PHYSICAL.  Zero-initialise the struct so unused fields are NULL. */
   const double *sigR_lo, *sigR_up, *sigL_up;
+  /* Overlap blocks S_l (nx blocks, shared by all energies) for the
+   * green_RGF_CMS solve (src/green_rgf.f90:4167-4383); NULL for green_RGF_RS.
+   * Needs RGF_MODE_COMPAT and no off-diagonal Sigma. */
+  const double *Sii;
 } rgf_batch;
 
 int rgf_solve_batched(const rgf_params *p, const rgf_batch *b, void *stream);

@@ -12,7 +12,7 @@ module rgf_shim
   use iso_c_binding
   implicit none
   private
-  public :: rgf_params, rgf_params_reference, green_RGF_RS_gpu, green_RGF_RS_energies_gpu
+  public :: rgf_params, rgf_params_reference, green_RGF_RS_gpu, green_RGF_RS_energies_gpu, green_RGF_CMS_gpu
   public :: RGF_MODE_COMPAT, RGF_MODE_PHYSICAL, rgf_last_error_f
 
   integer(c_int), parameter :: RGF_MODE_COMPAT = 0, RGF_MODE_PHYSICAL = 1
@@ -45,6 +45,18 @@ module rgf_shim
       integer(c_int), value :: nm, nx
       real(c_double), value :: E
       type(c_ptr), value :: Hii, H1i, sigL, sigG, sigR, ndens, pdens, ldos
+      real(c_double), intent(inout) :: tr, tre
+      type(c_ptr), value :: cur, GRi1, GLi1, GGi1
+    end function
+
+    integer(c_int) function rgf_cms_ex_c(p, temp, nm, nx, E, mu, Sii, Hii, H1i, sigL, sigG, sigR, &
+        ndens, pdens, ldos, tr, tre, cur, GRi1, GLi1, GGi1) bind(C, name="rgf_cms_ex")
+      import :: c_int, c_double, c_ptr, rgf_params
+      type(rgf_params), intent(in) :: p
+      real(c_double), intent(in) :: temp(2), mu(2)
+      integer(c_int), value :: nm, nx
+      real(c_double), value :: E
+      type(c_ptr), value :: Sii, Hii, H1i, sigL, sigG, sigR, ndens, pdens, ldos
       real(c_double), intent(inout) :: tr, tre
       type(c_ptr), value :: cur, GRi1, GLi1, GGi1
     end function
@@ -109,6 +121,40 @@ contains
         c_loc(ndens), c_loc(pdens), c_loc(ldos), tr, tre, c_loc(cur), pr, pl, pg)
     if (rc /= 0) then
       print *, 'green_RGF_RS_gpu: ', rgf_last_error_f()
+      stop 1
+    end if
+  end subroutine
+
+  ! Same arguments as green_RGF_CMS (src/green_rgf.f90:4167-4175): the overlap
+  ! variant (Sii before Hii).
+  subroutine green_RGF_CMS_gpu(TEMP, nm, nx, E, mu, Sii, Hii, H1i, sigma_lesser_ph, sigma_greater_ph, sigma_r_ph, &
+      ndens, pdens, ldos, tr, tre, cur, GRi1, GLi1, GGi1, params)
+    integer, intent(in) :: nm, nx
+    real(8), intent(in) :: E, mu(2), temp(2)
+    complex(8), intent(in), dimension(nm, nm, nx), target :: sigma_lesser_ph, sigma_greater_ph, sigma_r_ph
+    complex(8), intent(in), dimension(nm, nm, nx), target :: Sii, Hii, H1i
+    complex(8), intent(inout), dimension(nm, nm, nx), target :: ldos, ndens, pdens
+    real(8), intent(inout), dimension(nm, nm, nx), target :: cur
+    complex(8), intent(inout), dimension(nm, nm, nx), optional, target :: GRi1, GLi1, GGi1
+    real(8), intent(inout) :: tr, tre
+    type(rgf_params), intent(in), optional :: params
+    type(rgf_params) :: p
+    type(c_ptr) :: pr, pl, pg
+    integer(c_int) :: rc
+    if (present(params)) then
+      p = params
+    else
+      call rgf_params_reference(p)
+    end if
+    pr = c_null_ptr; pl = c_null_ptr; pg = c_null_ptr
+    if (present(GRi1)) pr = c_loc(GRi1)
+    if (present(GLi1)) pl = c_loc(GLi1)
+    if (present(GGi1)) pg = c_loc(GGi1)
+    rc = rgf_cms_ex_c(p, temp, int(nm, c_int), int(nx, c_int), E, mu, c_loc(Sii), c_loc(Hii), c_loc(H1i), &
+        c_loc(sigma_lesser_ph), c_loc(sigma_greater_ph), c_loc(sigma_r_ph), &
+        c_loc(ndens), c_loc(pdens), c_loc(ldos), tr, tre, c_loc(cur), pr, pl, pg)
+    if (rc /= 0) then
+      print *, 'green_RGF_CMS_gpu: ', rgf_last_error_f()
       stop 1
     end if
   end subroutine

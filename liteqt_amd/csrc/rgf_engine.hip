@@ -136,6 +136,16 @@ const void* sweep_fn() {
 }
 
 template <int T, int NMAX>
+void launch_sweep_cms(const rgf::SweepArgs& a, int grid, hipStream_t s) {
+  hipLaunchKernelGGL((rgf::rgf_sweep_cms<T, NMAX>), dim3(grid), dim3(rgf::Cfg<T>::NTH), 0, s, a);
+}
+
+template <int T, int NMAX>
+const void* sweep_cms_fn() {
+  return reinterpret_cast<const void*>(&rgf::rgf_sweep_cms<T, NMAX>);
+}
+
+template <int T, int NMAX>
 void launch_sweep_od(const rgf::SweepArgs& a, int grid, hipStream_t s) {
   hipLaunchKernelGGL((rgf::rgf_sweep_od<T, NMAX>), dim3(grid), dim3(rgf::Cfg<T>::NTH), 0, s, a);
 }
@@ -180,6 +190,20 @@ bool pick_variant_od(int n, Variant* v) {
   }
 }
 
+// The same tile variants for the overlap (green_RGF_CMS) kernel.
+bool pick_variant_cms(int n, Variant* v) {
+  if (n % 32 != 0 || n < 32 || n > 256) return false;
+  switch (n) {
+    case 32: *v = {launch_sweep_cms<1, 32>, sweep_cms_fn<1, 32>(), 1, 32, rgf::Cfg<1>::NTH}; return true;
+    case 64: *v = {launch_sweep_cms<2, 64>, sweep_cms_fn<2, 64>(), 2, 64, rgf::Cfg<2>::NTH}; return true;
+    case 96: *v = {launch_sweep_cms<3, 96>, sweep_cms_fn<3, 96>(), 3, 96, rgf::Cfg<3>::NTH}; return true;
+    case 128: *v = {launch_sweep_cms<4, 128>, sweep_cms_fn<4, 128>(), 4, 128, rgf::Cfg<4>::NTH}; return true;
+    case 192: *v = {launch_sweep_cms<3, 192>, sweep_cms_fn<3, 192>(), 3, 192, rgf::Cfg<3>::NTH}; return true;
+    case 256: *v = {launch_sweep_cms<4, 256>, sweep_cms_fn<4, 256>(), 4, 256, rgf::Cfg<4>::NTH}; return true;
+    default: *v = {launch_sweep_cms<1, 256>, sweep_cms_fn<1, 256>(), 1, 256, rgf::Cfg<1>::NTH}; return true;
+  }
+}
+
 long long slot_elems(int n, int nx) { return (long long)(2 * nx + rgf::kTmpBlocks) * n * n; }
 long long slot_elems_od(int n, int nx) { return (long long)(2 * nx + rgf::kTmpBlocksOd) * n * n; }
 
@@ -216,15 +240,18 @@ int prepare(const rgf_params* p, const rgf_batch* b, const rgf_scba* scba, Prepa
   }
   if (!b->E || !b->Hii || !b->H1i) return fail(RGF_EINVAL, "E, Hii and H1i are required");
   const bool od = has_offdiag(b) || scba;
+  const bool cms = b->Sii != nullptr;
   if (od && p->mode != RGF_MODE_PHYSICAL)
     return fail(RGF_EINVAL, "off-diagonal Sigma and the SCBA refresh need RGF_MODE_PHYSICAL");
+  if (cms && (od || p->mode != RGF_MODE_COMPAT))
+    return fail(RGF_EINVAL, "the overlap (green_RGF_CMS) solve has compat semantics only, without off-diagonal Sigma");
   if (scba) {
     if (!b->sigR || !b->sigL) return fail(RGF_EINVAL, "SCBA refresh: sigR and sigL are required");
     if (b->sig_stride != (long long)b->nm * b->nm * b->nx)
       return fail(RGF_EINVAL, "SCBA refresh: Sigma must be per energy (sig_stride = nm*nm*nx)");
   }
   Variant& v = out->v;
-  if (!(od ? pick_variant_od(b->nm, &v) : pick_variant(b->nm, &v)))
+  if (!(od ? pick_variant_od(b->nm, &v) : cms ? pick_variant_cms(b->nm, &v) : pick_variant(b->nm, &v)))
     return fail(RGF_EUNSUP, "rgf_solve_batched: nm must be a multiple of 32 in [32, 256]");
   int dev = 0;
   HIPCHK(hipGetDevice(&dev));
@@ -288,6 +315,7 @@ void fill_args(const rgf_params* p, const rgf_batch* b, long long se, rgf::Sweep
   a.sigR_lo = reinterpret_cast<const double2*>(b->sigR_lo);
   a.sigR_up = reinterpret_cast<const double2*>(b->sigR_up);
   a.sigL_up = reinterpret_cast<const double2*>(b->sigL_up);
+  a.Sii = reinterpret_cast<const double2*>(b->Sii);
 }
 
 int solve_impl(const rgf_params* p, const rgf_batch* b, const rgf_scba* scba, void* stream) {
@@ -419,7 +447,8 @@ int multi_jobs(const rgf_params* p, const rgf_batch* devs, int ndev, std::vector
     if (b->nx < 2 || b->nE < 0) return fail(RGF_EINVAL, "rgf_solve_devices: device " + std::to_string(j) + ": need nx >= 2, nE >= 0");
     if (b->nE > 0 && (!b->E || !b->Hii || !b->H1i))
       return fail(RGF_EINVAL, "rgf_solve_devices: device " + std::to_string(j) + ": E, Hii and H1i are required");
-    if (has_offdiag(b)) return fail(RGF_EUNSUP, "rgf_solve_devices: off-diagonal Sigma is not supported here");
+    if (has_offdiag(b) || b->Sii)
+      return fail(RGF_EUNSUP, "rgf_solve_devices: off-diagonal Sigma and overlaps are not supported here");
     fill_args(p, b, 0, &(*jobs)[j]);
   }
   return 0;
@@ -723,11 +752,55 @@ int rgf_graph_create_devices(const rgf_params* p, const rgf_batch* devs, int nde
 
 // Host-array solve of nE energies: stage (and pad) inputs in HBM, run the
 // batched engine chunk by chunk, copy (and unpad) the outputs back.
+static int rs_energies_impl(const rgf_params* p, const double temp[2], int nm, int nx, int nE, const double* E,
+                            const double mu[2], const double* Sii, const double* Hii, const double* H1i,
+                            const double* sigL, const double* sigG, const double* sigR, int sig_per_energy,
+                            double* ndens, double* pdens, double* ldos, double* tr, double* tre, double* cur,
+                            double* GRi1, double* GLi1, double* GGi1, int* info);
+
 int rgf_rs_energies(const rgf_params* p, const double temp[2], int nm, int nx, int nE, const double* E,
                     const double mu[2], const double* Hii, const double* H1i, const double* sigL,
                     const double* sigG, const double* sigR, int sig_per_energy, double* ndens,
                     double* pdens, double* ldos, double* tr, double* tre, double* cur, double* GRi1,
                     double* GLi1, double* GGi1, int* info) {
+  return rs_energies_impl(p, temp, nm, nx, nE, E, mu, nullptr, Hii, H1i, sigL, sigG, sigR, sig_per_energy, ndens,
+                          pdens, ldos, tr, tre, cur, GRi1, GLi1, GGi1, info);
+}
+
+int rgf_cms_energies(const rgf_params* p, const double temp[2], int nm, int nx, int nE, const double* E,
+                     const double mu[2], const double* Sii, const double* Hii, const double* H1i,
+                     const double* sigL, const double* sigG, const double* sigR, int sig_per_energy,
+                     double* ndens, double* pdens, double* ldos, double* tr, double* tre, double* cur,
+                     double* GRi1, double* GLi1, double* GGi1, int* info) {
+  g_err.clear();
+  if (!Sii) return fail(RGF_EINVAL, "rgf_cms_energies: Sii is required");
+  return rs_energies_impl(p, temp, nm, nx, nE, E, mu, Sii, Hii, H1i, sigL, sigG, sigR, sig_per_energy, ndens,
+                          pdens, ldos, tr, tre, cur, GRi1, GLi1, GGi1, info);
+}
+
+int rgf_cms_ex(const rgf_params* p, const double temp[2], int nm, int nx, double E, const double mu[2],
+               const double* Sii, const double* Hii, const double* H1i, const double* sigL, const double* sigG,
+               const double* sigR, double* ndens, double* pdens, double* ldos, double* tr, double* tre, double* cur,
+               double* GRi1, double* GLi1, double* GGi1) {
+  return rgf_cms_energies(p, temp, nm, nx, 1, &E, mu, Sii, Hii, H1i, sigL, sigG, sigR, 0, ndens, pdens, ldos, tr,
+                          tre, cur, GRi1, GLi1, GGi1, nullptr);
+}
+
+int rgf_cms(const double temp[2], int nm, int nx, double E, const double mu[2], const double* Sii,
+            const double* Hii, const double* H1i, const double* sigL, const double* sigG, const double* sigR,
+            double* ndens, double* pdens, double* ldos, double* tr, double* tre, double* cur, double* GRi1,
+            double* GLi1, double* GGi1) {
+  rgf_params p;
+  rgf_params_reference(&p);
+  return rgf_cms_ex(&p, temp, nm, nx, E, mu, Sii, Hii, H1i, sigL, sigG, sigR, ndens, pdens, ldos, tr, tre, cur,
+                    GRi1, GLi1, GGi1);
+}
+
+static int rs_energies_impl(const rgf_params* p, const double temp[2], int nm, int nx, int nE, const double* E,
+                            const double mu[2], const double* Sii, const double* Hii, const double* H1i,
+                            const double* sigL, const double* sigG, const double* sigR, int sig_per_energy,
+                            double* ndens, double* pdens, double* ldos, double* tr, double* tre, double* cur,
+                            double* GRi1, double* GLi1, double* GGi1, int* info) {
   (void)sigG;  // accepted and unused, as green_RGF_RS (src/green_rgf.f90:3953)
   g_err.clear();
   if (int rc = check_params(p)) return rc;
@@ -753,7 +826,7 @@ int rgf_rs_energies(const rgf_params* p, const double temp[2], int nm, int nx, i
                        (padded ? cslab_u : 0) + 64;
   size_t fr = 0, tot = 0;
   HIPCHK(hipMemGetInfo(&fr, &tot));
-  const size_t fixed = 2 * cslab + (sig_per_energy ? 0 : nsig * cslab) + (padded ? cslab_u : 0);
+  const size_t fixed = (Sii ? 3 : 2) * cslab + (sig_per_energy ? 0 : nsig * cslab) + (padded ? cslab_u : 0);
   const size_t ws_reserve = (size_t)rgf_workspace_bytes(np, nx, 256);
   const size_t budget = (size_t)((double)(fr + g_stage[dev].bytes) * 0.45);
   long long C = nE;
@@ -763,7 +836,7 @@ int rgf_rs_energies(const rgf_params* p, const double temp[2], int nm, int nx, i
   // staging layout
   size_t off = 0;
   auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) / 256 * 256; return o; };
-  const size_t oH = take(cslab), oC = take(cslab);
+  const size_t oH = take(cslab), oC = take(cslab), oS = Sii ? take(cslab) : 0;
   const size_t sig_n = sig_per_energy ? (size_t)C : 1;
   const size_t oSL = sigL ? take(cslab * sig_n) : 0, oSR = sigR ? take(cslab * sig_n) : 0;
   size_t oOut[6];
@@ -790,6 +863,17 @@ int rgf_rs_energies(const rgf_params* p, const double temp[2], int nm, int nx, i
   };
   if (int rc = h2d_c(Hii, oH, 1)) return rc;
   if (int rc = h2d_c(H1i, oC, 1)) return rc;
+  if (Sii) {
+    // overlaps pad with the identity (padded orbitals stay decoupled: z - 1e3 on their diagonal)
+    if (!padded) {
+      HIPCHK(hipMemcpyAsync(D(oS), Sii, cslab_u, hipMemcpyHostToDevice, s));
+    } else {
+      HIPCHK(hipMemcpyAsync(D(oU), Sii, cslab_u, hipMemcpyHostToDevice, s));
+      hipLaunchKernelGGL(pad_blocks, dim3(1024), dim3(256), 0, s, (const double2*)D(oU), nm, (double2*)D(oS), np,
+                         (long long)nx, make_double2(1, 0));
+      HIPCHK(hipGetLastError());
+    }
+  }
   if (padded) {
     // couplings pad with zeros (no diagonal fill)
     HIPCHK(hipMemcpyAsync(D(oU), H1i, cslab_u, hipMemcpyHostToDevice, s));
@@ -825,6 +909,7 @@ int rgf_rs_energies(const rgf_params* p, const double temp[2], int nm, int nx, i
     b.E = (const double*)D(oE);
     b.temp[0] = temp[0]; b.temp[1] = temp[1]; b.mu[0] = mu[0]; b.mu[1] = mu[1];
     b.Hii = (const double*)D(oH); b.H1i = (const double*)D(oC);
+    b.Sii = Sii ? (const double*)D(oS) : nullptr;
     b.sigL = sigL ? (const double*)D(oSL) : nullptr;
     b.sigR = sigR ? (const double*)D(oSR) : nullptr;
     b.sig_stride = sig_per_energy ? (long long)(bp * nx) : 0;

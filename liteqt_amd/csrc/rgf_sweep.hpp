@@ -52,18 +52,21 @@ struct SweepArgs {
   // Sigma <- (1 - scba_mix) Sigma + scba_mix * scba_D * G on every block given
   int scba;
   double scba_D, scba_mix;
+  // overlap blocks (green_RGF_CMS semantics, solve_energy<..., true> only)
+  const double2* Sii;
 };
 
 __device__ __forceinline__ double ferm_dev(double a) { return 1.0 / (1.0 + exp(a)); }
 
-// Sancho-Rubio decimation (src/green_rgf.f90:4451-4537) with S00 = I.  The
-// lead couples through Hc: left lead (left=1) H_10 = Hc^H, H_01 = Hc (the call
-// at 3995); right lead H_10 = Hc, H_01 = Hc^H (4035).  Writes the surface
-// function G00 = (z - H_SS)^-1 and returns the iterations run.
+// Sancho-Rubio decimation (src/green_rgf.f90:4451-4537) with the lead-cell
+// overlap S00 (null: identity, as green_RGF_RS passes).  The lead couples
+// through Hc: left lead (left=1) H_10 = Hc^H, H_01 = Hc (the call at 3995);
+// right lead H_10 = Hc, H_01 = Hc^H (4035).  Writes the surface function
+// G00 = (z S00 - H_SS)^-1 and returns the iterations run.
 template <int T, int NMAX>
 __device__ int wg_sancho(int n, double E, double eta, double tol, int nmax, const double2* H00,
                          const double2* Hc, int left, double2* G00, double2* tb, ldsd* lds,
-                         ldsd* red, int& hit, int& fails) {
+                         ldsd* red, int& hit, int& fails, const double2* S00 = nullptr) {
   const size_t nn = (size_t)n * n;
   const double2 one = make_double2(1, 0), mone = make_double2(-1, 0), zero = make_double2(0, 0);
   const double2 zl = make_double2(E, eta);
@@ -90,7 +93,10 @@ __device__ int wg_sancho(int n, double E, double eta, double tol, int nmax, cons
   int it_run = 0;
   for (int it = 1; it <= nmax; ++it) {
     it_run = it;
-    wg_ew(n, W, 1, zl, HBB, mone, nullptr, one, nullptr, one);
+    if (S00)
+      wg_ew(n, W, 1, zero, S00, zl, HBB, mone, nullptr, one);
+    else
+      wg_ew(n, W, 1, zl, HBB, mone, nullptr, one, nullptr, one);
     if (wg_inverse<T, NMAX>(n, W, A, lds)) ++fails;
     Epi st = epi_store();
     wg_gemm<T>(n, OP_N, A, OP_N, h10, B, 0, st, lds);
@@ -114,13 +120,20 @@ __device__ int wg_sancho(int n, double E, double eta, double tol, int nmax, cons
       wg_ew(n, HBB, 0, zero, H00, one, nullptr, one, nullptr, one);
     }
   }
-  wg_ew(n, W, 1, zl, HSS, mone, nullptr, one, nullptr, one);
+  if (S00)
+    wg_ew(n, W, 1, zero, S00, zl, HSS, mone, nullptr, one);
+  else
+    wg_ew(n, W, 1, zl, HSS, mone, nullptr, one, nullptr, one);
   if (wg_inverse<T, NMAX>(n, W, G00, lds)) ++fails;
   (void)red;
   return it_run;
 }
 
-template <int T, int NMAX>
+// CMS = true: green_RGF_CMS (src/green_rgf.f90:4167-4383), the compat solve
+// with the overlap blocks Sii: z S_l, Sigma^R_l S_l, Sigma^<_l S_l, the
+// decimation with S00, and pdens / GGi1 from the reference's G^> recursion
+// whose left-connected G^> is never filled: G^>_ll = g H^H G^>_{l+1} H g^H.
+template <int T, int NMAX, bool CMS = false>
 __device__ void solve_energy(const SweepArgs& a, int e, double2* ws, ldsd* lds, ldsd* red) {
   const int n = a.n, nx = a.nx;
   const size_t nn = (size_t)n * n;
@@ -161,29 +174,59 @@ __device__ void solve_energy(const SweepArgs& a, int e, double2* ws, ldsd* lds, 
   const Epi st = epi_store();
   int fails = 0, hitL = 0, hitR = 0;
 
+  auto Sb = [&](int l) -> const double2* { return CMS ? a.Sii + (size_t)l * nn : nullptr; };
+  // onsite block of slab l: H + Sigma^R (CMS: H + Sigma^R S)
+  auto onsite = [&](int l, double2* out) {
+    if (CMS && sR(l)) {
+      Epi ea = epi_store();
+      ea.D1 = Hb(l);
+      wg_gemm<T>(n, OP_N, sR(l), OP_N, Sb(l), out, 0, ea, lds);
+    } else {
+      wg_ew(n, out, 0, zero, Hb(l), one, CMS ? nullptr : sR(l), one, nullptr, one);
+    }
+  };
+  // lesser scattering source of slab l: Sigma^< (CMS: Sigma^< S, formed in T2)
+  auto lesser = [&](int l) -> const double2* {
+    if (CMS && sL(l)) {
+      wg_gemm<T>(n, OP_N, sL(l), OP_N, Sb(l), T2, 0, epi_store(), lds);
+      return T2;
+    }
+    return sL(l);
+  };
+
   // ---- left contact
-  wg_ew(n, H00, 0, zero, Hb(0), one, sR(0), one, nullptr, one);
+  onsite(0, H00);
   const int itL = wg_sancho<T, NMAX>(n, E, a.eta_lead, a.dec_tol, a.dec_nmax, H00, Cb(0), 1, G, stmp,
-                                     lds, red, hitL, fails);
+                                     lds, red, hitL, fails, Sb(0));
   wg_gemm<T>(n, OP_N, Cb(0), OP_N, G, T0, 0, st, lds);
   wg_gemm<T>(n, OP_N, T0, OP_C, Cb(0), SL, 0, st, lds);
   const double fL = ferm_dev((E - a.mu[0]) / (a.boltz * a.temp[0]));
-  wg_ew(n, sig, 0, zero, SL, make_double2(-fL, 0), sL(0), one, SL, make_double2(fL, 0));
-  wg_ew(n, W, 1, z, H00, mone, SL, mone, nullptr, one);
+  wg_ew(n, sig, 0, zero, SL, make_double2(-fL, 0), lesser(0), one, SL, make_double2(fL, 0));
+  if (CMS) {
+    wg_ew(n, T1, 0, zero, H00, one, SL, one, nullptr, one);
+    wg_ew(n, W, 1, zero, Sb(0), z, T1, mone, nullptr, one);
+  } else {
+    wg_ew(n, W, 1, z, H00, mone, SL, mone, nullptr, one);
+  }
   if (wg_inverse<T, NMAX>(n, W, gl(0), lds)) ++fails;
   wg_gemm<T>(n, OP_N, gl(0), OP_N, sig, T0, 0, st, lds);
   wg_gemm<T>(n, OP_N, T0, OP_C, gl(0), gln(0), 0, st, lds);
 
   // ---- forward sweep
   for (int l = 1; l < nx - 1; ++l) {
-    wg_gemm<T>(n, OP_N, Cb(l), OP_N, gl(l - 1), T0, 0, st, lds);
     Epi ew = epi_store();
-    ew.diag = z; ew.sacc = -1.0; ew.D1 = Hb(l); ew.s1 = mone; ew.D2 = sR(l); ew.s2 = mone;
+    if (CMS) {
+      onsite(l, T1);
+      ew.sacc = -1.0; ew.D1 = Sb(l); ew.s1 = z; ew.D2 = T1; ew.s2 = mone;
+    } else {
+      ew.diag = z; ew.sacc = -1.0; ew.D1 = Hb(l); ew.s1 = mone; ew.D2 = sR(l); ew.s2 = mone;
+    }
+    wg_gemm<T>(n, OP_N, Cb(l), OP_N, gl(l - 1), T0, 0, st, lds);
     wg_gemm<T>(n, OP_N, T0, OP_C, Cb(l), W, 1, ew, lds);
     if (wg_inverse<T, NMAX>(n, W, gl(l), lds)) ++fails;
-    wg_gemm<T>(n, OP_N, Cb(l), OP_N, gln(l - 1), T0, 0, st, lds);
     Epi es = epi_store();
-    es.D1 = sL(l);
+    es.D1 = lesser(l);
+    wg_gemm<T>(n, OP_N, Cb(l), OP_N, gln(l - 1), T0, 0, st, lds);
     wg_gemm<T>(n, OP_N, T0, OP_C, Cb(l), T1, 0, es, lds);
     wg_gemm<T>(n, OP_N, gl(l), OP_N, T1, T0, 0, st, lds);
     wg_gemm<T>(n, OP_N, T0, OP_C, gl(l), gln(l), 0, st, lds);
@@ -191,15 +234,20 @@ __device__ void solve_energy(const SweepArgs& a, int e, double2* ws, ldsd* lds, 
 
   // ---- right contact
   const int L1 = nx - 1;
-  wg_ew(n, H00, 0, zero, Hb(L1), one, sR(L1), one, nullptr, one);
+  onsite(L1, H00);
   const int itR = wg_sancho<T, NMAX>(n, E, a.eta_lead, a.dec_tol, a.dec_nmax, H00, Cb(L1), 0, G, stmp,
-                                     lds, red, hitR, fails);
+                                     lds, red, hitR, fails, Sb(L1));
   wg_gemm<T>(n, OP_C, Cb(L1), OP_N, G, T0, 0, st, lds);
   wg_gemm<T>(n, OP_N, T0, OP_N, Cb(L1), SR, 0, st, lds);
   wg_gemm<T>(n, OP_N, Cb(L1), OP_N, gl(L1 - 1), T0, 0, st, lds);
   {
     Epi ew = epi_store();
-    ew.diag = z; ew.sacc = -1.0; ew.D1 = H00; ew.s1 = mone; ew.D2 = SR; ew.s2 = mone;
+    if (CMS) {
+      wg_ew(n, T1, 0, zero, H00, one, SR, one, nullptr, one);
+      ew.sacc = -1.0; ew.D1 = Sb(L1); ew.s1 = z; ew.D2 = T1; ew.s2 = mone;
+    } else {
+      ew.diag = z; ew.sacc = -1.0; ew.D1 = H00; ew.s1 = mone; ew.D2 = SR; ew.s2 = mone;
+    }
     wg_gemm<T>(n, OP_N, T0, OP_C, Cb(L1), W, 1, ew, lds);
   }
   if (wg_inverse<T, NMAX>(n, W, G, lds)) ++fails;
@@ -207,7 +255,7 @@ __device__ void solve_energy(const SweepArgs& a, int e, double2* ws, ldsd* lds, 
   wg_gemm<T>(n, OP_N, Cb(L1), OP_N, gln(L1 - 1), T0, 0, st, lds);
   {
     Epi es = epi_store();
-    es.D1 = SR; es.s1 = make_double2(-fR, 0); es.D2 = sL(L1); es.s2 = one; es.D3 = SR; es.s3 = make_double2(fR, 0);
+    es.D1 = SR; es.s1 = make_double2(-fR, 0); es.D2 = lesser(L1); es.s2 = one; es.D3 = SR; es.s3 = make_double2(fR, 0);
     wg_gemm<T>(n, OP_N, T0, OP_C, Cb(L1), sig, 0, es, lds);
   }
   wg_gemm<T>(n, OP_N, G, OP_N, sig, T0, 0, st, lds);
@@ -276,15 +324,23 @@ __device__ void solve_energy(const SweepArgs& a, int e, double2* ws, ldsd* lds, 
       ea.D1 = T1;
       wg_gemm<T>(n, OP_N, Wm, OP_C, GN0, Gn, 0, ea, lds);     // + g< H^H G_{l,l+1}^H
     }
+    if (CMS) {
+      // G^>_ll = g H^H G^>_{l+1} H g^H  (the left-connected G^> of the reference is zero)
+      wg_gemm<T>(n, OP_N, X, OP_N, Gp, T0, 0, st, lds);
+      wg_gemm<T>(n, OP_N, T0, OP_C, X, Gp, 0, st, lds);
+    }
     if (a.ldos) wg_ew(n, ob(a.ldos, l), 0, zero, G, one, nullptr, one, nullptr, one);
     if (a.ndens) wg_ew(n, ob(a.ndens, l), 0, zero, Gn, one, nullptr, one, nullptr, one);
-    if (a.pdens) wg_ew(n, ob(a.pdens, l), 0, zero, Gn, one, G, one, G, mone);
+    if (a.pdens) {
+      if (CMS) wg_ew(n, ob(a.pdens, l), 0, zero, Gp, one, nullptr, one, nullptr, one);
+      else wg_ew(n, ob(a.pdens, l), 0, zero, Gn, one, G, one, G, mone);
+    }
     if (a.GRi1) wg_ew(n, ob(a.GRi1, l), 0, zero, GN0, one, nullptr, one, nullptr, one);
     if (a.GLi1) wg_ew(n, ob(a.GLi1, l), 0, zero, Y2, one, nullptr, one, nullptr, one);
     if (a.dens_n) { double v = wg_imtrace(n, Gn, red); if (threadIdx.x == 0) a.dens_n[(size_t)e * nx + l] = v; }
     if (a.dens_p) {
-      // -Im Tr G^>_ll with G^> = G^< + G - G^H
-      double v = wg_imtrace(n, Gn, red) + 2.0 * wg_imtrace(n, G, red);
+      // -Im Tr G^>_ll with G^> = G^< + G - G^H (CMS: the recursion's G^>)
+      double v = CMS ? wg_imtrace(n, Gp, red) : wg_imtrace(n, Gn, red) + 2.0 * wg_imtrace(n, G, red);
       if (threadIdx.x == 0) a.dens_p[(size_t)e * nx + l] = -v;
     }
   }
@@ -320,6 +376,16 @@ __global__ __launch_bounds__(Cfg<T>::NTH, 1) void rgf_sweep(SweepArgs a) {
   __shared__ double red[16];
   double2* ws = a.ws + (size_t)blockIdx.x * (size_t)a.ws_slot;
   for (int e = blockIdx.x; e < a.nE; e += gridDim.x) solve_energy<T, NMAX>(a, e, ws, (ldsd*)lds, (ldsd*)red);
+}
+
+// green_RGF_CMS: the same persistent loop with the overlap variant of the solve.
+template <int T, int NMAX>
+__global__ __launch_bounds__(Cfg<T>::NTH, 1) void rgf_sweep_cms(SweepArgs a) {
+  __shared__ double lds[SweepLds<T, NMAX>::DOUBLES];
+  __shared__ double red[16];
+  double2* ws = a.ws + (size_t)blockIdx.x * (size_t)a.ws_slot;
+  for (int e = blockIdx.x; e < a.nE; e += gridDim.x)
+    solve_energy<T, NMAX, true>(a, e, ws, (ldsd*)lds, (ldsd*)red);
 }
 
 // Batched building blocks: one workgroup per matrix (used by the block-level

@@ -48,7 +48,8 @@ class Batch(ctypes.Structure):
                 ("GRi1", ctypes.c_void_p), ("GLi1", ctypes.c_void_p), ("GGi1", ctypes.c_void_p),
                 ("cur", ctypes.c_void_p), ("tr", ctypes.c_void_p), ("tre", ctypes.c_void_p),
                 ("dens_n", ctypes.c_void_p), ("dens_p", ctypes.c_void_p), ("info", ctypes.c_void_p),
-                ("sigR_lo", ctypes.c_void_p), ("sigR_up", ctypes.c_void_p), ("sigL_up", ctypes.c_void_p)]
+                ("sigR_lo", ctypes.c_void_p), ("sigR_up", ctypes.c_void_p), ("sigL_up", ctypes.c_void_p),
+                ("Sii", ctypes.c_void_p)]
 
 
 class MultiReport(ctypes.Structure):
@@ -65,7 +66,7 @@ EXPORTS = ("rgf_params_reference", "rgf_last_error", "rgf_version", "rgf_rs", "r
            "rgf_rs_energies", "rgf_solve_batched", "rgf_workspace_bytes", "rgf_release",
            "rgf_zgemm_batched", "rgf_zinv_batched", "rgf_adjoint_batched", "rgf_adjoint_workspace_bytes",
            "rgf_scba_step", "rgf_graph_create", "rgf_graph_launch", "rgf_graph_destroy", "rgf_solve_devices",
-           "rgf_graph_create_devices")
+           "rgf_graph_create_devices", "rgf_cms", "rgf_cms_ex", "rgf_cms_energies")
 
 _lib = None
 
@@ -87,6 +88,9 @@ def lib():
     L.rgf_rs_energies.argtypes = ([ctypes.POINTER(Params), vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp,
                                    vp, vp, vp, vp, vp, ctypes.c_int] + [vp] * 10)
     L.rgf_solve_batched.argtypes = [ctypes.POINTER(Params), ctypes.POINTER(Batch), vp]
+    L.rgf_cms_ex.argtypes = [ctypes.POINTER(Params), vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, vp] + [vp] * 15
+    L.rgf_cms_energies.argtypes = ([ctypes.POINTER(Params), vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp,
+                                    vp, vp, vp, vp, vp, vp, ctypes.c_int] + [vp] * 10)
     L.rgf_scba_step.argtypes = [ctypes.POINTER(Params), ctypes.POINTER(Batch), ctypes.POINTER(Scba), vp]
     L.rgf_graph_create.argtypes = [ctypes.POINTER(Params), ctypes.POINTER(Batch), ctypes.POINTER(Scba), vp,
                                    ctypes.POINTER(vp)]
@@ -197,6 +201,27 @@ def rgf_rs_energies(p, temp, E, mu, Hii, H1i, sigL=None, sigG=None, sigR=None,
     out["tre"] = tre
     out["info"] = info
     return {k: v for k, v in out.items() if v is not None}
+
+
+def rgf_cms(temp, E, mu, Sii, Hii, H1i, sigL=None, sigG=None, sigR=None, p=None):
+    """One energy with green_RGF_CMS semantics (overlap Sii) on the GPU; the
+    outputs of rgf_rs."""
+    Sii, Hii, H1i = _cf(Sii), _cf(Hii), _cf(H1i)
+    sigL, sigG, sigR = _cf(sigL), _cf(sigG), _cf(sigR)
+    nm, _, nx = Hii.shape
+    shp = (nm, nm, nx)
+    out = {k: np.zeros(shp, np.complex128, order="F") for k in ("ndens", "pdens", "ldos", "GRi1", "GLi1", "GGi1")}
+    out["cur"] = np.zeros(shp, np.float64, order="F")
+    tr, tre = np.zeros(1), np.zeros(1)
+    t, m = np.ascontiguousarray(temp, np.float64), np.ascontiguousarray(mu, np.float64)
+    if p is None:
+        p = params()
+    _check(lib().rgf_cms_ex(ctypes.byref(p), _ptr(t), nm, nx, float(E), _ptr(m), _ptr(Sii), _ptr(Hii), _ptr(H1i),
+                            _ptr(sigL), _ptr(sigG), _ptr(sigR), _ptr(out["ndens"]), _ptr(out["pdens"]),
+                            _ptr(out["ldos"]), _ptr(tr), _ptr(tre), _ptr(out["cur"]), _ptr(out["GRi1"]),
+                            _ptr(out["GLi1"]), _ptr(out["GGi1"])))
+    out["tr"], out["tre"] = float(tr[0]), float(tre[0])
+    return out
 
 
 def _tp(t):

@@ -252,3 +252,49 @@ def rgf_offdiag(p, temp, E, mu, Hii, H1i, sigL=None, sigR=None, sigR_lo=None, si
     out["tr"], out["tre"] = tr.value, tre.value
     out["diag"] = (d.iters_left, d.iters_right, d.nmax_hit, d.inv_failures)
     return out
+
+
+def sancho_s(E, S00, H00, H10, eta_lead=1e-3, tol=1e-100, nmax=100):
+    """orc_sancho_s: the decimation with the overlap S00 of the lead cells."""
+    L = lib()
+    if not hasattr(L, "_ss"):
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.orc_sancho_s.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                   dp, dp, dp, dp, dp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+        L.orc_sancho_s.restype = ctypes.c_int
+        L._ss = True
+    S00, H00, H10 = _cf(S00), _cf(H00), _cf(H10)
+    n = H00.shape[0]
+    G00 = np.zeros((n, n), np.complex128, order="F")
+    GBB = np.zeros((n, n), np.complex128, order="F")
+    hit, fails = ctypes.c_int(0), ctypes.c_int(0)
+    it = L.orc_sancho_s(n, E, eta_lead, tol, nmax, _p(S00), _p(H00), _p(H10), _p(G00), _p(GBB),
+                        ctypes.byref(hit), ctypes.byref(fails))
+    return G00, GBB, it, hit.value, fails.value
+
+
+def rgf_cms(p, temp, E, mu, Sii, Hii, H1i, sigL=None, sigG=None, sigR=None):
+    """orc_rgf_cms: green_RGF_CMS semantics (one energy, overlap Sii)."""
+    L = lib()
+    if not hasattr(L, "_cms"):
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.orc_rgf_cms.argtypes = ([ctypes.POINTER(OrcParams), dp, ctypes.c_int, ctypes.c_int, ctypes.c_double, dp]
+                                  + [dp] * 15 + [ctypes.POINTER(OrcDiag)])
+        L._cms = True
+    Sii, Hii, H1i = _cf(Sii), _cf(Hii), _cf(H1i)
+    nm, _, nx = Hii.shape
+    sigs = [None if s is None else _cf(s) for s in (sigL, sigG, sigR)]
+    shp = (nm, nm, nx)
+    out = {k: np.zeros(shp, np.complex128, order="F") for k in ("ndens", "pdens", "ldos", "GRi1", "GLi1", "GGi1")}
+    out["cur"] = np.zeros(shp, np.float64, order="F")
+    tr, tre = ctypes.c_double(0.0), ctypes.c_double(0.0)
+    d = OrcDiag()
+    dp = ctypes.POINTER(ctypes.c_double)
+    t, m = np.asarray(temp, np.float64), np.asarray(mu, np.float64)
+    L.orc_rgf_cms(ctypes.byref(p), _p(t), nm, nx, float(E), _p(m), _p(Sii), _p(Hii), _p(H1i), *[_p(s) for s in sigs],
+                  _p(out["ndens"]), _p(out["pdens"]), _p(out["ldos"]), ctypes.cast(ctypes.byref(tr), dp),
+                  ctypes.cast(ctypes.byref(tre), dp), _p(out["cur"]), _p(out["GRi1"]), _p(out["GLi1"]),
+                  _p(out["GGi1"]), ctypes.byref(d))
+    out["tr"], out["tre"] = tr.value, tre.value
+    out["diag"] = (d.iters_left, d.iters_right, d.nmax_hit, d.inv_failures)
+    return out

@@ -261,12 +261,41 @@ static void mzminus(int n, double *d, double zr, double zi, const double *a, con
 static const double *blk(const double *a, int n, int l) { return a + 2 * (size_t)n * n * (size_t)l; }
 static double *blkw(double *a, int n, int l) { return a + 2 * (size_t)n * n * (size_t)l; }
 
+/* d = z*S - a (S may be NULL = identity) */
+static void mzs(int n, double *d, double zr, double zi, const double *S, const double *a) {
+  if (!S) {
+    mzminus(n, d, zr, zi, a, NULL);
+    return;
+  }
+  for (size_t k = 0; k < (size_t)n * n; ++k) {
+    RE(d, k) = zr * RE(S, k) - zi * IM(S, k) - RE(a, k);
+    IM(d, k) = zr * IM(S, k) + zi * RE(S, k) - IM(a, k);
+  }
+}
+
 /* ------------------------------------------------------------- sancho */
-/* Sancho-Rubio decimation, green_rgf.f90:4451-4537, with S00 = I and the
- * broadening, tolerance and cap as parameters. */
+/* Sancho-Rubio decimation, green_rgf.f90:4451-4537, with the overlap S00
+ * (NULL = identity, as green_RGF_RS passes) and the broadening, tolerance and
+ * cap as parameters. */
+static int sancho_impl(int n, double E, double eta_lead, double tol, int nmax, const double *S00,
+                       const double *H00, const double *H10, double *G00, double *GBB,
+                       int *nmax_hit, int *inv_failures);
+
 int orc_sancho(int n, double E, double eta_lead, double tol, int nmax,
                const double *H00, const double *H10, double *G00, double *GBB,
                int *nmax_hit, int *inv_failures) {
+  return sancho_impl(n, E, eta_lead, tol, nmax, NULL, H00, H10, G00, GBB, nmax_hit, inv_failures);
+}
+
+int orc_sancho_s(int n, double E, double eta_lead, double tol, int nmax, const double *S00,
+                 const double *H00, const double *H10, double *G00, double *GBB,
+                 int *nmax_hit, int *inv_failures) {
+  return sancho_impl(n, E, eta_lead, tol, nmax, S00, H00, H10, G00, GBB, nmax_hit, inv_failures);
+}
+
+static int sancho_impl(int n, double E, double eta_lead, double tol, int nmax, const double *S00,
+                       const double *H00, const double *H10, double *G00, double *GBB,
+                       int *nmax_hit, int *inv_failures) {
   double *A = mnew(n), *B = mnew(n), *C = mnew(n);
   double *HBB = mnew(n), *HSS = mnew(n), *H01 = mnew(n), *Hx10 = mnew(n);
   int fails = 0, it_run = 0;
@@ -282,7 +311,7 @@ int orc_sancho(int n, double E, double eta_lead, double tol, int nmax,
   mcopy(n, HSS, H00);
   for (int it = 1; it <= nmax; ++it) {
     it_run = it;
-    mzminus(n, A, E, eta_lead, HBB, NULL);
+    mzs(n, A, E, eta_lead, S00, HBB);
     if (orc_zinv(n, A)) ++fails;
     orc_zgemm('n', 'n', n, A, Hx10, B);
     orc_zgemm('n', 'n', n, H01, B, C);
@@ -308,10 +337,10 @@ int orc_sancho(int n, double E, double eta_lead, double tol, int nmax,
       mcopy(n, HBB, H00);
     }
   }
-  mzminus(n, G00, E, eta_lead, HSS, NULL);
+  mzs(n, G00, E, eta_lead, S00, HSS);
   if (orc_zinv(n, G00)) ++fails;
   if (GBB) {
-    mzminus(n, GBB, E, eta_lead, HBB, NULL);
+    mzs(n, GBB, E, eta_lead, S00, HBB);
     if (orc_zinv(n, GBB)) ++fails;
   }
   if (inv_failures) *inv_failures += fails;
@@ -780,4 +809,214 @@ void orc_rgf_offdiag(const orc_params *p, const double temp[2], int nm, int nx, 
   free(H00); free(Th); free(G00); free(A); free(B); free(C); free(S); free(V); free(d); free(u);
   free(sigmal); free(sigmar); free(GamL); free(GamR); free(G); free(Gn); free(Q); free(R); free(Glo);
   free(M); free(gs); free(QGn); free(Rd); free(Gnn); free(Gll); free(Gp); free(Z0);
+}
+
+
+/* ------------------------------------------------------------ overlap */
+/* green_RGF_CMS, green_rgf.f90:4167-4383: green_RGF_RS (compat arithmetic,
+ * same product order) with the overlap blocks Sii: z S_l instead of z,
+ * Sigma^R_l S_l and Sigma^<_l S_l instead of Sigma^R_l, Sigma^<_l (4198-4207,
+ * 4222-4237), the decimation with S00 (4200, 4244), and the G^> recursion of
+ * its backward sweep (4356-4374) whose left-connected G^> (Glp) is never
+ * filled, so pdens_l = g_l H^H pdens_{l+1} H g_l^H for l < nx and GGi1 reads
+ * that recursion's value (4306-4315).  tr uses TEMP(1) and tre TEMP(2) as in
+ * the reference. */
+void orc_rgf_cms(const orc_params *p, const double temp[2], int nm, int nx, double E,
+                 const double mu[2], const double *Sii, const double *Hii, const double *H1i,
+                 const double *sigL, const double *sigG, const double *sigR,
+                 double *ndens, double *pdens, double *ldos, double *tr, double *tre,
+                 double *cur, double *GRi1, double *GLi1, double *GGi1, orc_diag *diag) {
+  (void)sigG;
+  const int n = nm;
+  const size_t bs = 2 * (size_t)n * n;
+  const double mul = mu[0], mur = mu[1], templ = temp[0], tempr = temp[1];
+  const double zr = E, zi = p->eta_dev;
+  orc_diag dg = {0, 0, 0, 0};
+  double *Gl = (double *)calloc(bs * (size_t)nx, sizeof(double));
+  double *Gln = (double *)calloc(bs * (size_t)nx, sizeof(double));
+  double *S00 = mnew(n), *H00 = mnew(n), *H10 = mnew(n), *H10h = mnew(n), *A = mnew(n), *B = mnew(n);
+  double *C = mnew(n), *D = mnew(n), *G00 = mnew(n), *GBB = mnew(n), *GN0 = mnew(n), *Gn = mnew(n);
+  double *Gp = mnew(n), *sig = mnew(n), *sigmal = mnew(n), *sigmar = mnew(n), *T1 = mnew(n);
+  if (ldos) memset(ldos, 0, bs * (size_t)nx * sizeof(double));
+  if (ndens) memset(ndens, 0, bs * (size_t)nx * sizeof(double));
+  if (pdens) memset(pdens, 0, bs * (size_t)nx * sizeof(double));
+  if (cur) memset(cur, 0, (size_t)n * n * (size_t)nx * sizeof(double));
+#define ZERO_OR(a, l) ((a) ? blk((a), n, (l)) : NULL)
+  /* H00 = Hii + sigR S00 ; B = sigL S00 */
+#define ONSITE(l)                                                        \
+  do {                                                                   \
+    mcopy(n, S00, blk(Sii, n, (l)));                                     \
+    if (sigR) orc_zgemm('n', 'n', n, blk(sigR, n, (l)), S00, B);         \
+    else mzero(n, B);                                                    \
+    madd(n, H00, blk(Hii, n, (l)), B);                                   \
+  } while (0)
+  /* left contact, 4196-4216 */
+  ONSITE(0);
+  mcopy(n, H10, blk(H1i, n, 0));
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) {
+      size_t a = (size_t)i + (size_t)j * n, b = (size_t)j + (size_t)i * n;
+      RE(H10h, a) = RE(H10, b);
+      IM(H10h, a) = -IM(H10, b);
+    }
+  {
+    int hit = 0;
+    dg.iters_left = orc_sancho_s(n, E, p->eta_lead, p->dec_tol, p->dec_nmax, S00, H00, H10h, G00, GBB, &hit,
+                                 &dg.inv_failures);
+    if (hit) dg.nmax_hit |= 1;
+  }
+  orc_zgemm('n', 'n', n, H10, G00, A);
+  orc_zgemm('n', 'c', n, A, H10, sigmal);
+  if (sigL) orc_zgemm('n', 'n', n, blk(sigL, n, 0), S00, B); else mzero(n, B);
+  {
+    double f = orc_ferm((E - mul) / (p->boltz * templ));
+    mantiherm(n, sig, sigmal);
+    for (size_t k = 0; k < bs; ++k) sig[k] = -sig[k] * f + B[k];
+  }
+  mzs(n, A, zr, zi, S00, H00);
+  for (size_t k = 0; k < bs; ++k) A[k] -= sigmal[k];
+  if (orc_zinv(n, A)) ++dg.inv_failures;
+  mcopy(n, blkw(Gl, n, 0), A);
+  orc_zgemm('n', 'n', n, A, sig, B);
+  orc_zgemm('n', 'c', n, B, A, C);
+  mcopy(n, blkw(Gln, n, 0), C);
+  /* forward, 4217-4241 */
+  for (int l = 1; l < nx - 1; ++l) {
+    ONSITE(l);
+    mcopy(n, H10, blk(H1i, n, l));
+    orc_zgemm('n', 'n', n, H10, blk(Gl, n, l - 1), B);
+    orc_zgemm('n', 'c', n, B, H10, C);
+    mzs(n, A, zr, zi, S00, H00);
+    for (size_t k = 0; k < bs; ++k) A[k] -= C[k];
+    if (orc_zinv(n, A)) ++dg.inv_failures;
+    mcopy(n, blkw(Gl, n, l), A);
+    mcopy(n, sig, blk(Gln, n, l - 1));
+    orc_zgemm('n', 'n', n, H10, sig, B);
+    orc_zgemm('n', 'c', n, B, H10, C);
+    if (sigL) orc_zgemm('n', 'n', n, blk(sigL, n, l), S00, B); else mzero(n, B);
+    madd(n, C, C, B);
+    orc_zgemm('n', 'n', n, A, C, B);
+    orc_zgemm('n', 'c', n, B, A, Gn);
+    mcopy(n, blkw(Gln, n, l), Gn);
+  }
+  /* right contact, 4242-4287 */
+  ONSITE(nx - 1);
+  mcopy(n, H10, blk(H1i, n, nx - 1));
+  {
+    int hit = 0;
+    dg.iters_right = orc_sancho_s(n, E, p->eta_lead, p->dec_tol, p->dec_nmax, S00, H00, H10, G00, GBB, &hit,
+                                  &dg.inv_failures);
+    if (hit) dg.nmax_hit |= 2;
+  }
+  orc_zgemm('c', 'n', n, H10, G00, A);
+  orc_zgemm('n', 'n', n, A, H10, sigmar);
+  orc_zgemm('n', 'n', n, H10, blk(Gl, n, nx - 2), B);
+  orc_zgemm('n', 'c', n, B, H10, C);
+  mzs(n, G00, zr, zi, S00, H00);
+  for (size_t k = 0; k < bs; ++k) G00[k] -= sigmar[k] + C[k];
+  if (orc_zinv(n, G00)) ++dg.inv_failures;
+  if (ldos) mcopy(n, blkw(ldos, n, nx - 1), G00);
+  mcopy(n, sig, blk(Gln, n, nx - 2));
+  orc_zgemm('n', 'n', n, H10, sig, B);
+  orc_zgemm('n', 'c', n, B, H10, C);
+  if (sigL) orc_zgemm('n', 'n', n, blk(sigL, n, nx - 1), S00, B); else mzero(n, B);
+  const double fR = orc_ferm((E - mur) / (p->boltz * templ));
+  mantiherm(n, T1, sigmar);
+  for (size_t k = 0; k < bs; ++k) sig[k] = -T1[k] * fR + C[k] + B[k];
+  orc_zgemm('n', 'n', n, G00, sig, B);
+  orc_zgemm('n', 'c', n, B, G00, Gn);
+  if (ndens) mcopy(n, blkw(ndens, n, nx - 1), Gn);
+  mantiherm(n, Gp, G00);
+  madd(n, Gp, Gn, Gp);
+  if (pdens) mcopy(n, blkw(pdens, n, nx - 1), Gp);
+  for (size_t k = 0; k < bs; ++k) A[k] = -T1[k] * fR;
+  orc_zgemm('n', 'n', n, A, Gp, B);
+  for (size_t k = 0; k < bs; ++k) A[k] = -T1[k] * (fR - 1.0);
+  orc_zgemm('n', 'n', n, A, Gn, C);
+  {
+    double tim = 0.0;
+    for (int i = 0; i < n; ++i) {
+      size_t d = (size_t)i + (size_t)i * n;
+      tim = tim - (RE(B, d) - RE(C, d));
+    }
+    if (tr) *tr = tim;
+  }
+  /* backward, 4290-4379 (H1i(:,:,l) as the l,l+1 coupling, as the reference) */
+  for (int l = nx - 2; l >= 0; --l) {
+    mcopy(n, H10, blk(H1i, n, l));
+    const double *gl = blk(Gl, n, l), *gln = blk(Gln, n, l);
+    double *curl = cur ? cur + (size_t)n * n * (size_t)l : NULL;
+    orc_zgemm('n', 'c', n, H10, gl, B);
+    orc_zgemm('n', 'n', n, Gn, B, C);
+    orc_zgemm('n', 'n', n, H10, gln, B);
+    orc_zgemm('n', 'n', n, G00, B, A);
+    madd(n, B, C, A);
+    orc_zgemm('c', 'n', n, H10, B, A);
+    if (curl) for (size_t k = 0; k < (size_t)n * n; ++k) curl[k] = RE(A, k);
+    orc_zgemm('n', 'c', n, gl, H10, B);
+    orc_zgemm('n', 'n', n, B, Gn, C);
+    orc_zgemm('n', 'c', n, gln, H10, B);
+    orc_zgemm('n', 'c', n, B, G00, A);
+    madd(n, B, C, A);
+    if (GLi1) mcopy(n, blkw(GLi1, n, l), B);
+    orc_zgemm('n', 'n', n, H10, B, A);
+    if (curl) for (size_t k = 0; k < (size_t)n * n; ++k) curl[k] -= RE(A, k);
+    if (GGi1) {
+      /* g^H H^H G>_{l+1} + (Glp == 0) terms */
+      orc_zgemm('c', 'c', n, gl, H10, B);
+      orc_zgemm('n', 'n', n, B, Gp, C);
+      mcopy(n, blkw(GGi1, n, l), C);
+    }
+    mcopy(n, D, gl);
+    orc_zgemm('n', 'c', n, D, H10, B);
+    orc_zgemm('n', 'n', n, B, G00, GN0);
+    if (GRi1) mcopy(n, blkw(GRi1, n, l), GN0);
+    orc_zgemm('n', 'n', n, GN0, H10, A);
+    orc_zgemm('n', 'n', n, A, D, C);
+    madd(n, G00, gl, C);
+    if (ldos) mcopy(n, blkw(ldos, n, l), G00);
+    mcopy(n, A, Gn);
+    orc_zgemm('n', 'c', n, D, H10, B);
+    orc_zgemm('n', 'n', n, B, A, C);
+    orc_zgemm('n', 'n', n, C, H10, A);
+    orc_zgemm('n', 'c', n, A, D, C);
+    madd(n, Gn, gln, C);
+    mcopy(n, A, gln);
+    orc_zgemm('n', 'n', n, GN0, H10, B);
+    orc_zgemm('n', 'n', n, B, A, C);
+    madd(n, Gn, Gn, C);
+    orc_zgemm('n', 'c', n, A, H10, B);
+    orc_zgemm('n', 'c', n, B, GN0, C);
+    madd(n, Gn, Gn, C);
+    /* G>_ll recursion with Glp == 0: g H^H G> H g^H */
+    orc_zgemm('n', 'c', n, D, H10, B);
+    orc_zgemm('n', 'n', n, B, Gp, C);
+    orc_zgemm('n', 'n', n, C, H10, A);
+    orc_zgemm('n', 'c', n, A, D, Gp);
+    if (ndens) mcopy(n, blkw(ndens, n, l), Gn);
+    if (pdens) mcopy(n, blkw(pdens, n, l), Gp);
+  }
+  /* tre, 4380-4390: G^> from the identity, TEMP(2) */
+  mantiherm(n, Gp, G00);
+  madd(n, Gp, Gn, Gp);
+  {
+    double fL = orc_ferm((E - mul) / (p->boltz * tempr));
+    mantiherm(n, T1, sigmal);
+    for (size_t k = 0; k < bs; ++k) A[k] = -T1[k] * fL;
+    orc_zgemm('n', 'n', n, A, Gp, B);
+    for (size_t k = 0; k < bs; ++k) A[k] = -T1[k] * (fL - 1.0);
+    orc_zgemm('n', 'n', n, A, Gn, C);
+    double tim = 0.0;
+    for (int i = 0; i < n; ++i) {
+      size_t d = (size_t)i + (size_t)i * n;
+      tim = tim + (RE(B, d) - RE(C, d));
+    }
+    if (tre) *tre = tim;
+  }
+#undef ONSITE
+#undef ZERO_OR
+  if (diag) *diag = dg;
+  free(Gl); free(Gln);
+  free(S00); free(H00); free(H10); free(H10h); free(A); free(B); free(C); free(D); free(G00); free(GBB);
+  free(GN0); free(Gn); free(Gp); free(sig); free(sigmal); free(sigmar); free(T1);
 }

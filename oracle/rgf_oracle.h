@@ -88,6 +88,23 @@ void orc_rgf_fixed_leads(const orc_params *p, const double temp[2], int nm, int 
                          const double *leadR_H00, double *ndens, double *ldos, double *tr,
                          double *tre, orc_diag *diag);
 
+/* orc_sancho with the overlap S00 of the lead cells (green_rgf.f90:4481,
+ * 4521, 4525: z*S00 - H); orc_sancho is this with S00 = I. */
+int orc_sancho_s(int n, double E, double eta_lead, double tol, int nmax, const double *S00,
+                 const double *H00, const double *H10, double *G00, double *GBB,
+                 int *nmax_hit, int *inv_failures);
+
+/* green_RGF_CMS (green_rgf.f90:4167-4383): the compat RGF with overlap blocks
+ * Sii (z S_l, Sigma^R_l S_l, Sigma^<_l S_l, decimation with S00) and the
+ * reference's G^> recursion with the never-filled left-connected G^> (Glp = 0):
+ * pdens_l = g_l H1i_l^H pdens_{l+1} H1i_l g_l^H for l < nx, GGi1 from it.
+ * Same argument order as the reference (Sii before Hii). */
+void orc_rgf_cms(const orc_params *p, const double temp[2], int nm, int nx, double E,
+                 const double mu[2], const double *Sii, const double *Hii, const double *H1i,
+                 const double *sigL, const double *sigG, const double *sigR,
+                 double *ndens, double *pdens, double *ldos, double *tr, double *tre,
+                 double *cur, double *GRi1, double *GLi1, double *GGi1, orc_diag *diag);
+
 /* Physical-mode RGF with block-tridiagonal scattering self-energies (an
  * extension beyond green_RGF_RS, whose sigma arrays are diagonal blocks only,
  * src/green_rgf.f90:3952).  sigR, sigL: diagonal blocks as in orc_rgf;
