@@ -233,7 +233,8 @@ int rgf_graph_create_devices(const rgf_params *p, const rgf_batch *devs, int nde
 long long rgf_workspace_bytes(int nm, int nx, int slots);
 
 /* Adjoint (reverse-mode) RGF of the physical-mode device with the contact
- * self-energies held fixed.  Per energy e the objective is
+ * self-energies and the scattering self-energy held fixed (diagonal sigL/sigR
+ * and, when given, the off-diagonal sigR_lo/sigR_up/sigL_up of b).  Per energy e the objective is
  *   J_e = w_tr[e] * tr_e + sum_l w_n[e*nx + l] * Im Tr G^<_ll
  * (tr: current spectrum into the right contact, src/green_rgf.f90:4058-4066;
  * Im Tr G^<_ll: the density integrand of src/green_rgf.f90:3919-3945); w_tr

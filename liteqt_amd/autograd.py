@@ -4,7 +4,9 @@ RGFObjective.apply(U, T, ...) returns J = sum_e J_e with
   J_e = w_tr[e] * tr_e + sum_l w_n[e, l] * Im Tr G^<_ll(E_e)
 for the physical-mode device whose slab l carries an extra potential U[l] on
 its diagonal and whose device-internal couplings H1i[..., l] (l = 1..nx-2,
-0-based) are replaced by T[l-1].  The leads stay those of the unperturbed
+0-based) are replaced by T[l-1]; sig_off optionally holds the first
+off-diagonal scattering blocks (sigR_lo, sigR_up, sigL_up), held fixed like
+the diagonal ones.  The leads stay those of the unperturbed
 device: their onsite blocks are the edge slabs of Hii (plus sigR there), so a
 potential on an edge slab moves the device slab only.  Forward and gradients
 come from one call of rgf_adjoint_batched; the backward only scales the stored
@@ -17,7 +19,7 @@ from liteqt_amd import rgf
 
 class RGFObjective(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, U, Tint, p, E, temp, mu, Hii, H1i, sigL, sigR, w_tr, w_n):
+    def forward(ctx, U, Tint, p, E, temp, mu, Hii, H1i, sigL, sigR, w_tr, w_n, sig_off=None):
         # Hii, H1i: device tensors (nx, nm, nm) holding column-major blocks (to_device_blocks)
         nx, nm = Hii.shape[0], Hii.shape[-1]
         if sigR is not None and sigR.dim() == 4:
@@ -32,7 +34,7 @@ class RGFObjective(torch.autograd.Function):
         # Tint[k] is the column-major block of H1i at slab k+1 (same memory layout)
         Cm[1:nx - 1] = Tint
         J, dU, dH = rgf.adjoint_batched(p, E, temp, mu, Hm, Cm, sigL=sigL, sigR=sigR, w_tr=w_tr, w_n=w_n,
-                                        lead_H00=lead.contiguous())
+                                        lead_H00=lead.contiguous(), **(sig_off or {}))
         ctx.save_for_backward(dU.sum(0), dH[1:nx - 1].clone())
         return J.sum()
 
@@ -40,4 +42,4 @@ class RGFObjective(torch.autograd.Function):
     def backward(ctx, gJ):
         dU, dT = ctx.saved_tensors
         # dT holds dJ/dRe + i dJ/dIm of column-major blocks, the layout of Tint
-        return gJ * dU, gJ * dT, None, None, None, None, None, None, None, None, None, None
+        return gJ * dU, gJ * dT, None, None, None, None, None, None, None, None, None, None, None

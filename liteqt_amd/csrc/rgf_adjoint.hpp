@@ -2,24 +2,25 @@
 //   J(E) = w_tr(E) * tr(E) + sum_l w_n(E,l) * Im Tr G^<_ll(E)
 // with respect to a potential U_l on the diagonal of every slab and to the
 // device-internal coupling blocks H1i(:,:,l), l = 2..nx-1 (1-based), for the
-// consistent (physical) device with the contact self-energies held fixed.
+// consistent (physical) device with the contact self-energies and the
+// scattering self-energy (diagonal and, optionally, first off-diagonal
+// blocks) held fixed.
 // tr is the Meir-Wingreen current spectrum into the right contact
 // (src/green_rgf.f90:4058-4066): with f_L = 1, f_R = 0 it is T(E); with Fermi
 // factors and weights dE*e/h it gives dI/dU_l.  Densities are the integrands of
 // src/green_rgf.f90:3919-3945.
 //
-// Per energy a workgroup runs the forward sweep (stores g_l, g^<_l, s_l), the
-// backward sweep (stores G_ll, G^<_ll), the reverse of the backward sweep
-// (l = 1..nx-1, accumulating adjoints of g_l and g^<_l), the reverse of the
-// right boundary and the reverse of the forward sweep (l = nx-1..1), where
-// dJ/dU_l = -Re Tr(Abar_l) and the coupling adjoints are accumulated.  The
-// adjoint convention is Mbar = dJ/dRe M + i dJ/dIm M.
+// Per energy a workgroup runs the forward sweep, the backward sweep, the
+// reverse of the backward sweep and the reverse of the forward sweep (the
+// recursion is written out above adjoint_energy).  The adjoint convention is
+// Mbar = dJ/dRe M + i dJ/dIm M, so C = A B gives Abar += Cbar B^H,
+// Bbar += A^H Cbar and C = A^-1 gives Abar -= C^H Cbar C^H.
 #pragma once
 #include "rgf_sweep.hpp"
 
 namespace rgf {
 
-constexpr int kAdjTmp = 40;
+constexpr int kAdjTmp = 48;
 
 struct AdjArgs {
   int n, nx, nE, dec_nmax;
@@ -30,6 +31,9 @@ struct AdjArgs {
   const double2* H1i;
   const double2* sigL;
   const double2* sigR;
+  const double2* sigR_lo;  // Sigma^R_{l,l-1} (block l), or null
+  const double2* sigR_up;  // Sigma^R_{l-1,l}, or null
+  const double2* sigL_up;  // Sigma^<_{l-1,l}, or null
   long long sig_stride;
   const double* w_tr;  // nE or null (0)
   const double* w_n;   // nE x nx or null (0)
@@ -43,6 +47,16 @@ struct AdjArgs {
   long long ws_slot;
 };
 
+// Per energy, with d_l = H1i_l + Sigma^R_{l,l-1}, u_l = H1i_l^H + Sigma^R_{l-1,l},
+// s_l = Sigma^<_{l-1,l} (the plain couplings when the off-diagonal blocks are absent):
+//   forward   g_l = (z - H_l - Sigma^R_l - B_l u_l)^-1, B_l = d_l g_{l-1} (contacts on the
+//             edge slabs), g<_l = g_l S_l g_l^H, S_l = Sigma^<_l + Y_l d_l^H + V_l - V_l^H,
+//             Y_l = d_l g<_{l-1}, V_l = B_l s_l            (g, g<, S stored; slab nx-1 is G, G<)
+//   backward  Q = g_l u, R = Q G_{l+1}, G_l = g_l + R d g_l,
+//             G<_l = g<_l + Q G<_{l+1} Q^H + W - W^H, W = g_l s R^H + R d g<_l   (G, G< stored)
+//   pass A    reverse of the backward sweep, l = 0..nx-2 (Q, R, R d, Q G<, g s recomputed)
+//   pass B    reverse of the forward sweep, l = nx-1..0: dJ/dU_l = -Re Tr Abar_l
+// Coupling adjoints d-bar + u-bar^H accumulate per slab in dH.
 template <int T, int NMAX>
 __device__ void adjoint_energy(const AdjArgs& a, int e, double2* ws, double2* dH, ldsd* lds, ldsd* red) {
   const int n = a.n, N = a.nx;
@@ -50,29 +64,44 @@ __device__ void adjoint_energy(const AdjArgs& a, int e, double2* ws, double2* dH
   const double E = a.E[e];
   const double2 z = make_double2(E, a.eta_dev);
   const double2 one = make_double2(1, 0), mone = make_double2(-1, 0), zero = make_double2(0, 0);
-  double2* g = ws;
-  double2* gl = ws + (size_t)N * nn;
-  double2* sv = ws + 2 * (size_t)N * nn;
-  double2* G = ws + 3 * (size_t)N * nn;
-  double2* Gn = ws + 4 * (size_t)N * nn;
-  double2* gb = ws + 5 * (size_t)N * nn;
-  double2* glb = ws + 6 * (size_t)N * nn;
+  double2* gv = ws;
+  double2* glv = ws + (size_t)N * nn;
+  double2* Sv = ws + 2 * (size_t)N * nn;
+  double2* Gv = ws + 3 * (size_t)N * nn;
+  double2* Gnv = ws + 4 * (size_t)N * nn;
+  double2* gbv = ws + 5 * (size_t)N * nn;
+  double2* glbv = ws + 6 * (size_t)N * nn;
   double2* tb = ws + 7 * (size_t)N * nn;
-  auto B = [&](double2* base, int l) { return base + (size_t)l * nn; };
+  auto B_ = [&](double2* base, int l) { return base + (size_t)l * nn; };
   auto Tm = [&](int k) { return tb + (size_t)k * nn; };
-  double2 *H00 = Tm(0), *W = Tm(1), *T0 = Tm(2), *T1 = Tm(3), *SL = Tm(4), *SR = Tm(5);
-  double2 *X = Tm(6), *P = Tm(7), *R = Tm(8), *Q = Tm(9), *Wm = Tm(10);
-  double2 *Gb = Tm(11), *Lb = Tm(12), *Gb2 = Tm(13), *Lb2 = Tm(14), *Rb = Tm(15), *Qb = Tm(16);
-  double2 *Xb = Tm(17), *Wb = Tm(18), *Pb = Tm(19), *M = Tm(20), *Mb = Tm(21), *sb = Tm(22), *Ab = Tm(23);
-  double2 *K = Tm(24), *Kb = Tm(25), *Nn = Tm(26), *Nb = Tm(27), *Gs = Tm(28), *stmp = Tm(29);
+  double2 *H00 = Tm(0), *W = Tm(1), *T0 = Tm(2), *T1 = Tm(3), *T2 = Tm(4), *SL = Tm(5), *SR = Tm(6);
+  double2 *Dd = Tm(7), *Du = Tm(8), *V = Tm(9), *Q = Tm(10), *R = Tm(11), *Rd = Tm(12), *QGn = Tm(13);
+  double2 *gs = Tm(14), *Wb = Tm(15), *gsb = Tm(16), *Rb = Tm(17), *Rdb = Tm(18), *QGnb = Tm(19), *Qb = Tm(20);
+  double2 *Gb = Tm(21), *Lb = Tm(22), *Gb2 = Tm(23), *Lb2 = Tm(24), *M = Tm(25), *Mb = Tm(26), *Sb = Tm(27);
+  double2 *Ab = Tm(28), *Bb = Tm(29), *Yb = Tm(30), *Bm = Tm(31), *Y = Tm(32), *K = Tm(33), *Gs = Tm(34);
+  double2* stmp = Tm(36);  // 11 blocks for the decimation
+  const size_t so = (size_t)e * a.sig_stride;
   auto Hb = [&](int l) { return a.Hii + (size_t)l * nn; };
   auto Cb = [&](int l) { return a.H1i + (size_t)l * nn; };
-  auto sR = [&](int l) -> const double2* { return a.sigR ? a.sigR + (size_t)e * a.sig_stride + (size_t)l * nn : nullptr; };
-  auto sL = [&](int l) -> const double2* { return a.sigL ? a.sigL + (size_t)e * a.sig_stride + (size_t)l * nn : nullptr; };
+  auto sR = [&](int l) -> const double2* { return a.sigR ? a.sigR + so + (size_t)l * nn : nullptr; };
+  auto sL = [&](int l) -> const double2* { return a.sigL ? a.sigL + so + (size_t)l * nn : nullptr; };
+  auto sRlo = [&](int l) -> const double2* { return a.sigR_lo ? a.sigR_lo + so + (size_t)l * nn : nullptr; };
+  auto sRup = [&](int l) -> const double2* { return a.sigR_up ? a.sigR_up + so + (size_t)l * nn : nullptr; };
+  auto sLup = [&](int l) -> const double2* { return a.sigL_up ? a.sigL_up + so + (size_t)l * nn : nullptr; };
+  auto g = [&](int l) { return B_(gv, l); };
+  auto gl = [&](int l) { return B_(glv, l); };
+  auto Gf = [&](int l) { return l == N - 1 ? g(l) : B_(Gv, l); };    // G_ll
+  auto Gnf = [&](int l) { return l == N - 1 ? gl(l) : B_(Gnv, l); };  // G<_ll
   const double wtr = a.w_tr ? a.w_tr[e] : 0.0;
   auto wn = [&](int l) { return a.w_n ? a.w_n[(size_t)e * N + l] : 0.0; };
   const Epi st = epi_store();
   auto acc = [&](const double2* D) { Epi x = epi_store(); x.D1 = D; return x; };
+  Epi neg = epi_store();
+  neg.sacc = -1.0;
+  auto couple = [&](int l) {
+    wg_ew(n, Dd, 0, zero, Cb(l), one, sRlo(l), one, nullptr, one);
+    wg_ew(n, Du, 0, zero, sRup(l), one, nullptr, one, Cb(l), one);
+  };
   int fails = 0, hitL = 0, hitR = 0;
   double Jv = 0.0;
 
@@ -96,138 +125,131 @@ __device__ void adjoint_energy(const AdjArgs& a, int e, double2* ws, double2* dH
   const double fL = ferm_dev((E - a.mu[0]) / (a.boltz * a.temp[0]));
   const double fR = ferm_dev((E - a.mu[1]) / (a.boltz * a.temp[1]));
 
-  // ---- forward sweep, storing g, g^<, s
+  // ---- forward sweep, storing g, g<, S (slab N-1: G, G<)
   wg_ew(n, T1, 0, zero, Hb(0), one, sR(0), one, nullptr, one);
   wg_ew(n, W, 1, z, T1, mone, SL, mone, nullptr, one);  // z - (H_0 + sigR_0) - Sigma_L
-  if (wg_inverse<T, NMAX>(n, W, B(g, 0), lds)) ++fails;
-  wg_ew(n, B(sv, 0), 0, zero, SL, make_double2(-fL, 0), sL(0), one, SL, make_double2(fL, 0));
-  wg_gemm<T>(n, OP_N, B(g, 0), OP_N, B(sv, 0), T0, 0, st, lds);
-  wg_gemm<T>(n, OP_N, T0, OP_C, B(g, 0), B(gl, 0), 0, st, lds);
+  if (wg_inverse<T, NMAX>(n, W, g(0), lds)) ++fails;
+  wg_ew(n, B_(Sv, 0), 0, zero, SL, make_double2(-fL, 0), sL(0), one, SL, make_double2(fL, 0));
+  wg_gemm<T>(n, OP_N, g(0), OP_N, B_(Sv, 0), T0, 0, st, lds);
+  wg_gemm<T>(n, OP_N, T0, OP_C, g(0), gl(0), 0, st, lds);
   for (int l = 1; l < N; ++l) {
     const bool last = (l == N - 1);
-    wg_gemm<T>(n, OP_N, Cb(l), OP_N, B(g, l - 1), T0, 0, st, lds);
+    couple(l);
+    wg_gemm<T>(n, OP_N, Dd, OP_N, g(l - 1), T0, 0, st, lds);  // B = d g_{l-1}
     Epi ew = epi_store();
     ew.diag = z; ew.sacc = -1.0; ew.D1 = Hb(l); ew.s1 = mone; ew.D2 = sR(l); ew.s2 = mone;
     if (last) {
       wg_ew(n, T1, 0, zero, sR(l), one, SR, one, nullptr, one);  // sigR_N + Sigma_R
       ew.D2 = T1;
     }
-    wg_gemm<T>(n, OP_N, T0, OP_C, Cb(l), W, 1, ew, lds);
-    if (wg_inverse<T, NMAX>(n, W, last ? B(G, l) : B(g, l), lds)) ++fails;
-    wg_gemm<T>(n, OP_N, Cb(l), OP_N, B(gl, l - 1), T0, 0, st, lds);
+    wg_gemm<T>(n, OP_N, T0, OP_N, Du, W, 1, ew, lds);         // z - H - Sigma^R - B u
+    if (wg_inverse<T, NMAX>(n, W, g(l), lds)) ++fails;
     Epi es = epi_store();
     es.D1 = sL(l);
-    if (last) { es.D2 = SR; es.s2 = make_double2(-fR, 0); es.D3 = SR; es.s3 = make_double2(fR, 0); }
-    wg_gemm<T>(n, OP_N, T0, OP_C, Cb(l), B(sv, l), 0, es, lds);
-    double2* gg = last ? B(G, l) : B(g, l);
-    wg_gemm<T>(n, OP_N, gg, OP_N, B(sv, l), T0, 0, st, lds);
-    wg_gemm<T>(n, OP_N, T0, OP_C, gg, last ? B(Gn, l) : B(gl, l), 0, st, lds);
+    if (sLup(l)) {
+      wg_gemm<T>(n, OP_N, T0, OP_N, sLup(l), V, 0, st, lds);  // V = B s
+      es.D2 = V; es.D3 = V; es.s3 = mone;
+    }
+    if (last) {
+      wg_ew(n, T2, 0, zero, SR, make_double2(-fR, 0), sL(l), one, SR, make_double2(fR, 0));
+      es.D1 = T2;
+    }
+    wg_gemm<T>(n, OP_N, Dd, OP_N, gl(l - 1), T1, 0, st, lds);  // Y = d g<_{l-1}
+    wg_gemm<T>(n, OP_N, T1, OP_C, Dd, B_(Sv, l), 0, es, lds);  // S
+    wg_gemm<T>(n, OP_N, g(l), OP_N, B_(Sv, l), T0, 0, st, lds);
+    wg_gemm<T>(n, OP_N, T0, OP_C, g(l), gl(l), 0, st, lds);
   }
   // tr = -Re Tr(KR Gn_N) - fR Re Tr(KR (G_N - G_N^H)),  KR = -(SR - SR^H)
   wg_ew(n, K, 0, zero, SR, mone, nullptr, one, SR, one);
-  wg_ew(n, T1, 0, zero, B(G, N - 1), one, nullptr, one, B(G, N - 1), mone);
+  wg_ew(n, T1, 0, zero, g(N - 1), one, nullptr, one, g(N - 1), mone);
   {
-    const double a1 = wg_retrace_prod(n, K, B(Gn, N - 1), red);
+    const double a1 = wg_retrace_prod(n, K, gl(N - 1), red);
     const double a2 = wg_retrace_prod(n, K, T1, red);
     const double trv = -a1 - fR * a2;
     Jv += wtr * trv;
     if (a.tr && threadIdx.x == 0) a.tr[e] = trv;
   }
-  Jv += wn(N - 1) * wg_imtrace(n, B(Gn, N - 1), red);
+  Jv += wn(N - 1) * wg_imtrace(n, gl(N - 1), red);
 
-  // ---- backward sweep, storing G_ll and G^<_ll
+  // ---- backward sweep, storing G_ll and G<_ll
   for (int l = N - 2; l >= 0; --l) {
-    const double2* Tc = Cb(l + 1);
-    wg_gemm<T>(n, OP_N, B(g, l), OP_C, Tc, X, 0, st, lds);
-    wg_gemm<T>(n, OP_N, X, OP_N, B(G, l + 1), P, 0, st, lds);
-    wg_gemm<T>(n, OP_N, P, OP_N, Tc, R, 0, st, lds);
-    wg_gemm<T>(n, OP_N, R, OP_N, B(g, l), B(G, l), 0, acc(B(g, l)), lds);
-    wg_gemm<T>(n, OP_N, X, OP_N, B(Gn, l + 1), Q, 0, st, lds);
-    wg_gemm<T>(n, OP_N, Q, OP_C, X, T0, 0, acc(B(gl, l)), lds);
-    wg_gemm<T>(n, OP_N, R, OP_N, B(gl, l), T1, 0, acc(T0), lds);
-    wg_gemm<T>(n, OP_N, B(gl, l), OP_C, Tc, Wm, 0, st, lds);
-    wg_gemm<T>(n, OP_N, Wm, OP_C, P, B(Gn, l), 0, acc(T1), lds);
-    Jv += wn(l) * wg_imtrace(n, B(Gn, l), red);
+    const double2* s = sLup(l + 1);
+    couple(l + 1);
+    wg_gemm<T>(n, OP_N, g(l), OP_N, Du, Q, 0, st, lds);             // Q = g u
+    wg_gemm<T>(n, OP_N, Q, OP_N, Gf(l + 1), R, 0, st, lds);         // R = Q G
+    wg_gemm<T>(n, OP_N, R, OP_N, Dd, Rd, 0, st, lds);               // R d
+    wg_gemm<T>(n, OP_N, Rd, OP_N, g(l), Gf(l), 0, acc(g(l)), lds);  // G_l = g + R d g
+    wg_gemm<T>(n, OP_N, Q, OP_N, Gnf(l + 1), QGn, 0, st, lds);      // Q G<
+    wg_gemm<T>(n, OP_N, QGn, OP_C, Q, T0, 0, acc(gl(l)), lds);      // g< + Q G< Q^H
+    wg_gemm<T>(n, OP_N, Rd, OP_N, gl(l), T1, 0, st, lds);           // R d g<
+    if (s) {
+      wg_gemm<T>(n, OP_N, g(l), OP_N, s, gs, 0, st, lds);           // g s
+      wg_gemm<T>(n, OP_N, gs, OP_C, R, T2, 0, acc(T1), lds);        // W = g s R^H + R d g<
+      wg_ew(n, Gnf(l), 0, zero, T0, one, T2, one, T2, mone);
+    } else {
+      wg_ew(n, Gnf(l), 0, zero, T0, one, T1, one, T1, mone);
+    }
+    Jv += wn(l) * wg_imtrace(n, Gnf(l), red);
   }
 
-  // ---- reverse of the backward sweep, l = 0 .. N-2
+  // ---- pass A: reverse of the backward sweep, l = 0 .. N-2
   wg_ew(n, Gb, 0, zero, nullptr, one, nullptr, one, nullptr, one);
   wg_ew(n, Lb, 0, make_double2(0, wn(0)), nullptr, one, nullptr, one, nullptr, one);
   for (int l = 0; l < N - 1; ++l) {
-    const double2* Tc = Cb(l + 1);
+    const double2* s = sLup(l + 1);
     double2* Tbar = dH + (size_t)(l + 1) * nn;
-    wg_gemm<T>(n, OP_N, B(g, l), OP_C, Tc, X, 0, st, lds);
-    wg_gemm<T>(n, OP_N, X, OP_N, B(G, l + 1), P, 0, st, lds);
-    wg_gemm<T>(n, OP_N, P, OP_N, Tc, R, 0, st, lds);
-    wg_gemm<T>(n, OP_N, X, OP_N, B(Gn, l + 1), Q, 0, st, lds);
-    wg_gemm<T>(n, OP_N, B(gl, l), OP_C, Tc, Wm, 0, st, lds);
-    wg_gemm<T>(n, OP_C, R, OP_N, Gb, B(gb, l), 0, acc(Gb), lds);      // gb_l = Gb + R^H Gb
-    wg_gemm<T>(n, OP_N, Gb, OP_C, B(g, l), Rb, 0, st, lds);            // Rb = Gb g^H
-    wg_gemm<T>(n, OP_N, Lb, OP_C, B(gl, l), Rb, 0, acc(Rb), lds);      //    + Lb gl^H
-    wg_gemm<T>(n, OP_C, R, OP_N, Lb, B(glb, l), 0, acc(Lb), lds);     // glb_l = Lb + R^H Lb
-    wg_gemm<T>(n, OP_N, Lb, OP_N, X, Qb, 0, st, lds);                  // Qb = Lb X
-    wg_gemm<T>(n, OP_C, Lb, OP_N, Q, Xb, 0, st, lds);                  // Xb = Lb^H Q
-    wg_gemm<T>(n, OP_N, Lb, OP_N, P, Wb, 0, st, lds);                  // Wb = Lb P
-    wg_gemm<T>(n, OP_C, Lb, OP_N, Wm, Pb, 0, st, lds);                 // Pb = Lb^H W
-    wg_gemm<T>(n, OP_N, Wb, OP_N, Tc, B(glb, l), 0, acc(B(glb, l)), lds);
-    wg_gemm<T>(n, OP_C, Wb, OP_N, B(gl, l), Tbar, 0, acc(Tbar), lds);
-    wg_gemm<T>(n, OP_N, Qb, OP_C, B(Gn, l + 1), Xb, 0, acc(Xb), lds);
-    {
-      Epi d = epi_store();
-      d.diag = make_double2(0, wn(l + 1));
-      wg_gemm<T>(n, OP_C, X, OP_N, Qb, Lb2, 0, d, lds);                // Lb_{l+1} = X^H Qb + i w I
+    couple(l + 1);
+    wg_gemm<T>(n, OP_N, g(l), OP_N, Du, Q, 0, st, lds);
+    wg_gemm<T>(n, OP_N, Q, OP_N, Gf(l + 1), R, 0, st, lds);
+    wg_gemm<T>(n, OP_N, R, OP_N, Dd, Rd, 0, st, lds);
+    wg_gemm<T>(n, OP_N, Q, OP_N, Gnf(l + 1), QGn, 0, st, lds);
+    wg_ew(n, Wb, 0, zero, Lb, one, nullptr, one, Lb, mone);          // Wb = Lb - Lb^H
+    if (s) {
+      wg_gemm<T>(n, OP_N, g(l), OP_N, s, gs, 0, st, lds);
+      wg_gemm<T>(n, OP_N, Wb, OP_N, R, gsb, 0, st, lds);             // gsb = Wb R
+      wg_gemm<T>(n, OP_C, Wb, OP_N, gs, Rb, 0, st, lds);             // Rb = Wb^H g s
     }
-    wg_gemm<T>(n, OP_N, Rb, OP_C, Tc, Pb, 0, acc(Pb), lds);
-    wg_gemm<T>(n, OP_C, P, OP_N, Rb, Tbar, 0, acc(Tbar), lds);
-    wg_gemm<T>(n, OP_N, Pb, OP_C, B(G, l + 1), Xb, 0, acc(Xb), lds);
-    wg_gemm<T>(n, OP_C, X, OP_N, Pb, Gb2, 0, st, lds);                 // Gb_{l+1} = X^H Pb
-    wg_gemm<T>(n, OP_N, Xb, OP_N, Tc, B(gb, l), 0, acc(B(gb, l)), lds);
-    wg_gemm<T>(n, OP_C, Xb, OP_N, B(g, l), Tbar, 0, acc(Tbar), lds);
+    wg_gemm<T>(n, OP_N, Wb, OP_C, gl(l), Rdb, 0, st, lds);           // Rdb = Wb g<^H
+    wg_gemm<T>(n, OP_C, Rd, OP_N, Wb, B_(glbv, l), 0, acc(Lb), lds); // glb_l = Lb + (R d)^H Wb
+    wg_gemm<T>(n, OP_N, Lb, OP_N, Q, QGnb, 0, st, lds);              // QGnb = Lb Q
+    wg_gemm<T>(n, OP_C, Lb, OP_N, QGn, Qb, 0, st, lds);              // Qb = Lb^H Q G<
+    wg_gemm<T>(n, OP_C, Rd, OP_N, Gb, B_(gbv, l), 0, acc(Gb), lds);  // gb_l = Gb + (R d)^H Gb
+    wg_gemm<T>(n, OP_N, Gb, OP_C, g(l), Rdb, 0, acc(Rdb), lds);      // Rdb += Gb g^H
+    if (s)
+      wg_gemm<T>(n, OP_N, Rdb, OP_C, Dd, Rb, 0, acc(Rb), lds);       // Rb += Rdb d^H
+    else
+      wg_gemm<T>(n, OP_N, Rdb, OP_C, Dd, Rb, 0, st, lds);
+    wg_gemm<T>(n, OP_C, R, OP_N, Rdb, Tbar, 0, acc(Tbar), lds);      // d-bar += R^H Rdb
+    if (s) wg_gemm<T>(n, OP_N, gsb, OP_C, s, B_(gbv, l), 0, acc(B_(gbv, l)), lds);  // gb_l += gsb s^H
+    wg_gemm<T>(n, OP_N, QGnb, OP_C, Gnf(l + 1), Qb, 0, acc(Qb), lds);  // Qb += QGnb G<^H
+    wg_gemm<T>(n, OP_N, Rb, OP_C, Gf(l + 1), Qb, 0, acc(Qb), lds);     // Qb += Rb G^H
+    {
+      Epi dg = epi_store();
+      dg.diag = make_double2(0, wn(l + 1));
+      wg_gemm<T>(n, OP_C, Q, OP_N, QGnb, Lb2, 0, dg, lds);           // Lb_{l+1} = Q^H QGnb + i w I
+    }
+    wg_gemm<T>(n, OP_C, Q, OP_N, Rb, Gb2, 0, st, lds);               // Gb_{l+1} = Q^H Rb
+    wg_gemm<T>(n, OP_N, Qb, OP_C, Du, B_(gbv, l), 0, acc(B_(gbv, l)), lds);  // gb_l += Qb u^H
+    wg_gemm<T>(n, OP_C, Qb, OP_N, g(l), Tbar, 0, acc(Tbar), lds);   // u-bar^H = Qb^H g
     double2* t = Gb; Gb = Gb2; Gb2 = t;
     t = Lb; Lb = Lb2; Lb2 = t;
   }
-
-  // ---- objective on the last slab: Lb += -w KR^H, Gb += -w fR (KR^H - KR)
+  // objective terms on the last slab: Lb += -w KR^H, Gb += -w fR (KR^H - KR)
   wg_ew(n, Lb, 0, zero, Lb, one, nullptr, one, K, make_double2(-wtr, 0));
   wg_ew(n, Gb, 0, zero, Gb, one, K, make_double2(wtr * fR, 0), K, make_double2(-wtr * fR, 0));
 
-  // ---- reverse of the right boundary
-  {
-    const double2* GN = B(G, N - 1);
-    const double2* sN = B(sv, N - 1);
-    const double2* Tc = Cb(N - 1);
-    wg_gemm<T>(n, OP_N, GN, OP_N, sN, M, 0, st, lds);                  // M = G s
-    wg_gemm<T>(n, OP_N, Lb, OP_N, GN, Mb, 0, st, lds);                 // Mb = Lb G
-    wg_gemm<T>(n, OP_C, Lb, OP_N, M, Gb, 0, acc(Gb), lds);             // Gb += Lb^H M
-    wg_gemm<T>(n, OP_N, Mb, OP_C, sN, Gb, 0, acc(Gb), lds);            // Gb += Mb s^H
-    wg_gemm<T>(n, OP_C, GN, OP_N, Mb, sb, 0, st, lds);                 // sb = G^H Mb
-    wg_gemm<T>(n, OP_C, GN, OP_N, Gb, T0, 0, st, lds);
-    Epi ng = epi_store();
-    ng.sacc = -1.0;
-    wg_gemm<T>(n, OP_N, T0, OP_C, GN, Ab, 0, ng, lds);                 // Ab = -G^H Gb G^H
-    {
-      double tr_ab = 0.0;
-      for (int i = threadIdx.x; i < n; i += blockDim.x) tr_ab += gc(Ab)[i + (size_t)i * n].x;
-      tr_ab = block_sum(tr_ab, red);
-      if (threadIdx.x == 0) a.dU[(size_t)e * N + N - 1] = -tr_ab;
-    }
-    wg_gemm<T>(n, OP_N, Ab, OP_N, Tc, Kb, 0, ng, lds);                 // Kb = -Ab T
-    wg_gemm<T>(n, OP_C, Tc, OP_N, Kb, B(gb, N - 2), 0, acc(B(gb, N - 2)), lds);
-    wg_gemm<T>(n, OP_N, sb, OP_N, Tc, Nb, 0, st, lds);                 // Nb = sb T
-    wg_gemm<T>(n, OP_C, Tc, OP_N, Nb, B(glb, N - 2), 0, acc(B(glb, N - 2)), lds);
-  }
-
-  // ---- reverse of the forward sweep, l = N-2 .. 0
-  for (int l = N - 2; l >= 0; --l) {
-    wg_gemm<T>(n, OP_N, B(g, l), OP_N, B(sv, l), M, 0, st, lds);       // M = g s
-    wg_gemm<T>(n, OP_N, B(glb, l), OP_N, B(g, l), Mb, 0, st, lds);     // Mb = glb g
-    wg_gemm<T>(n, OP_C, B(glb, l), OP_N, M, B(gb, l), 0, acc(B(gb, l)), lds);
-    wg_gemm<T>(n, OP_N, Mb, OP_C, B(sv, l), B(gb, l), 0, acc(B(gb, l)), lds);
-    wg_gemm<T>(n, OP_C, B(g, l), OP_N, Mb, sb, 0, st, lds);            // sb = g^H Mb
-    wg_gemm<T>(n, OP_C, B(g, l), OP_N, B(gb, l), T0, 0, st, lds);
-    Epi ng = epi_store();
-    ng.sacc = -1.0;
-    wg_gemm<T>(n, OP_N, T0, OP_C, B(g, l), Ab, 0, ng, lds);            // Ab = -g^H gb g^H
+  // ---- pass B: reverse of the forward sweep, l = N-1 .. 0
+  for (int l = N - 1; l >= 0; --l) {
+    double2* gbl = (l == N - 1) ? Gb : B_(gbv, l);
+    const double2* glbl = (l == N - 1) ? Lb : B_(glbv, l);
+    const double2* Sl = B_(Sv, l);
+    wg_gemm<T>(n, OP_N, g(l), OP_N, Sl, M, 0, st, lds);              // M = g S
+    wg_gemm<T>(n, OP_N, glbl, OP_N, g(l), Mb, 0, st, lds);           // Mb = glb g
+    wg_gemm<T>(n, OP_C, glbl, OP_N, M, gbl, 0, acc(gbl), lds);       // gb += glb^H M
+    wg_gemm<T>(n, OP_N, Mb, OP_C, Sl, gbl, 0, acc(gbl), lds);        // gb += Mb S^H
+    wg_gemm<T>(n, OP_C, g(l), OP_N, Mb, Sb, 0, st, lds);             // Sb = g^H Mb
+    wg_gemm<T>(n, OP_C, g(l), OP_N, gbl, T0, 0, st, lds);
+    wg_gemm<T>(n, OP_N, T0, OP_C, g(l), Ab, 0, neg, lds);            // Ab = -g^H gb g^H
     {
       double tr_ab = 0.0;
       for (int i = threadIdx.x; i < n; i += blockDim.x) tr_ab += gc(Ab)[i + (size_t)i * n].x;
@@ -235,20 +257,25 @@ __device__ void adjoint_energy(const AdjArgs& a, int e, double2* ws, double2* dH
       if (threadIdx.x == 0) a.dU[(size_t)e * N + l] = -tr_ab;
     }
     if (l >= 1) {
-      const double2* Tc = Cb(l);
+      const double2* s = sLup(l);
       double2* Tbar = dH + (size_t)l * nn;
-      wg_gemm<T>(n, OP_N, Tc, OP_N, B(g, l - 1), K, 0, st, lds);       // K = T g_{l-1}
-      wg_gemm<T>(n, OP_N, Ab, OP_N, Tc, Kb, 0, ng, lds);               // Kb = -Ab T
+      couple(l);
+      wg_gemm<T>(n, OP_N, Dd, OP_N, g(l - 1), Bm, 0, st, lds);       // B = d g_{l-1}
+      wg_gemm<T>(n, OP_N, Ab, OP_C, Du, Bb, 0, neg, lds);            // Bb = -Ab u^H
       Epi na = epi_store();
       na.sacc = -1.0; na.D1 = Tbar;
-      wg_gemm<T>(n, OP_C, Ab, OP_N, K, Tbar, 0, na, lds);              // Tbar -= Ab^H K
-      wg_gemm<T>(n, OP_N, Kb, OP_C, B(g, l - 1), Tbar, 0, acc(Tbar), lds);
-      wg_gemm<T>(n, OP_C, Tc, OP_N, Kb, B(gb, l - 1), 0, acc(B(gb, l - 1)), lds);
-      wg_gemm<T>(n, OP_N, Tc, OP_N, B(gl, l - 1), Nn, 0, st, lds);     // N = T gl_{l-1}
-      wg_gemm<T>(n, OP_N, sb, OP_N, Tc, Nb, 0, st, lds);               // Nb = sb T
-      wg_gemm<T>(n, OP_C, sb, OP_N, Nn, Tbar, 0, acc(Tbar), lds);
-      wg_gemm<T>(n, OP_N, Nb, OP_C, B(gl, l - 1), Tbar, 0, acc(Tbar), lds);
-      wg_gemm<T>(n, OP_C, Tc, OP_N, Nb, B(glb, l - 1), 0, acc(B(glb, l - 1)), lds);
+      wg_gemm<T>(n, OP_C, Ab, OP_N, Bm, Tbar, 0, na, lds);           // u-bar^H = -Ab^H B
+      wg_gemm<T>(n, OP_N, Dd, OP_N, gl(l - 1), Y, 0, st, lds);       // Y = d g<_{l-1}
+      wg_gemm<T>(n, OP_N, Sb, OP_N, Dd, Yb, 0, st, lds);             // Yb = Sb d
+      wg_gemm<T>(n, OP_C, Sb, OP_N, Y, Tbar, 0, acc(Tbar), lds);     // d-bar += Sb^H Y
+      wg_gemm<T>(n, OP_N, Yb, OP_C, gl(l - 1), Tbar, 0, acc(Tbar), lds);  // d-bar += Yb g<^H
+      wg_gemm<T>(n, OP_C, Dd, OP_N, Yb, B_(glbv, l - 1), 0, acc(B_(glbv, l - 1)), lds);
+      if (s) {
+        wg_ew(n, V, 0, zero, Sb, one, nullptr, one, Sb, mone);       // Vb = Sb - Sb^H
+        wg_gemm<T>(n, OP_N, V, OP_C, s, Bb, 0, acc(Bb), lds);        // Bb += Vb s^H
+      }
+      wg_gemm<T>(n, OP_N, Bb, OP_C, g(l - 1), Tbar, 0, acc(Tbar), lds);  // d-bar += Bb g^H
+      wg_gemm<T>(n, OP_C, Dd, OP_N, Bb, B_(gbv, l - 1), 0, acc(B_(gbv, l - 1)), lds);
     }
   }
   if (threadIdx.x == 0) {

@@ -986,6 +986,7 @@ int rgf_adjoint_batched(const rgf_params* p, const rgf_batch* b, const double* w
   g_err.clear();
   if (int rc = check_params(p)) return rc;
   if (p->mode != RGF_MODE_PHYSICAL) return fail(RGF_EINVAL, "the adjoint differentiates the physical mode");
+  if (b && b->Sii) return fail(RGF_EUNSUP, "the adjoint does not take overlaps");
   if (!b || !b->E || !b->Hii || !b->H1i || !J || !dU) return fail(RGF_EINVAL, "missing arguments");
   if (b->nx < 2 || b->nE < 0) return fail(RGF_EINVAL, "need nx >= 2, nE >= 0");
   if (b->nE == 0) return 0;
@@ -1031,6 +1032,9 @@ int rgf_adjoint_batched(const rgf_params* p, const rgf_batch* b, const double* w
   a.H1i = reinterpret_cast<const double2*>(b->H1i);
   a.sigL = reinterpret_cast<const double2*>(b->sigL);
   a.sigR = reinterpret_cast<const double2*>(b->sigR);
+  a.sigR_lo = reinterpret_cast<const double2*>(b->sigR_lo);
+  a.sigR_up = reinterpret_cast<const double2*>(b->sigR_up);
+  a.sigL_up = reinterpret_cast<const double2*>(b->sigL_up);
   a.sig_stride = b->sig_stride;
   a.w_tr = w_tr; a.w_n = w_n; a.lead = reinterpret_cast<const double2*>(lead_H00); a.J = J; a.tr = b->tr; a.dU = dU;
   a.dH = reinterpret_cast<double2*>(dh.p);

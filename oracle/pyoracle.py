@@ -298,3 +298,32 @@ def rgf_cms(p, temp, E, mu, Sii, Hii, H1i, sigL=None, sigG=None, sigR=None):
     out["tr"], out["tre"] = tr.value, tre.value
     out["diag"] = (d.iters_left, d.iters_right, d.nmax_hit, d.inv_failures)
     return out
+
+
+def rgf_offdiag_fixed_leads(p, temp, E, mu, Hii, H1i, sigL=None, sigR=None, sigR_lo=None, sigR_up=None,
+                            sigL_up=None, leadL_H00=None, leadR_H00=None):
+    """orc_rgf_offdiag_fixed_leads: ndens and tr with the lead onsite blocks given
+    (default: the unperturbed edge blocks Hii + sigR)."""
+    L = lib()
+    if not hasattr(L, "_odfl"):
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.orc_rgf_offdiag_fixed_leads.argtypes = ([ctypes.POINTER(OrcParams), dp, ctypes.c_int, ctypes.c_int,
+                                                   ctypes.c_double, dp] + [dp] * 11)
+        L._odfl = True
+    Hii, H1i = _cf(Hii), _cf(H1i)
+    nm, _, nx = Hii.shape
+    sigs = [None if s is None else _cf(s) for s in (sigL, sigR, sigR_lo, sigR_up, sigL_up)]
+    sR = sigs[1]
+    if leadL_H00 is None:
+        leadL_H00 = Hii[:, :, 0] + (sR[:, :, 0] if sR is not None else 0)
+    if leadR_H00 is None:
+        leadR_H00 = Hii[:, :, nx - 1] + (sR[:, :, nx - 1] if sR is not None else 0)
+    leadL_H00, leadR_H00 = _cf(leadL_H00), _cf(leadR_H00)
+    ndens = np.zeros((nm, nm, nx), np.complex128, order="F")
+    tr = ctypes.c_double(0.0)
+    t, m = np.asarray(temp, np.float64), np.asarray(mu, np.float64)
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.orc_rgf_offdiag_fixed_leads(ctypes.byref(p), _p(t), nm, nx, float(E), _p(m), _p(Hii), _p(H1i),
+                                  *[_p(s) for s in sigs], _p(leadL_H00), _p(leadR_H00), _p(ndens),
+                                  ctypes.cast(ctypes.byref(tr), dp))
+    return {"ndens": ndens, "tr": tr.value}

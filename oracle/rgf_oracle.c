@@ -651,12 +651,40 @@ static void maddanti(int n, double *d, const double *a, const double *m) {
     }
 }
 
+static void offdiag_impl(const orc_params *p, const double temp[2], int nm, int nx, double E,
+                         const double mu[2], const double *Hii, const double *H1i,
+                         const double *sigL, const double *sigR, const double *sigR_lo,
+                         const double *sigR_up, const double *sigL_up, double *ndens,
+                         double *pdens, double *ldos, double *tr, double *tre, double *cur,
+                         double *GRi1, double *GLi1, double *GGi1, double *GRi1_lo, orc_diag *diag,
+                         const double *leadL_H00, const double *leadR_H00);
+
 void orc_rgf_offdiag(const orc_params *p, const double temp[2], int nm, int nx, double E,
                      const double mu[2], const double *Hii, const double *H1i,
                      const double *sigL, const double *sigR, const double *sigR_lo,
                      const double *sigR_up, const double *sigL_up, double *ndens,
                      double *pdens, double *ldos, double *tr, double *tre, double *cur,
                      double *GRi1, double *GLi1, double *GGi1, double *GRi1_lo, orc_diag *diag) {
+  offdiag_impl(p, temp, nm, nx, E, mu, Hii, H1i, sigL, sigR, sigR_lo, sigR_up, sigL_up, ndens, pdens, ldos, tr,
+               tre, cur, GRi1, GLi1, GGi1, GRi1_lo, diag, NULL, NULL);
+}
+
+void orc_rgf_offdiag_fixed_leads(const orc_params *p, const double temp[2], int nm, int nx, double E,
+                                 const double mu[2], const double *Hii, const double *H1i,
+                                 const double *sigL, const double *sigR, const double *sigR_lo,
+                                 const double *sigR_up, const double *sigL_up, const double *leadL_H00,
+                                 const double *leadR_H00, double *ndens, double *tr) {
+  offdiag_impl(p, temp, nm, nx, E, mu, Hii, H1i, sigL, sigR, sigR_lo, sigR_up, sigL_up, ndens, NULL, NULL, tr,
+               NULL, NULL, NULL, NULL, NULL, NULL, NULL, leadL_H00, leadR_H00);
+}
+
+static void offdiag_impl(const orc_params *p, const double temp[2], int nm, int nx, double E,
+                         const double mu[2], const double *Hii, const double *H1i,
+                         const double *sigL, const double *sigR, const double *sigR_lo,
+                         const double *sigR_up, const double *sigL_up, double *ndens,
+                         double *pdens, double *ldos, double *tr, double *tre, double *cur,
+                         double *GRi1, double *GLi1, double *GGi1, double *GRi1_lo, orc_diag *diag,
+                         const double *leadL_H00, const double *leadR_H00) {
   const int n = nm, N = nx;
   const size_t bs = 2 * (size_t)n * n, nn = (size_t)n * n;
   const double zr = E, zi = p->eta_dev;
@@ -671,6 +699,7 @@ void orc_rgf_offdiag(const orc_params *p, const double temp[2], int nm, int nx, 
 #define SB(a, l) ((a) ? blk((a), n, (l)) : NULL)
   /* leads: onsite Hii + sigR at the edge slabs, couplings H1i(:,:,1), H1i(:,:,nx) */
   if (sigR) madd(n, H00, blk(Hii, n, 0), blk(sigR, n, 0)); else mcopy(n, H00, blk(Hii, n, 0));
+  if (leadL_H00) mcopy(n, H00, leadL_H00);
   maddh(n, Th, G00, 1.0, blk(H1i, n, 0)); /* G00 is zero here: Th = H1i_1^H */
   {
     int hit = 0;
@@ -681,6 +710,7 @@ void orc_rgf_offdiag(const orc_params *p, const double temp[2], int nm, int nx, 
   orc_zgemm('n', 'n', n, blk(H1i, n, 0), G00, A);
   orc_zgemm('n', 'c', n, A, blk(H1i, n, 0), sigmal);
   if (sigR) madd(n, H00, blk(Hii, n, N - 1), blk(sigR, n, N - 1)); else mcopy(n, H00, blk(Hii, n, N - 1));
+  if (leadR_H00) mcopy(n, H00, leadR_H00);
   {
     int hit = 0;
     dg.iters_right = orc_sancho(n, E, p->eta_lead, p->dec_tol, p->dec_nmax, H00, blk(H1i, n, N - 1), G00,

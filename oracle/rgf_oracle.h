@@ -123,6 +123,15 @@ void orc_rgf_offdiag(const orc_params *p, const double temp[2], int nm, int nx, 
                      double *pdens, double *ldos, double *tr, double *tre, double *cur,
                      double *GRi1, double *GLi1, double *GGi1, double *GRi1_lo, orc_diag *diag);
 
+/* orc_rgf_offdiag with the lead decimations fed by explicit onsite blocks
+ * (NULL: the edge slabs plus sigR), for derivatives with the contacts held
+ * fixed; ndens and tr only. */
+void orc_rgf_offdiag_fixed_leads(const orc_params *p, const double temp[2], int nm, int nx, double E,
+                                 const double mu[2], const double *Hii, const double *H1i,
+                                 const double *sigL, const double *sigR, const double *sigR_lo,
+                                 const double *sigR_up, const double *sigL_up, const double *leadL_H00,
+                                 const double *leadR_H00, double *ndens, double *tr);
+
 /* Energy loop of the reference drivers (green_rgf.f90:1305-1316): OpenMP over
  * energies, one energy per thread.  Per-energy arrays have a trailing energy
  * index (stride nm*nm*nx complex); sig_stride_e is the energy stride of the

@@ -121,3 +121,76 @@ def test_autograd_function_gradcheck():
     Tint = Cd[1:nx - 1].clone().requires_grad_(True)
     f = lambda U, Tint: RGFObjective.apply(U, Tint, p, E, (300.0, 300.0), (0.1, -0.1), Hd, Cd, sLd, sRd, w_tr, w_n)
     assert torch.autograd.gradcheck(f, (U, Tint), eps=1e-6, atol=1e-6, rtol=1e-5, fast_mode=True)
+
+
+def _od_case(nm, nx, seed, E0):
+    Hii, H1i = configs.random_device(nm, nx, seed, identical_leads=False)
+    blocks = configs.scattering_blocks(nm, nx, seed + 1, scale=0.02)
+    po = orc.params(orc.PHYSICAL, eta_dev=1e-4, eta_lead=1e-4, dec_tol=1e-12)
+    return Hii, H1i, configs.offdiag_self_energies(blocks, fermi(E0, 0.0, po.boltz, 300.0))
+
+
+@pytest.mark.parametrize("nm,nx", [(32, 5), (64, 4)])
+def test_offdiag_adjoint_kernel_matches_reference_reverse_pass(nm, nx):
+    """Block-tridiagonal Sigma: the kernel against tests/adjoint_ref_od.py."""
+    from tests import adjoint_ref_od
+    torch, rgf = _env()
+    dev = torch.device("cuda:0")
+    E = np.array([-0.2, 0.15])
+    Hii, H1i, sig = _od_case(nm, nx, 60 + nm, 0.0)
+    kw = dict(eta_dev=1e-4, eta_lead=1e-4, dec_tol=1e-12)
+    p, po = rgf.params(rgf.MODE_PHYSICAL, **kw), orc.params(orc.PHYSICAL, **kw)
+    temp, mu = (300.0, 250.0), (0.1, -0.1)
+    rng = np.random.default_rng(nm)
+    w_tr, w_n = rng.standard_normal(len(E)), rng.standard_normal((len(E), nx))
+    names = ("sigL", "sigR", "sigR_lo", "sigR_up", "sigL_up")
+    sd = {k: rgf.to_device_blocks(v, dev) for k, v in zip(names, sig)}
+    Jg, dUg, dHg = rgf.adjoint_batched(p, torch.tensor(E, dtype=torch.float64, device=dev), temp, mu,
+                                       rgf.to_device_blocks(Hii, dev), rgf.to_device_blocks(H1i, dev),
+                                       w_tr=torch.tensor(w_tr, device=dev), w_n=torch.tensor(w_n, device=dev), **sd)
+    torch.cuda.synchronize()
+    Jg, dUg, dHg = Jg.cpu().numpy(), dUg.cpu().numpy(), rgf.from_device_blocks(dHg)
+    dH_ref = np.zeros((nm, nm, nx), complex)
+    for e, En in enumerate(E):
+        SigL, SigR = lead_self_energies(orc.sancho, En, Hii, H1i, sig[1], po.eta_lead, po.dec_tol, po.dec_nmax)
+        fL, fR = fermi(En, mu[0], po.boltz, temp[0]), fermi(En, mu[1], po.boltz, temp[1])
+        J, dU, dH = adjoint_ref_od.adjoint(En, po.eta_dev, Hii, H1i, *sig, SigL, SigR, fL, fR, w_tr[e], w_n[e])
+        assert abs(Jg[e] - J) <= 1e-10 * max(1.0, abs(J)), (e, Jg[e], J)
+        assert np.linalg.norm(dUg[e] - dU) <= 1e-9 * max(1.0, np.linalg.norm(dU)), (e, dUg[e], dU)
+        dH_ref += dH
+    assert np.linalg.norm(dHg - dH_ref) <= 1e-9 * max(1.0, np.linalg.norm(dH_ref))
+
+
+def test_offdiag_adjoint_density_gradient_matches_oracle_fd():
+    """dn_l/dU_m (the density adjoint) with block-tridiagonal Sigma against
+    central differences (h = 1e-6) of the oracle with contacts and Sigma fixed."""
+    torch, rgf = _env()
+    dev = torch.device("cuda:0")
+    nm, nx, E0 = 32, 8, 0.07
+    Hii, H1i, sig = _od_case(nm, nx, 99, E0)
+    kw = dict(eta_dev=1e-4, eta_lead=1e-4, dec_tol=1e-12)
+    p, po = rgf.params(rgf.MODE_PHYSICAL, **kw), orc.params(orc.PHYSICAL, **kw)
+    temp, mu = (300.0, 300.0), (0.1, -0.1)
+    w_n = np.zeros(nx)
+    w_n[3] = 1.0  # J = Im Tr G<_33: the density of slab 3
+    names = ("sigL", "sigR", "sigR_lo", "sigR_up", "sigL_up")
+    sd = {k: rgf.to_device_blocks(v, dev) for k, v in zip(names, sig)}
+    _, dU, _ = rgf.adjoint_batched(p, torch.tensor([E0], dtype=torch.float64, device=dev), temp, mu,
+                                   rgf.to_device_blocks(Hii, dev), rgf.to_device_blocks(H1i, dev),
+                                   w_n=torch.tensor(w_n[None, :], device=dev), **sd)
+    torch.cuda.synchronize()
+    dU = dU.cpu().numpy()[0]
+
+    def Jo(Hm):
+        o = orc.rgf_offdiag_fixed_leads(po, temp, E0, mu, Hm, H1i, *sig,
+                                        leadL_H00=Hii[:, :, 0] + sig[1][:, :, 0],
+                                        leadR_H00=Hii[:, :, nx - 1] + sig[1][:, :, nx - 1])
+        return np.imag(np.trace(o["ndens"][:, :, 3]))
+
+    h = 1e-6
+    for l in range(nx):
+        Hp, Hm = Hii.copy(order="F"), Hii.copy(order="F")
+        Hp[:, :, l] += h * np.eye(nm)
+        Hm[:, :, l] -= h * np.eye(nm)
+        fd = (Jo(Hp) - Jo(Hm)) / (2 * h)
+        assert abs(fd - dU[l]) <= 1e-6 * max(1.0, abs(fd)), (l, fd, dU[l])
