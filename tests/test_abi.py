@@ -103,3 +103,16 @@ def test_solve_devices_rejects_bad_devices_before_any_device_call():
     assert L.rgf_solve_devices(ctypes.byref(p), arr, 2, 0, None, None, None) == 4
     h = ctypes.c_void_p()
     assert L.rgf_graph_create_devices(ctypes.byref(p), arr, 2, 0, None, ctypes.byref(h)) == 1  # NULL stream
+
+
+def test_overlap_solve_is_switched_off_before_any_device_call():
+    """The overlap (green_RGF_CMS) GPU solve reports RGF_EUNSUP; the check runs
+    before the device is touched."""
+    from liteqt_amd import rgf as r
+    L = r.lib()
+    b = r.Batch()
+    b.nm, b.nx, b.nE = 32, 3, 1
+    b.E = b.Hii = b.H1i = b.Sii = 16
+    p = r.params(r.MODE_COMPAT)
+    assert L.rgf_solve_batched(ctypes.byref(p), ctypes.byref(b), None) == 4
+    assert b"switched off" in L.rgf_last_error()

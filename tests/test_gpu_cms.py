@@ -9,7 +9,9 @@ from oracle import pyoracle as orc
 from tests import parity
 from tests.test_oracle_cms import _overlap
 
-pytestmark = pytest.mark.gpu
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skip(reason="the overlap GPU solve is switched off: its first run at nm=64 ended in a GPU "
+                                      "memory fault with no cause found yet (the engine returns RGF_EUNSUP)")]
 
 
 def _rgf():
