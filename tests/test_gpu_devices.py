@@ -23,7 +23,7 @@ def _env():
     return torch, rgf
 
 
-def _devices(torch, rgf, dev, scatter_every=2):
+def _devices(torch, rgf, dev, scatter_every=1):
     host, devs = [], []
     for j, (nm, nx, nE) in enumerate(SPECS):
         Hii, H1i = configs.random_device(nm, nx, 300 + j, identical_leads=False)
@@ -67,26 +67,51 @@ def _check(rgf, po, mode, host, devs):
         assert (d["out"]["info"].cpu().numpy()[:, :2] == np.array([x[0:2] for x in r["diag"]])).all(), j
 
 
-@pytest.mark.parametrize("mode,autotune", [("compat", False), ("physical", True)])
-def test_devices_one_launch_matches_oracle(mode, autotune):
+def _solve_each(rgf, p, devs):
+    """The same devices, one rgf_solve_batched launch each (fresh output tensors)."""
+    outs = []
+    for d in devs:
+        o = {k: v.clone().zero_() for k, v in d["out"].items()}
+        rgf.solve_batched(p, d["E"], d["temp"], d["mu"], d["Hii"], d["H1i"], sigL=d.get("sigL"), sigR=d.get("sigR"),
+                          out=o)
+        outs.append(o)
+    return outs
+
+
+@pytest.mark.parametrize("mode", ["compat", "physical"])
+def test_devices_one_launch_equals_per_device_launches(mode):
+    """Without the autotuner every device runs the tile variant rgf_solve_batched
+    picks, so the one-launch results are bit-identical to per-device launches;
+    devices without scattering included."""
     torch, rgf = _env()
     dev = torch.device("cuda:0")
-    host, devs = _devices(torch, rgf, dev)
-    m = rgf.MODE_COMPAT if mode == "compat" else rgf.MODE_PHYSICAL
-    p = rgf.params(m, **KW)
-    po = orc.params(orc.COMPAT if mode == "compat" else orc.PHYSICAL, **KW)
-    if mode == "compat":
-        # compat backward sweep: uniform couplings keep the oracle pinned
-        for h, d in zip(host, devs):
-            h1 = configs.uniform_coupling(h[1])
-            h[1][...] = h1
-            d["H1i"].copy_(rgf.to_device_blocks(h1, dev))
-    tiles, rep = rgf.solve_devices(p, devs, autotune=autotune)
+    host, devs = _devices(torch, rgf, dev, scatter_every=2)
+    p = rgf.params(rgf.MODE_COMPAT if mode == "compat" else rgf.MODE_PHYSICAL, **KW)
+    tiles, rep = rgf.solve_devices(p, devs, autotune=False)
+    torch.cuda.synchronize()
+    assert rep["items"] == sum(s[2] for s in SPECS) and rep["slots"] >= 1 and rep["tuned"] == 0
+    ref = _solve_each(rgf, p, devs)
+    torch.cuda.synchronize()
+    for j, (d, r) in enumerate(zip(devs, ref)):
+        for k in r:
+            assert torch.equal(d["out"][k], r[k]), (j, k)
+
+
+def test_devices_autotuned_match_oracle():
+    """With the autotuner (tile variants may differ from rgf_solve_batched's)
+    every device's outputs against the CPU oracle at the parity bar; the devices
+    carry scattering self-energies (broadened states)."""
+    torch, rgf = _env()
+    dev = torch.device("cuda:0")
+    host, devs = _devices(torch, rgf, dev, scatter_every=1)
+    p = rgf.params(rgf.MODE_PHYSICAL, **KW)
+    po = orc.params(orc.PHYSICAL, **KW)
+    tiles, rep = rgf.solve_devices(p, devs, autotune=True)
     torch.cuda.synchronize()
     assert rep["items"] == sum(s[2] for s in SPECS) and rep["slots"] >= 1
     for (nm, _, _), t in zip(SPECS, tiles):
         assert nm % (32 * t) == 0 or (t == 4 and nm % 128 == 0), (nm, t)
-    _check(rgf, po, mode, host, devs)
+    _check(rgf, po, "physical", host, devs)
 
 
 def test_devices_graph_replays_match_eager():
