@@ -229,12 +229,13 @@ def random_device_torch(torch, nm, nx, gen, dev, u_range=0.3, coupling_sigma=0.5
     return H.contiguous(), C.contiguous()
 
 
-def devices_cost(rgf, torch, dev, ndev, e_div, seed=3):
+def devices_cost(rgf, torch, dev, ndev, e_div, compare, seed=3):
     """64 uneven devices in one persistent launch: nm in {64, 96, 128, 192, 256},
     nx ~ U[50, 400], energy count ~ U[256, 2048] divided by e_div (energies in
     [-2, 2] eV), eta 1e-5, Fermi leads, compat mode, tr/tre/dens per energy.
-    Timed: the autotuned one-launch solve (rgf_solve_devices) and, for
-    comparison, the same devices one rgf_solve_batched launch after another."""
+    Timed: the autotuned one-launch solve (rgf_solve_devices; the first call
+    includes the autotuner) and, with `compare`, the same devices one
+    rgf_solve_batched launch after another."""
     rgf.lib().rgf_release()
     torch.cuda.empty_cache()
     rng = np.random.default_rng(seed)
@@ -270,15 +271,17 @@ def devices_cost(rgf, torch, dev, ndev, e_div, seed=3):
     flops = 0.0
     for (nm, nx, nE), d in zip(shapes, devs):
         flops += float(np.sum(algorithmic_flops_per_energy(nm, nx, d["out"]["info"].cpu().numpy())))
-    tr_one = [d["out"]["tr"].clone() for d in devs]
-    rgf.lib().rgf_release()
-    t0 = time.perf_counter()
-    for d in devs:
-        rgf.solve_batched(p, d["E"], temp, mu, d["Hii"], d["H1i"], out=d["out"])
-    torch.cuda.synchronize()
-    t_seq = time.perf_counter() - t0
-    print(f"bench: {ndev} devices one launch each {t_seq:.2f}s", file=sys.stderr, flush=True)
-    same = all(torch.equal(a, d["out"]["tr"]) for a, d in zip(tr_one, devs))
+    t_seq, same = None, None
+    if compare:
+        tr_one = [d["out"]["tr"].clone() for d in devs]
+        rgf.lib().rgf_release()
+        t0 = time.perf_counter()
+        for d in devs:
+            rgf.solve_batched(p, d["E"], temp, mu, d["Hii"], d["H1i"], out=d["out"])
+        torch.cuda.synchronize()
+        t_seq = time.perf_counter() - t0
+        print(f"bench: {ndev} devices one launch each {t_seq:.2f}s", file=sys.stderr, flush=True)
+        same = all(torch.equal(a, d["out"]["tr"]) for a, d in zip(tr_one, devs))
     rgf.lib().rgf_release()
     del devs
     torch.cuda.empty_cache()
@@ -288,7 +291,7 @@ def devices_cost(rgf, torch, dev, ndev, e_div, seed=3):
                         "in [-2,2] eV, eta 1e-5, Fermi leads, compat mode; tr, tre, dens per energy",
             "devices": ndev, "energy_points": items, "s_one_launch": t_one, "s_first_call_with_autotune": t_first,
             "energy_points_per_s": items / t_one, "achieved_tflops": achieved, "frac": achieved / PEAK_FP64_MFMA,
-            "s_one_launch_per_device": t_seq, "one_launch_over_per_device": t_seq / t_one,
+            "s_one_launch_per_device": t_seq, "one_launch_over_per_device": (t_seq / t_one) if t_seq else None,
             "tr_identical_to_per_device_launches": bool(same), "slots": rep["slots"],
             "workspace_gib": rep["ws_bytes"] / 2 ** 30,
             "tile_T_by_nm": {str(nm): int(tiles[[s[0] for s in shapes].index(nm)]) for nm in sizes}}
@@ -310,6 +313,8 @@ def main():
     ap.add_argument("--scba-iterations", type=int, default=10)
     ap.add_argument("--no-devices", action="store_true", help="skip the 64-device one-launch measurement")
     ap.add_argument("--devices", type=int, default=64)
+    ap.add_argument("--devices-compare", action="store_true",
+                    help="also time the devices one rgf_solve_batched launch after another")
     ap.add_argument("--devices-energy-div", type=int, default=32,
                     help="divide each device's energy count (U[256,2048]) by this")
     ap.add_argument("--plain-energies", type=int, default=4,
@@ -413,7 +418,7 @@ def main():
         scba = scba_cost(rgf, torch, dev, args.scba_energies, args.scba_iterations)
     devices = None
     if not args.no_devices:
-        devices = devices_cost(rgf, torch, dev, args.devices, args.devices_energy_div)
+        devices = devices_cost(rgf, torch, dev, args.devices, args.devices_energy_div, args.devices_compare)
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tfile):
