@@ -20,6 +20,9 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int PAD = 16;   // LDS row padding (doubles): row stride = 16 mod 32 doubles
 constexpr int KP = 16;    // K panel depth (complex)
+// Row stride (doubles) of an LDS panel stored k-contiguous ([row][k]): 2 mod 32, so
+// the fragment reads of 16 rows x 2 k-steps per half-wave hit distinct 8-byte bank pairs.
+constexpr int LDK = KP + 2;
 
 enum { OP_N = 0, OP_C = 1 };
 
@@ -187,6 +190,7 @@ struct Cfg {
   static constexpr int PERB = (MC * KP) / NTH;
   static_assert(TA * 16 * WR == MR && TB * 16 * WC == MC, "wave grid must tile the macro tile");
   static_assert((MR * KP) % NTH == 0 && (MC * KP) % NTH == 0, "panels must split evenly");
+  static_assert(MR * LDK <= KP * LDA && MC * LDK <= KP * LDB, "k-contiguous panels fit the planes");
 };
 
 // ----------------------------------------------------------- MFMA micro-tile
@@ -194,7 +198,9 @@ struct Cfg {
 // complex tiles.  A values at Ar/Ai[k*lda + arow + m], B at Br/Bi[k*ldb + bcol + m].
 // ROWMAJ=0: lane holds C[arow + ti*16 + l%16][bcol + tj*16 + 4r + l/16];
 // ROWMAJ=1: lane holds C[arow + ti*16 + 4r + l/16][bcol + tj*16 + l%16].
-template <int TA, int TB, int ROWMAJ, int NKS>
+// AKC / BKC: the panel is stored k-contiguous (A at Ar[(arow + m) * LDK + k], B at
+// Br[(bcol + m) * LDK + k]) instead of row-contiguous (A at Ar[k * lda + arow + m]).
+template <int TA, int TB, int ROWMAJ, int NKS, bool AKC = false, bool BKC = false>
 __device__ __forceinline__ void mma_steps(d4 (&cr)[TA][TB], d4 (&ci)[TA][TB],
                                           const ldsd* Ar, const ldsd* Ai, int lda, int arow,
                                           const ldsd* Br, const ldsd* Bi, int ldb, int bcol) {
@@ -208,14 +214,16 @@ __device__ __forceinline__ void mma_steps(d4 (&cr)[TA][TB], d4 (&ci)[TA][TB],
     double ar[TA], ai[TA], nai[TA], br[TB], bi[TB];
 #pragma unroll
     for (int t = 0; t < TA; ++t) {
-      ar[t] = Ar[k * lda + arow + t * 16 + m];
-      ai[t] = Ai[k * lda + arow + t * 16 + m];
+      const int ia = AKC ? (arow + t * 16 + m) * LDK + k : k * lda + arow + t * 16 + m;
+      ar[t] = Ar[ia];
+      ai[t] = Ai[ia];
       nai[t] = -ai[t];
     }
 #pragma unroll
     for (int t = 0; t < TB; ++t) {
-      br[t] = Br[k * ldb + bcol + t * 16 + m];
-      bi[t] = Bi[k * ldb + bcol + t * 16 + m];
+      const int ib = BKC ? (bcol + t * 16 + m) * LDK + k : k * ldb + bcol + t * 16 + m;
+      br[t] = Br[ib];
+      bi[t] = Bi[ib];
     }
 #pragma unroll
     for (int ti = 0; ti < TA; ++ti)
@@ -299,16 +307,19 @@ __device__ __forceinline__ void gemm_store_panel(ldsd* st, const double2 (&ra)[C
     const int idx = q * NTH + tid;
     int i, k;
     if (OPA == OP_N) { i = idx % MR; k = idx / MR; } else { k = idx % KP; i = idx / KP; }
-    Ar[k * LDA + i] = ra[q].x;
-    Ai[k * LDA + i] = ra[q].y;
+    // k-contiguous layout when the global read runs along k (OP_C): conflict-free stores
+    const int o = (OPA == OP_N) ? k * LDA + i : i * LDK + k;
+    Ar[o] = ra[q].x;
+    Ai[o] = ra[q].y;
   }
 #pragma unroll
   for (int q = 0; q < Cfg<T>::PERB; ++q) {
     const int idx = q * NTH + tid;
     int j, k;
     if (OPB == OP_N) { k = idx % KP; j = idx / KP; } else { j = idx % MC; k = idx / MC; }
-    Br[k * LDB + j] = rb[q].x;
-    Bi[k * LDB + j] = rb[q].y;
+    const int o = (OPB == OP_N) ? j * LDK + k : k * LDB + j;
+    Br[o] = rb[q].x;
+    Bi[o] = rb[q].y;
   }
 }
 
@@ -353,10 +364,11 @@ __device__ __noinline__ double wg_gemm_k(int n_, const double2* __restrict__ A_,
       const ldsd* sAi = sAr + KP * LDA;
       const ldsd* sBr = sAi + KP * LDA;
       const ldsd* sBi = sBr + KP * LDB;
+      constexpr bool AKC = (OPA == OP_C), BKC = (OPB == OP_N);
       if (c_rowmajor)
-        mma_steps<TA, TB, 1, KP / 4>(cr, ci, sAr, sAi, LDA, arow, sBr, sBi, LDB, bcol);
+        mma_steps<TA, TB, 1, KP / 4, AKC, BKC>(cr, ci, sAr, sAi, LDA, arow, sBr, sBi, LDB, bcol);
       else
-        mma_steps<TA, TB, 0, KP / 4>(cr, ci, sAr, sAi, LDA, arow, sBr, sBi, LDB, bcol);
+        mma_steps<TA, TB, 0, KP / 4, AKC, BKC>(cr, ci, sAr, sAi, LDA, arow, sBr, sBi, LDB, bcol);
       gemm_store_panel<T, OPA, OPB>(lds + ((p + 1) & 1) * STAGE, ra, rb);
       __syncthreads();
     }
