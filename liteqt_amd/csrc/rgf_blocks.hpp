@@ -486,54 +486,75 @@ __device__ __noinline__ int wg_inverse(int n, double2* W_, double2* out_, ldsd* 
   ldsi* redi = (ldsi*)(redv + 4);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int arow = (wave % C::WR) * 16 * TA, bcol = (wave / C::WR) * 16 * TB;
+  static_assert(NMAX <= NTH, "one panel row per thread");
   for (int i = tid; i < n; i += NTH) used[i] = 0;
   __syncthreads();
   int info = 0;
+  const bool own = tid < n;  // thread tid holds row tid of the current panel in registers
+  bool myused = false;
   for (int k0 = 0; k0 < n && !info; k0 += KP) {
-    for (int idx = tid; idx < n * KP; idx += NTH) {
-      const int c = idx % KP, i = idx / KP;
-      const double2 v = W[(size_t)i * n + k0 + c];
-      Mr[c * LDI + i] = v.x;
-      Mi[c * LDI + i] = v.y;
-    }
-    __syncthreads();
+    double2 rv[KP];
+#pragma unroll
+    for (int c2 = 0; c2 < KP; ++c2) rv[c2] = own ? W[(size_t)tid * n + k0 + c2] : make_double2(0, 0);
     for (int c = 0; c < KP; ++c) {
+      // pivot: first maximum of |Re|+|Im| over unused rows (wave reduction, then one
+      // exchange of the 4 wave winners and their rows through LDS; Br/Bi hold the
+      // exchange area until the snapshot below, double-buffered by column parity)
       double best = -1.0;
       int bi = 0x7fffffff;
-      for (int i = tid; i < n; i += NTH) {
-        if (!used[i]) {
-          const double v = fabs(Mr[c * LDI + i]) + fabs(Mi[c * LDI + i]);
-          if (v > best) { best = v; bi = i; }
+      double2 cand = make_double2(0, 0);
+#pragma unroll
+      for (int c2 = 0; c2 < KP; ++c2)
+        if (c2 == c) cand = rv[c2];
+      if (own && !myused) { best = fabs(cand.x) + fabs(cand.y); bi = tid; }
+      for (int off = 32; off > 0; off >>= 1) {
+        const double ov = __shfl_xor(best, off, 64);
+        const int oi = __shfl_xor(bi, off, 64);
+        if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+      }
+      ldsd* xrow = Br + (c & 1) * (4 * 2 * KP);  // [wave][c2] re, im
+      ldsd* xval = Bi + (c & 1) * 8;               // [wave] value, [4 + wave] row index
+      if (tid == bi) {
+#pragma unroll
+        for (int c2 = 0; c2 < KP; ++c2) {
+          xrow[wave * 2 * KP + 2 * c2] = rv[c2].x;
+          xrow[wave * 2 * KP + 2 * c2 + 1] = rv[c2].y;
         }
       }
-      block_argmax(best, bi, redv, redi);
+      if (lane == 0) { xval[wave] = best; xval[4 + wave] = (double)bi; }
+      __syncthreads();
+      best = xval[0];
+      bi = (int)xval[4];
+      for (int w = 1; w < NTH / 64; ++w) {
+        const double ov = xval[w];
+        const int oi = (int)xval[4 + w];
+        if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+      }
       if (!(best > 0.0)) { info = k0 + c + 1; break; }
       const int p = bi;
-      const double2 a = make_double2(Mr[c * LDI + p], Mi[c * LDI + p]);
-      const double2 ainv = crecip(a);
+      const ldsd* prow = xrow + (p >> 6) * 2 * KP;
+      const double2 ainv = crecip(make_double2(prow[2 * c], prow[2 * c + 1]));
       double2 rp[KP];
 #pragma unroll
       for (int c2 = 0; c2 < KP; ++c2)
-        rp[c2] = (c2 == c) ? ainv : cmul(make_double2(Mr[c2 * LDI + p], Mi[c2 * LDI + p]), ainv);
-      __syncthreads();
-      if (tid == 0) { used[p] = 1; piv[k0 + c] = p; }
-      for (int i = tid; i < n; i += NTH) {
-        if (i == p) {
+        rp[c2] = (c2 == c) ? ainv : cmul(make_double2(prow[2 * c2], prow[2 * c2 + 1]), ainv);
+      if (tid == p) {
+        myused = true;
+        used[p] = 1;
+        piv[k0 + c] = p;
 #pragma unroll
-          for (int c2 = 0; c2 < KP; ++c2) { Mr[c2 * LDI + i] = rp[c2].x; Mi[c2 * LDI + i] = rp[c2].y; }
-        } else {
-          const double2 f = make_double2(Mr[c * LDI + i], Mi[c * LDI + i]);
+        for (int c2 = 0; c2 < KP; ++c2) rv[c2] = rp[c2];
+      } else if (own) {
+        const double2 f = cand;
 #pragma unroll
-          for (int c2 = 0; c2 < KP; ++c2) {
-            double2 v = (c2 == c) ? make_double2(0, 0) : make_double2(Mr[c2 * LDI + i], Mi[c2 * LDI + i]);
-            v = csub(v, cmul(f, rp[c2]));
-            Mr[c2 * LDI + i] = v.x;
-            Mi[c2 * LDI + i] = v.y;
-          }
-        }
+        for (int c2 = 0; c2 < KP; ++c2) rv[c2] = csub((c2 == c) ? make_double2(0, 0) : rv[c2], cmul(f, rp[c2]));
       }
-      __syncthreads();
     }
+    if (!info && own) {
+#pragma unroll
+      for (int c2 = 0; c2 < KP; ++c2) { Mr[c2 * LDI + tid] = rv[c2].x; Mi[c2 * LDI + tid] = rv[c2].y; }
+    }
+    __syncthreads();
     if (info) break;
     // snapshot of the pivot rows (all columns) before they are updated
     for (int idx = tid; idx < KP * n; idx += NTH) {
