@@ -60,6 +60,29 @@ __device__ __forceinline__ double2 ldg(const double2* p, size_t i) {
   return make_double2(v.x, v.y);
 }
 
+// Loads and stores at a wave-uniform base plus a 32-bit byte offset: the base
+// stays in scalar registers and the address is one VGPR (global_* saddr form).
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef __attribute__((address_space(1))) const char gcb;
+typedef __attribute__((address_space(1))) char gwb;
+typedef __attribute__((address_space(1))) dvec2 gv2;
+#else
+typedef const char gcb;
+typedef char gwb;
+typedef dvec2 gv2;
+#endif
+__device__ __forceinline__ double2 ldgb(const double2* base, uint32_t boff) {
+  const dvec2 v = *(gcv2*)((gcb*)base + boff);
+  return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ void stgb(double2* base, uint32_t boff, double2 v) {
+  dvec2 w;
+  w.x = v.x;
+  w.y = v.y;
+  *(gv2*)((gwb*)base + boff) = w;
+}
+__device__ __forceinline__ void stgb_re(double* base, uint32_t boff, double v) { *(gd*)((gwb*)base + boff) = v; }
+
 // Uniform values arrive in VGPRs through the calling convention of a
 // non-inlined function; moving them to SGPRs frees those VGPRs.
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -253,43 +276,63 @@ __device__ __forceinline__ void mma_steps(d4 (&cr)[TA][TB], d4 (&ci)[TA][TB],
 }
 
 // ------------------------------------------------------------------- GEMM
-// C = op(A) op(B) (+ epilogue), n x n column-major (or row-major C when
-// c_rowmajor), n a multiple of MR and MC.  K panels of 16 are double-buffered in LDS;
-// the next panel is fetched into registers while the current one is
-// multiplied, then stored to the other stage before the single barrier of the
-// panel.  Returns sum |C_ij| when ep.abs_sum (same value in every thread).
+// C = op(A) op(B) (+ epilogue), n x n column-major (or row-major C when CRM),
+// n a multiple of MR, MC and 32.  K panels of 16 are double-buffered in LDS; the
+// next panel is fetched into registers while the current one is multiplied,
+// then stored to the other stage before the single barrier of the panel.  The
+// last panel of a tile fetches panel 0 of the next tile, so only the first tile
+// has an exposed prologue.  Global addresses are a wave-uniform origin (scalar
+// registers, advanced per panel and tile) plus a 32-bit byte offset per thread
+// that is fixed for the whole GEMM.  Returns sum |C_ij| when ep.abs_sum (same
+// value in every thread).
 template <int T>
 struct GemmLds {
   static constexpr int DOUBLES = 2 * Cfg<T>::STAGE + 8;  // two stages + reduction scratch
 };
 
+// Byte offset of each of a thread's panel elements from the panel origin.
 template <int T, int OPA, int OPB>
-__device__ __forceinline__ void gemm_load_panel(int n, const double2* __restrict__ A, const double2* __restrict__ B,
-                                                int row0, int col0, int k0,
-                                                double2 (&ra)[Cfg<T>::PERA], double2 (&rb)[Cfg<T>::PERB]) {
+__device__ __forceinline__ void gemm_panel_offsets(int n, uint32_t (&oa)[Cfg<T>::PERA],
+                                                   uint32_t (&ob)[Cfg<T>::PERB]) {
   constexpr int MR = Cfg<T>::MR, MC = Cfg<T>::MC, NTH = Cfg<T>::NTH;
   const int tid = threadIdx.x;
 #pragma unroll
   for (int q = 0; q < Cfg<T>::PERA; ++q) {
     const int idx = q * NTH + tid;
     if (OPA == OP_N) {
-      const int i = idx % MR, k = idx / MR;
-      ra[q] = ldg(A, (size_t)(row0 + i) + (size_t)(k0 + k) * n);
+      const int i = idx % MR, k = idx / MR;  // A(row0 + i, k0 + k)
+      oa[q] = (uint32_t)(i + k * n) * 16u;
     } else {
-      const int k = idx % KP, i = idx / KP;
-      ra[q] = cconj(ldg(A, (size_t)(k0 + k) + (size_t)(row0 + i) * n));
+      const int k = idx % KP, i = idx / KP;  // conj A(k0 + k, row0 + i)
+      oa[q] = (uint32_t)(k + i * n) * 16u;
     }
   }
 #pragma unroll
   for (int q = 0; q < Cfg<T>::PERB; ++q) {
     const int idx = q * NTH + tid;
     if (OPB == OP_N) {
-      const int k = idx % KP, j = idx / KP;
-      rb[q] = ldg(B, (size_t)(k0 + k) + (size_t)(col0 + j) * n);
+      const int k = idx % KP, j = idx / KP;  // B(k0 + k, col0 + j)
+      ob[q] = (uint32_t)(k + j * n) * 16u;
     } else {
-      const int j = idx % MC, k = idx / MC;
-      rb[q] = cconj(ldg(B, (size_t)(col0 + j) + (size_t)(k0 + k) * n));
+      const int j = idx % MC, k = idx / MC;  // conj B(col0 + j, k0 + k)
+      ob[q] = (uint32_t)(j + k * n) * 16u;
     }
+  }
+}
+
+template <int T, int OPA, int OPB>
+__device__ __forceinline__ void gemm_load_panel(const double2* Ao, const double2* Bo,
+                                                const uint32_t (&oa)[Cfg<T>::PERA], const uint32_t (&ob)[Cfg<T>::PERB],
+                                                double2 (&ra)[Cfg<T>::PERA], double2 (&rb)[Cfg<T>::PERB]) {
+#pragma unroll
+  for (int q = 0; q < Cfg<T>::PERA; ++q) {
+    ra[q] = ldgb(Ao, oa[q]);
+    if (OPA == OP_C) ra[q] = cconj(ra[q]);
+  }
+#pragma unroll
+  for (int q = 0; q < Cfg<T>::PERB; ++q) {
+    rb[q] = ldgb(Bo, ob[q]);
+    if (OPB == OP_C) rb[q] = cconj(rb[q]);
   }
 }
 
@@ -323,16 +366,16 @@ __device__ __forceinline__ void gemm_store_panel(ldsd* st, const double2 (&ra)[C
   }
 }
 
-template <int T, int OPA, int OPB>
+template <int T, int OPA, int OPB, int CRM>
 __device__ __noinline__ double wg_gemm_k(int n_, const double2* __restrict__ A_, const double2* __restrict__ B_,
-                                         void* Cout_, int c_rowmajor_, Epi ep_, ldsd* lds) {
+                                         void* Cout_, Epi ep_, ldsd* lds) {
   using C = Cfg<T>;
   constexpr int MR = C::MR, MC = C::MC, LDA = C::LDA, LDB = C::LDB, STAGE = C::STAGE, TA = C::TA, TB = C::TB;
+  constexpr bool AKC = (OPA == OP_C), BKC = (OPB == OP_N);
   const int n = uni(n_);
   const double2* A = uni(A_);
   const double2* B = uni(B_);
   void* Cout = uni(Cout_);
-  const int c_rowmajor = uni(c_rowmajor_);
   Epi ep;
   ep.diag = uni(ep_.diag); ep.sacc = uni(ep_.sacc);
   ep.D1 = uni(ep_.D1); ep.s1 = uni(ep_.s1);
@@ -340,62 +383,104 @@ __device__ __noinline__ double wg_gemm_k(int n_, const double2* __restrict__ A_,
   ep.D3 = uni(ep_.D3); ep.s3 = uni(ep_.s3);
   ep.real_out = uni(ep_.real_out); ep.abs_sum = uni(ep_.abs_sum);
   ldsd* red = lds + 2 * STAGE;
+  const ldsd* s0Ar = lds;
+  const ldsd* s0Ai = s0Ar + KP * LDA;
+  const ldsd* s0Br = s0Ai + KP * LDA;
+  const ldsd* s0Bi = s0Br + KP * LDB;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int arow = (wave % C::WR) * 16 * TA, bcol = (wave / C::WR) * 16 * TB;
-  const int ntr = n / MR, ntc = n / MC, npan = n / KP;
+  const int ntr = n / MR, ntiles = ntr * (n / MC), npan = n / KP;  // npan is even
+  uint32_t oa[C::PERA], ob[C::PERB];
+  gemm_panel_offsets<T, OPA, OPB>(n, oa, ob);
+  // panel origins: A rows row0.. / B columns col0.., k from k0
+  auto a_org = [&](int row0, int k0) -> const double2* {
+    return OPA == OP_N ? A + row0 + (size_t)k0 * n : A + k0 + (size_t)row0 * n;
+  };
+  auto b_org = [&](int col0, int k0) -> const double2* {
+    return OPB == OP_N ? B + k0 + (size_t)col0 * n : B + col0 + (size_t)k0 * n;
+  };
+  // epilogue: the thread's element of (ti, tj, r) is (row0 + it + di, col0 + jt + dj)
+  const int m = lane & 15, kq = lane >> 4;
+  const int it = CRM ? arow + kq : arow + m;
+  const int jt = CRM ? bcol + m : bcol + kq;
+  const uint32_t t_cm = (uint32_t)(it + jt * n), t_tr = (uint32_t)(jt + it * n);
+  const uint32_t t_out = CRM ? t_tr : t_cm;
   double asum = 0.0;
-  for (int tile = 0; tile < ntr * ntc; ++tile) {
+  double2 ra[C::PERA], rb[C::PERB];
+  gemm_load_panel<T, OPA, OPB>(a_org(0, 0), b_org(0, 0), oa, ob, ra, rb);
+  gemm_store_panel<T, OPA, OPB>(lds, ra, rb);
+  __syncthreads();
+  for (int tile = 0; tile < ntiles; ++tile) {
     const int row0 = (tile % ntr) * MR, col0 = (tile / ntr) * MC;
+    const int tn = tile + 1 < ntiles ? tile + 1 : tile;  // the last tile refetches its own panel 0
+    const int nrow0 = (tn % ntr) * MR, ncol0 = (tn / ntr) * MC;
     d4 cr[TA][TB], ci[TA][TB];
 #pragma unroll
     for (int a = 0; a < TA; ++a)
 #pragma unroll
       for (int b = 0; b < TB; ++b) { cr[a][b] = d4{0, 0, 0, 0}; ci[a][b] = d4{0, 0, 0, 0}; }
-    double2 ra[C::PERA], rb[C::PERB];
-    gemm_load_panel<T, OPA, OPB>(n, A, B, row0, col0, 0, ra, rb);
-    gemm_store_panel<T, OPA, OPB>(lds, ra, rb);
-    __syncthreads();
-    for (int p = 0; p < npan; ++p) {
-      // branch-free pipeline: the last pass refetches its own panel into the idle stage
-      const int kn = (p + 1 < npan ? p + 1 : p) * KP;
-      gemm_load_panel<T, OPA, OPB>(n, A, B, row0, col0, kn, ra, rb);
-      const ldsd* st = lds + (p & 1) * STAGE;
-      const ldsd* sAr = st;
-      const ldsd* sAi = sAr + KP * LDA;
-      const ldsd* sBr = sAi + KP * LDA;
-      const ldsd* sBi = sBr + KP * LDB;
-      constexpr bool AKC = (OPA == OP_C), BKC = (OPB == OP_N);
-      if (c_rowmajor)
-        mma_steps<TA, TB, 1, KP / 4, AKC, BKC>(cr, ci, sAr, sAi, LDA, arow, sBr, sBi, LDB, bcol);
-      else
-        mma_steps<TA, TB, 0, KP / 4, AKC, BKC>(cr, ci, sAr, sAi, LDA, arow, sBr, sBi, LDB, bcol);
-      gemm_store_panel<T, OPA, OPB>(lds + ((p + 1) & 1) * STAGE, ra, rb);
+    for (int p = 0; p < npan; p += 2) {
+      // stage 0 holds panel p; panel p + 1 goes to stage 1
+      gemm_load_panel<T, OPA, OPB>(a_org(row0, (p + 1) * KP), b_org(col0, (p + 1) * KP), oa, ob, ra, rb);
+      mma_steps<TA, TB, CRM, KP / 4, AKC, BKC>(cr, ci, s0Ar, s0Ai, LDA, arow, s0Br, s0Bi, LDB, bcol);
+      gemm_store_panel<T, OPA, OPB>(lds + STAGE, ra, rb);
+      __syncthreads();
+      // stage 1 holds panel p + 1; panel p + 2 (or panel 0 of the next tile) goes to stage 0
+      const bool last = p + 2 == npan;
+      gemm_load_panel<T, OPA, OPB>(last ? a_org(nrow0, 0) : a_org(row0, (p + 2) * KP),
+                                   last ? b_org(ncol0, 0) : b_org(col0, (p + 2) * KP), oa, ob, ra, rb);
+      mma_steps<TA, TB, CRM, KP / 4, AKC, BKC>(cr, ci, s0Ar + STAGE, s0Ai + STAGE, LDA, arow, s0Br + STAGE,
+                                              s0Bi + STAGE, LDB, bcol);
+      gemm_store_panel<T, OPA, OPB>(lds, ra, rb);
       __syncthreads();
     }
-    // epilogue
-    const int m = lane & 15, kq = lane >> 4;
+    // epilogue, in batches of one 16x16 tile (4 elements per thread); the D-term
+    // loads of the next batch are issued before the current batch is combined
+    const size_t cm0 = row0 + (size_t)col0 * n, tr0 = col0 + (size_t)row0 * n;
+    auto di = [&](int ti, int r) { return CRM ? ti * 16 + 4 * r : ti * 16; };
+    auto dj = [&](int tj, int r) { return CRM ? tj * 16 : tj * 16 + 4 * r; };
+    auto load_d = [&](int b, double2 (&d1)[4], double2 (&d2)[4], double2 (&d3)[4]) {
+      const int ti = b / TB, tj = b % TB;
+      if (ep.D1) {
 #pragma unroll
-    for (int ti = 0; ti < TA; ++ti)
+        for (int r = 0; r < 4; ++r) d1[r] = ldgb(ep.D1 + cm0 + di(ti, r) + (size_t)dj(tj, r) * n, 16u * t_cm);
+      }
+      if (ep.D2) {
 #pragma unroll
-      for (int tj = 0; tj < TB; ++tj)
+        for (int r = 0; r < 4; ++r) d2[r] = ldgb(ep.D2 + cm0 + di(ti, r) + (size_t)dj(tj, r) * n, 16u * t_cm);
+      }
+      if (ep.D3) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          int i, j;
-          if (c_rowmajor) {
-            i = row0 + arow + ti * 16 + 4 * r + kq;
-            j = col0 + bcol + tj * 16 + m;
-          } else {
-            i = row0 + arow + ti * 16 + m;
-            j = col0 + bcol + tj * 16 + 4 * r + kq;
-          }
-          double2 v = epi_value(ep, n, i, j, cr[ti][tj][r], ci[ti][tj][r]);
-          if (ep.abs_sum) asum += sqrt(v.y * v.y + v.x * v.x);
-          size_t off = c_rowmajor ? (size_t)i * n + j : (size_t)i + (size_t)j * n;
-          if (ep.real_out)
-            ((gd*)Cout)[off] = v.x;
-          else
-            ((gd2*)Cout)[off] = v;
-        }
+        for (int r = 0; r < 4; ++r) d3[r] = ldgb(ep.D3 + tr0 + dj(tj, r) + (size_t)di(ti, r) * n, 16u * t_tr);
+      }
+    };
+    double2 n1[4], n2[4], n3[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) n1[r] = n2[r] = n3[r] = make_double2(0, 0);
+    load_d(0, n1, n2, n3);
+#pragma unroll
+    for (int b = 0; b < TA * TB; ++b) {
+      const int ti = b / TB, tj = b % TB;
+      double2 c1[4], c2[4], c3[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) { c1[r] = n1[r]; c2[r] = n2[r]; c3[r] = n3[r]; }
+      if (b + 1 < TA * TB) load_d(b + 1, n1, n2, n3);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = row0 + it + di(ti, r), j = col0 + jt + dj(tj, r);
+        double2 v = make_double2(ep.sacc * cr[ti][tj][r], ep.sacc * ci[ti][tj][r]);
+        if (i == j) v = cadd(v, ep.diag);
+        if (ep.D1) v = cadd(v, cmul(ep.s1, c1[r]));
+        if (ep.D2) v = cadd(v, cmul(ep.s2, c2[r]));
+        if (ep.D3) v = cadd(v, cmul(ep.s3, cconj(c3[r])));
+        if (ep.abs_sum) asum += sqrt(v.y * v.y + v.x * v.x);
+        const size_t u = CRM ? tr0 + dj(tj, r) + (size_t)di(ti, r) * n : cm0 + di(ti, r) + (size_t)dj(tj, r) * n;
+        if (ep.real_out)
+          stgb_re((double*)Cout + u, 8u * t_out, v.x);
+        else
+          stgb((double2*)Cout + u, 16u * t_out, v);
+      }
+    }
   }
   double s = 0.0;
   if (ep.abs_sum) s = block_sum(asum, red);
@@ -406,12 +491,20 @@ __device__ __noinline__ double wg_gemm_k(int n_, const double2* __restrict__ A_,
 template <int T>
 __device__ __forceinline__ double wg_gemm(int n, int opA, const double2* A, int opB, const double2* B, void* Cout,
                                           int c_rowmajor, Epi ep, ldsd* lds) {
-  if (opA == OP_N) {
-    if (opB == OP_N) return wg_gemm_k<T, OP_N, OP_N>(n, A, B, Cout, c_rowmajor, ep, lds);
-    return wg_gemm_k<T, OP_N, OP_C>(n, A, B, Cout, c_rowmajor, ep, lds);
+  if (c_rowmajor) {
+    if (opA == OP_N) {
+      if (opB == OP_N) return wg_gemm_k<T, OP_N, OP_N, 1>(n, A, B, Cout, ep, lds);
+      return wg_gemm_k<T, OP_N, OP_C, 1>(n, A, B, Cout, ep, lds);
+    }
+    if (opB == OP_N) return wg_gemm_k<T, OP_C, OP_N, 1>(n, A, B, Cout, ep, lds);
+    return wg_gemm_k<T, OP_C, OP_C, 1>(n, A, B, Cout, ep, lds);
   }
-  if (opB == OP_N) return wg_gemm_k<T, OP_C, OP_N>(n, A, B, Cout, c_rowmajor, ep, lds);
-  return wg_gemm_k<T, OP_C, OP_C>(n, A, B, Cout, c_rowmajor, ep, lds);
+  if (opA == OP_N) {
+    if (opB == OP_N) return wg_gemm_k<T, OP_N, OP_N, 0>(n, A, B, Cout, ep, lds);
+    return wg_gemm_k<T, OP_N, OP_C, 0>(n, A, B, Cout, ep, lds);
+  }
+  if (opB == OP_N) return wg_gemm_k<T, OP_C, OP_N, 0>(n, A, B, Cout, ep, lds);
+  return wg_gemm_k<T, OP_C, OP_C, 0>(n, A, B, Cout, ep, lds);
 }
 
 // --------------------------------------------------------------- elementwise
