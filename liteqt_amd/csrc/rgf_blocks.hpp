@@ -229,49 +229,55 @@ __device__ __forceinline__ void mma_steps(d4 (&cr)[TA][TB], d4 (&ci)[TA][TB],
                                           const ldsd* Br, const ldsd* Bi, int ldb, int bcol) {
   const int lane = threadIdx.x & 63;
   const int m = lane & 15, kq = lane >> 4;
-  // not unrolled: unrolling lets the scheduler hoist every k-step's fragment
-  // loads, which overflows the 256 registers a wave has at 2 waves per SIMD
-#pragma unroll 1
-  for (int ks = 0; ks < NKS; ++ks) {
+  // fragments of k-step ks + 1 are read while the MFMAs of k-step ks run
+  double ar[2][TA], ai[2][TA], br[2][TB], bi[2][TB];
+  auto frag = [&](int ks, int s) {
     const int k = ks * 4 + kq;
-    double ar[TA], ai[TA], nai[TA], br[TB], bi[TB];
 #pragma unroll
     for (int t = 0; t < TA; ++t) {
       const int ia = AKC ? (arow + t * 16 + m) * LDK + k : k * lda + arow + t * 16 + m;
-      ar[t] = Ar[ia];
-      ai[t] = Ai[ia];
-      nai[t] = -ai[t];
+      ar[s][t] = Ar[ia];
+      ai[s][t] = Ai[ia];
     }
 #pragma unroll
     for (int t = 0; t < TB; ++t) {
       const int ib = BKC ? (bcol + t * 16 + m) * LDK + k : k * ldb + bcol + t * 16 + m;
-      br[t] = Br[ib];
-      bi[t] = Bi[ib];
+      br[s][t] = Br[ib];
+      bi[s][t] = Bi[ib];
     }
+  };
+  frag(0, 0);
+#pragma unroll
+  for (int ks = 0; ks < NKS; ++ks) {
+    const int s = ks & 1;
+    if (ks + 1 < NKS) frag(ks + 1, s ^ 1);
+    double nai[TA];
+#pragma unroll
+    for (int t = 0; t < TA; ++t) nai[t] = -ai[s][t];
 #pragma unroll
     for (int ti = 0; ti < TA; ++ti)
 #pragma unroll
       for (int tj = 0; tj < TB; ++tj)
-        cr[ti][tj] = ROWMAJ ? __builtin_amdgcn_mfma_f64_16x16x4f64(ar[ti], br[tj], cr[ti][tj], 0, 0, 0)
-                            : __builtin_amdgcn_mfma_f64_16x16x4f64(br[tj], ar[ti], cr[ti][tj], 0, 0, 0);
+        cr[ti][tj] = ROWMAJ ? __builtin_amdgcn_mfma_f64_16x16x4f64(ar[s][ti], br[s][tj], cr[ti][tj], 0, 0, 0)
+                            : __builtin_amdgcn_mfma_f64_16x16x4f64(br[s][tj], ar[s][ti], cr[ti][tj], 0, 0, 0);
 #pragma unroll
     for (int ti = 0; ti < TA; ++ti)
 #pragma unroll
       for (int tj = 0; tj < TB; ++tj)
-        ci[ti][tj] = ROWMAJ ? __builtin_amdgcn_mfma_f64_16x16x4f64(ar[ti], bi[tj], ci[ti][tj], 0, 0, 0)
-                            : __builtin_amdgcn_mfma_f64_16x16x4f64(bi[tj], ar[ti], ci[ti][tj], 0, 0, 0);
+        ci[ti][tj] = ROWMAJ ? __builtin_amdgcn_mfma_f64_16x16x4f64(ar[s][ti], bi[s][tj], ci[ti][tj], 0, 0, 0)
+                            : __builtin_amdgcn_mfma_f64_16x16x4f64(bi[s][tj], ar[s][ti], ci[ti][tj], 0, 0, 0);
 #pragma unroll
     for (int ti = 0; ti < TA; ++ti)
 #pragma unroll
       for (int tj = 0; tj < TB; ++tj)
-        cr[ti][tj] = ROWMAJ ? __builtin_amdgcn_mfma_f64_16x16x4f64(nai[ti], bi[tj], cr[ti][tj], 0, 0, 0)
-                            : __builtin_amdgcn_mfma_f64_16x16x4f64(bi[tj], nai[ti], cr[ti][tj], 0, 0, 0);
+        cr[ti][tj] = ROWMAJ ? __builtin_amdgcn_mfma_f64_16x16x4f64(nai[ti], bi[s][tj], cr[ti][tj], 0, 0, 0)
+                            : __builtin_amdgcn_mfma_f64_16x16x4f64(bi[s][tj], nai[ti], cr[ti][tj], 0, 0, 0);
 #pragma unroll
     for (int ti = 0; ti < TA; ++ti)
 #pragma unroll
       for (int tj = 0; tj < TB; ++tj)
-        ci[ti][tj] = ROWMAJ ? __builtin_amdgcn_mfma_f64_16x16x4f64(ai[ti], br[tj], ci[ti][tj], 0, 0, 0)
-                            : __builtin_amdgcn_mfma_f64_16x16x4f64(br[tj], ai[ti], ci[ti][tj], 0, 0, 0);
+        ci[ti][tj] = ROWMAJ ? __builtin_amdgcn_mfma_f64_16x16x4f64(ai[s][ti], br[s][tj], ci[ti][tj], 0, 0, 0)
+                            : __builtin_amdgcn_mfma_f64_16x16x4f64(br[s][tj], ai[s][ti], ci[ti][tj], 0, 0, 0);
   }
 }
 
