@@ -667,14 +667,25 @@ __device__ __noinline__ int wg_inverse(int n, double2* W_, double2* out_, ldsd* 
     __syncthreads();
     const int ntr = n / MR, ntc = n / MC;
     const int m = lane & 15, kq = lane >> 4;
+    // a thread's element (ti, tj, r) of a tile is W[row0 + arow + kq + ti*16 + 4r][col0 + bcol + m + tj*16]
+    const uint32_t t_off = 16u * (uint32_t)((arow + kq) * n + bcol + m);
     for (int tile = 0; tile < ntr * ntc; ++tile) {
       const int row0 = (tile % ntr) * MR, col0 = (tile / ntr) * MC;
+      // W += M W[P, :]: the MFMAs accumulate onto the tile's old values, which are
+      // all requested before the first MFMA waits for them
       d4 cr[TA][TB], ci[TA][TB];
 #pragma unroll
-      for (int a = 0; a < TA; ++a)
+      for (int ti = 0; ti < TA; ++ti)
 #pragma unroll
-        for (int b = 0; b < TB; ++b) { cr[a][b] = d4{0, 0, 0, 0}; ci[a][b] = d4{0, 0, 0, 0}; }
+        for (int tj = 0; tj < TB; ++tj)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const double2 o = ldgb(W_ + (size_t)(row0 + ti * 16 + 4 * r) * n + col0 + tj * 16, t_off);
+            cr[ti][tj][r] = o.x;
+            ci[ti][tj][r] = o.y;
+          }
       mma_steps<TA, TB, 1, KP / 4>(cr, ci, Mr, Mi, LDI, row0 + arow, Br, Bi, LDI, col0 + bcol);
+      const bool panel = col0 <= k0 && k0 < col0 + MC;  // the panel's columns take F
 #pragma unroll
       for (int ti = 0; ti < TA; ++ti)
 #pragma unroll
@@ -683,16 +694,13 @@ __device__ __noinline__ int wg_inverse(int n, double2* W_, double2* out_, ldsd* 
           for (int r = 0; r < 4; ++r) {
             const int i = row0 + arow + ti * 16 + 4 * r + kq;
             const int j = col0 + bcol + tj * 16 + m;
-            gd2* w = &W[(size_t)i * n + j];
-            if (j >= k0 && j < k0 + KP) {
+            double2 v = make_double2(cr[ti][tj][r], ci[ti][tj][r]);
+            if (panel && j >= k0 && j < k0 + KP) {
               const int c = j - k0;
-              double2 v = make_double2(Mr[c * LDI + i], Mi[c * LDI + i]);
+              v = make_double2(Mr[c * LDI + i], Mi[c * LDI + i]);
               if (piv[j] == i) v.x += 1.0;
-              *w = v;
-            } else {
-              const double2 o = *w;
-              *w = make_double2(o.x + cr[ti][tj][r], o.y + ci[ti][tj][r]);
             }
+            stgb(W_ + (size_t)(row0 + ti * 16 + 4 * r) * n + col0 + tj * 16, t_off, v);
           }
     }
     __syncthreads();
