@@ -154,6 +154,34 @@ __device__ __forceinline__ double2 epi_value(const Epi& ep, int n, int i, int j,
 }
 
 // ---------------------------------------------------------------- reductions
+// DPP lane exchange of a double within a row of 16 lanes (CTRL: a DPP pattern that
+// reads only lanes of the same row).
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xf, 0xf, false);
+  return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+__device__ __forceinline__ double lane_f64(double v, int l) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = __builtin_amdgcn_readlane((unsigned)b, l);
+  const unsigned hi = __builtin_amdgcn_readlane((unsigned)(b >> 32), l);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// Maximum over the wave (uniform result): quad swaps, half-row and row mirrors,
+// one 16-lane swap, then the two 32-lane halves.
+__device__ __forceinline__ double wave_max(double v) {
+  v = fmax(v, dpp_f64<0xB1>(v));   // quad_perm [1,0,3,2]
+  v = fmax(v, dpp_f64<0x4E>(v));   // quad_perm [2,3,0,1]
+  v = fmax(v, dpp_f64<0x141>(v));  // row_half_mirror
+  v = fmax(v, dpp_f64<0x140>(v));  // row_mirror
+  v = fmax(v, __shfl_xor(v, 16, 64));
+  return fmax(lane_f64(v, 0), lane_f64(v, 32));
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
@@ -599,18 +627,15 @@ __device__ __noinline__ int wg_inverse(int n, double2* W_, double2* out_, ldsd* 
       // pivot: first maximum of |Re|+|Im| over unused rows (wave reduction, then one
       // exchange of the 4 wave winners and their rows through LDS; Br/Bi hold the
       // exchange area until the snapshot below, double-buffered by column parity)
-      double best = -1.0;
-      int bi = 0x7fffffff;
       double2 cand = make_double2(0, 0);
 #pragma unroll
       for (int c2 = 0; c2 < KP; ++c2)
         if (c2 == c) cand = rv[c2];
-      if (own && !myused) { best = fabs(cand.x) + fabs(cand.y); bi = tid; }
-      for (int off = 32; off > 0; off >>= 1) {
-        const double ov = __shfl_xor(best, off, 64);
-        const int oi = __shfl_xor(bi, off, 64);
-        if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
-      }
+      const bool valid = own && !myused;
+      const double myv = valid ? fabs(cand.x) + fabs(cand.y) : -1.0;
+      double best = wave_max(myv);
+      const unsigned long long hit = __ballot(valid && myv == best);  // lowest lane = lowest row
+      int bi = hit ? wave * 64 + __builtin_ctzll(hit) : 0x7fffffff;
       ldsd* xrow = Br + (c & 1) * (4 * 2 * KP);  // [wave][c2] re, im
       ldsd* xval = Bi + (c & 1) * 8;               // [wave] value, [4 + wave] row index
       if (tid == bi) {
@@ -713,11 +738,30 @@ __device__ __noinline__ int wg_inverse(int n, double2* W_, double2* out_, ldsd* 
   // inverse permutation into `used`
   for (int q = tid; q < n; q += NTH) used[piv[q]] = q;
   __syncthreads();
-  for (int idx = tid; idx < n * n; idx += NTH) {
-    const int i = idx % n, q = idx / n;
-    out[idx] = W[(size_t)piv[i] * n + used[q]];
+  // out(i, q) = W[piv[i]][used[q]], RB rows of out at a time: the source rows are
+  // read whole into LDS and out is written in column segments of RB elements
+  constexpr int RB = 16;
+  static_assert(2 * RB * (NMAX + 1) <= 4 * KP * LDI, "row block fits the panel area");
+  const int ldr = n + 1;  // odd stride: the RB rows of one column sit in distinct banks
+  ldsd* Lr = Mr;
+  ldsd* Li = Mr + RB * ldr;
+  for (int i0 = 0; i0 < n; i0 += RB) {
+    for (int rr = 0; rr < RB; ++rr) {
+      const gd2* src = W + (size_t)piv[i0 + rr] * n;
+      for (int c = tid; c < n; c += NTH) {
+        const double2 v = src[c];
+        Lr[rr * ldr + c] = v.x;
+        Li[rr * ldr + c] = v.y;
+      }
+    }
+    __syncthreads();
+    for (int idx = tid; idx < RB * n; idx += NTH) {
+      const int rr = idx % RB, q = idx / RB;
+      const int c = used[q];
+      out[(size_t)(i0 + rr) + (size_t)q * n] = make_double2(Lr[rr * ldr + c], Li[rr * ldr + c]);
+    }
+    __syncthreads();
   }
-  __syncthreads();
   return 0;
 }
 
