@@ -45,7 +45,14 @@ struct AdjArgs {
   int* info;
   double2* ws;
   long long ws_slot;
+  int keep;  // 1: the products the reverse passes reuse are kept per slab (larger slot)
 };
+
+// Blocks of a workspace slot: per slab g, g<, S, G, G<, gb, glb (+ R, R d, g S when
+// keep), then kAdjTmp temporaries.
+__host__ __device__ inline long long adj_slot_blocks(int nx, int keep) {
+  return (long long)(keep ? 10 : 7) * nx + kAdjTmp;
+}
 
 // Per energy, with d_l = H1i_l + Sigma^R_{l,l-1}, u_l = H1i_l^H + Sigma^R_{l-1,l},
 // s_l = Sigma^<_{l-1,l} (the plain couplings when the off-diagonal blocks are absent):
@@ -54,8 +61,12 @@ struct AdjArgs {
 //             Y_l = d_l g<_{l-1}, V_l = B_l s_l            (g, g<, S stored; slab nx-1 is G, G<)
 //   backward  Q = g_l u, R = Q G_{l+1}, G_l = g_l + R d g_l,
 //             G<_l = g<_l + Q G<_{l+1} Q^H + W - W^H, W = g_l s R^H + R d g<_l   (G, G< stored)
-//   pass A    reverse of the backward sweep, l = 0..nx-2 (Q, R, R d, Q G<, g s recomputed)
-//   pass B    reverse of the forward sweep, l = nx-1..0: dJ/dU_l = -Re Tr Abar_l
+//   pass A    reverse of the backward sweep, l = 0..nx-2 (Q, R, R d, Q G< kept from the
+//             backward sweep when keep, else recomputed; g s recomputed)
+//   pass B    reverse of the forward sweep, l = nx-1..0: dJ/dU_l = -Re Tr Abar_l (M = g S
+//             kept from the forward sweep when keep)
+// With keep, Q_l and (Q G<)_l sit in the slab-l blocks of glb and gb, which pass A
+// writes only after their last use in its step l.
 // Coupling adjoints d-bar + u-bar^H accumulate per slab in dH.
 template <int T, int NMAX>
 __device__ void adjoint_energy(const AdjArgs& a, int e, double2* ws, double2* dH, ldsd* lds, ldsd* red) {
@@ -71,13 +82,17 @@ __device__ void adjoint_energy(const AdjArgs& a, int e, double2* ws, double2* dH
   double2* Gnv = ws + 4 * (size_t)N * nn;
   double2* gbv = ws + 5 * (size_t)N * nn;
   double2* glbv = ws + 6 * (size_t)N * nn;
-  double2* tb = ws + 7 * (size_t)N * nn;
+  const bool keep = a.keep != 0;
+  double2* Rv = ws + 7 * (size_t)N * nn;   // keep only
+  double2* Rdv = ws + 8 * (size_t)N * nn;  // keep only
+  double2* Mv = ws + 9 * (size_t)N * nn;   // keep only
+  double2* tb = ws + (keep ? 10 : 7) * (size_t)N * nn;
   auto B_ = [&](double2* base, int l) { return base + (size_t)l * nn; };
   auto Tm = [&](int k) { return tb + (size_t)k * nn; };
   double2 *H00 = Tm(0), *W = Tm(1), *T0 = Tm(2), *T1 = Tm(3), *T2 = Tm(4), *SL = Tm(5), *SR = Tm(6);
-  double2 *Dd = Tm(7), *Du = Tm(8), *V = Tm(9), *Q = Tm(10), *R = Tm(11), *Rd = Tm(12), *QGn = Tm(13);
+  double2 *Dd = Tm(7), *Du = Tm(8), *V = Tm(9), *Q = Tm(10), *R = Tm(11), *Rd = Tm(12), *QGn = Tm(13);  // 10-13: Q, R, R d, Q G< when not kept
   double2 *gs = Tm(14), *Wb = Tm(15), *gsb = Tm(16), *Rb = Tm(17), *Rdb = Tm(18), *QGnb = Tm(19), *Qb = Tm(20);
-  double2 *Gb = Tm(21), *Lb = Tm(22), *Gb2 = Tm(23), *Lb2 = Tm(24), *M = Tm(25), *Mb = Tm(26), *Sb = Tm(27);
+  double2 *Gb = Tm(21), *Lb = Tm(22), *Gb2 = Tm(23), *Lb2 = Tm(24), *Mb = Tm(26), *Sb = Tm(27);  // 25: g S when not kept
   double2 *Ab = Tm(28), *Bb = Tm(29), *Yb = Tm(30), *Bm = Tm(31), *Y = Tm(32), *K = Tm(33), *Gs = Tm(34);
   double2* stmp = Tm(36);  // 11 blocks for the decimation
   const size_t so = (size_t)e * a.sig_stride;
@@ -130,8 +145,11 @@ __device__ void adjoint_energy(const AdjArgs& a, int e, double2* ws, double2* dH
   wg_ew(n, W, 1, z, T1, mone, SL, mone, nullptr, one);  // z - (H_0 + sigR_0) - Sigma_L
   if (wg_inverse<T, NMAX>(n, W, g(0), lds)) ++fails;
   wg_ew(n, B_(Sv, 0), 0, zero, SL, make_double2(-fL, 0), sL(0), one, SL, make_double2(fL, 0));
-  wg_gemm<T>(n, OP_N, g(0), OP_N, B_(Sv, 0), T0, 0, st, lds);
-  wg_gemm<T>(n, OP_N, T0, OP_C, g(0), gl(0), 0, st, lds);
+  {
+    double2* Mp = keep ? B_(Mv, 0) : T0;  // g S
+    wg_gemm<T>(n, OP_N, g(0), OP_N, B_(Sv, 0), Mp, 0, st, lds);
+    wg_gemm<T>(n, OP_N, Mp, OP_C, g(0), gl(0), 0, st, lds);
+  }
   for (int l = 1; l < N; ++l) {
     const bool last = (l == N - 1);
     couple(l);
@@ -156,8 +174,9 @@ __device__ void adjoint_energy(const AdjArgs& a, int e, double2* ws, double2* dH
     }
     wg_gemm<T>(n, OP_N, Dd, OP_N, gl(l - 1), T1, 0, st, lds);  // Y = d g<_{l-1}
     wg_gemm<T>(n, OP_N, T1, OP_C, Dd, B_(Sv, l), 0, es, lds);  // S
-    wg_gemm<T>(n, OP_N, g(l), OP_N, B_(Sv, l), T0, 0, st, lds);
-    wg_gemm<T>(n, OP_N, T0, OP_C, g(l), gl(l), 0, st, lds);
+    double2* Mp = keep ? B_(Mv, l) : T0;  // g S
+    wg_gemm<T>(n, OP_N, g(l), OP_N, B_(Sv, l), Mp, 0, st, lds);
+    wg_gemm<T>(n, OP_N, Mp, OP_C, g(l), gl(l), 0, st, lds);
   }
   // tr = -Re Tr(KR Gn_N) - fR Re Tr(KR (G_N - G_N^H)),  KR = -(SR - SR^H)
   wg_ew(n, K, 0, zero, SR, mone, nullptr, one, SR, one);
@@ -175,16 +194,20 @@ __device__ void adjoint_energy(const AdjArgs& a, int e, double2* ws, double2* dH
   for (int l = N - 2; l >= 0; --l) {
     const double2* s = sLup(l + 1);
     couple(l + 1);
-    wg_gemm<T>(n, OP_N, g(l), OP_N, Du, Q, 0, st, lds);             // Q = g u
-    wg_gemm<T>(n, OP_N, Q, OP_N, Gf(l + 1), R, 0, st, lds);         // R = Q G
-    wg_gemm<T>(n, OP_N, R, OP_N, Dd, Rd, 0, st, lds);               // R d
-    wg_gemm<T>(n, OP_N, Rd, OP_N, g(l), Gf(l), 0, acc(g(l)), lds);  // G_l = g + R d g
-    wg_gemm<T>(n, OP_N, Q, OP_N, Gnf(l + 1), QGn, 0, st, lds);      // Q G<
-    wg_gemm<T>(n, OP_N, QGn, OP_C, Q, T0, 0, acc(gl(l)), lds);      // g< + Q G< Q^H
-    wg_gemm<T>(n, OP_N, Rd, OP_N, gl(l), T1, 0, st, lds);           // R d g<
+    double2* Qp = keep ? B_(glbv, l) : Q;
+    double2* Rp = keep ? B_(Rv, l) : R;
+    double2* Rdp = keep ? B_(Rdv, l) : Rd;
+    double2* QGnp = keep ? B_(gbv, l) : QGn;
+    wg_gemm<T>(n, OP_N, g(l), OP_N, Du, Qp, 0, st, lds);             // Q = g u
+    wg_gemm<T>(n, OP_N, Qp, OP_N, Gf(l + 1), Rp, 0, st, lds);        // R = Q G
+    wg_gemm<T>(n, OP_N, Rp, OP_N, Dd, Rdp, 0, st, lds);              // R d
+    wg_gemm<T>(n, OP_N, Rdp, OP_N, g(l), Gf(l), 0, acc(g(l)), lds);  // G_l = g + R d g
+    wg_gemm<T>(n, OP_N, Qp, OP_N, Gnf(l + 1), QGnp, 0, st, lds);     // Q G<
+    wg_gemm<T>(n, OP_N, QGnp, OP_C, Qp, T0, 0, acc(gl(l)), lds);     // g< + Q G< Q^H
+    wg_gemm<T>(n, OP_N, Rdp, OP_N, gl(l), T1, 0, st, lds);           // R d g<
     if (s) {
-      wg_gemm<T>(n, OP_N, g(l), OP_N, s, gs, 0, st, lds);           // g s
-      wg_gemm<T>(n, OP_N, gs, OP_C, R, T2, 0, acc(T1), lds);        // W = g s R^H + R d g<
+      wg_gemm<T>(n, OP_N, g(l), OP_N, s, gs, 0, st, lds);            // g s
+      wg_gemm<T>(n, OP_N, gs, OP_C, Rp, T2, 0, acc(T1), lds);        // W = g s R^H + R d g<
       wg_ew(n, Gnf(l), 0, zero, T0, one, T2, one, T2, mone);
     } else {
       wg_ew(n, Gnf(l), 0, zero, T0, one, T1, one, T1, mone);
@@ -199,10 +222,16 @@ __device__ void adjoint_energy(const AdjArgs& a, int e, double2* ws, double2* dH
     const double2* s = sLup(l + 1);
     double2* Tbar = dH + (size_t)(l + 1) * nn;
     couple(l + 1);
-    wg_gemm<T>(n, OP_N, g(l), OP_N, Du, Q, 0, st, lds);
-    wg_gemm<T>(n, OP_N, Q, OP_N, Gf(l + 1), R, 0, st, lds);
-    wg_gemm<T>(n, OP_N, R, OP_N, Dd, Rd, 0, st, lds);
-    wg_gemm<T>(n, OP_N, Q, OP_N, Gnf(l + 1), QGn, 0, st, lds);
+    const double2* Q = keep ? B_(glbv, l) : Tm(10);
+    const double2* R = keep ? B_(Rv, l) : Tm(11);
+    const double2* Rd = keep ? B_(Rdv, l) : Tm(12);
+    const double2* QGn = keep ? B_(gbv, l) : Tm(13);
+    if (!keep) {
+      wg_gemm<T>(n, OP_N, g(l), OP_N, Du, Tm(10), 0, st, lds);
+      wg_gemm<T>(n, OP_N, Q, OP_N, Gf(l + 1), Tm(11), 0, st, lds);
+      wg_gemm<T>(n, OP_N, R, OP_N, Dd, Tm(12), 0, st, lds);
+      wg_gemm<T>(n, OP_N, Q, OP_N, Gnf(l + 1), Tm(13), 0, st, lds);
+    }
     wg_ew(n, Wb, 0, zero, Lb, one, nullptr, one, Lb, mone);          // Wb = Lb - Lb^H
     if (s) {
       wg_gemm<T>(n, OP_N, g(l), OP_N, s, gs, 0, st, lds);
@@ -210,9 +239,8 @@ __device__ void adjoint_energy(const AdjArgs& a, int e, double2* ws, double2* dH
       wg_gemm<T>(n, OP_C, Wb, OP_N, gs, Rb, 0, st, lds);             // Rb = Wb^H g s
     }
     wg_gemm<T>(n, OP_N, Wb, OP_C, gl(l), Rdb, 0, st, lds);           // Rdb = Wb g<^H
-    wg_gemm<T>(n, OP_C, Rd, OP_N, Wb, B_(glbv, l), 0, acc(Lb), lds); // glb_l = Lb + (R d)^H Wb
     wg_gemm<T>(n, OP_N, Lb, OP_N, Q, QGnb, 0, st, lds);              // QGnb = Lb Q
-    wg_gemm<T>(n, OP_C, Lb, OP_N, QGn, Qb, 0, st, lds);              // Qb = Lb^H Q G<
+    wg_gemm<T>(n, OP_C, Lb, OP_N, QGn, Qb, 0, st, lds);              // Qb = Lb^H Q G< (last use of Q G<)
     wg_gemm<T>(n, OP_C, Rd, OP_N, Gb, B_(gbv, l), 0, acc(Gb), lds);  // gb_l = Gb + (R d)^H Gb
     wg_gemm<T>(n, OP_N, Gb, OP_C, g(l), Rdb, 0, acc(Rdb), lds);      // Rdb += Gb g^H
     if (s)
@@ -231,6 +259,7 @@ __device__ void adjoint_energy(const AdjArgs& a, int e, double2* ws, double2* dH
     wg_gemm<T>(n, OP_C, Q, OP_N, Rb, Gb2, 0, st, lds);               // Gb_{l+1} = Q^H Rb
     wg_gemm<T>(n, OP_N, Qb, OP_C, Du, B_(gbv, l), 0, acc(B_(gbv, l)), lds);  // gb_l += Qb u^H
     wg_gemm<T>(n, OP_C, Qb, OP_N, g(l), Tbar, 0, acc(Tbar), lds);   // u-bar^H = Qb^H g
+    wg_gemm<T>(n, OP_C, Rd, OP_N, Wb, B_(glbv, l), 0, acc(Lb), lds); // glb_l = Lb + (R d)^H Wb (after the last use of Q)
     double2* t = Gb; Gb = Gb2; Gb2 = t;
     t = Lb; Lb = Lb2; Lb2 = t;
   }
@@ -243,7 +272,8 @@ __device__ void adjoint_energy(const AdjArgs& a, int e, double2* ws, double2* dH
     double2* gbl = (l == N - 1) ? Gb : B_(gbv, l);
     const double2* glbl = (l == N - 1) ? Lb : B_(glbv, l);
     const double2* Sl = B_(Sv, l);
-    wg_gemm<T>(n, OP_N, g(l), OP_N, Sl, M, 0, st, lds);              // M = g S
+    const double2* M = keep ? B_(Mv, l) : Tm(25);
+    if (!keep) wg_gemm<T>(n, OP_N, g(l), OP_N, Sl, Tm(25), 0, st, lds);  // M = g S
     wg_gemm<T>(n, OP_N, glbl, OP_N, g(l), Mb, 0, st, lds);           // Mb = glb g
     wg_gemm<T>(n, OP_C, glbl, OP_N, M, gbl, 0, acc(gbl), lds);       // gb += glb^H M
     wg_gemm<T>(n, OP_N, Mb, OP_C, Sl, gbl, 0, acc(gbl), lds);        // gb += Mb S^H

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -981,7 +982,7 @@ int rgf_rs(const double temp[2], int nm, int nx, double E, const double mu[2], c
 }
 
 long long rgf_adjoint_workspace_bytes(int nm, int nx, int slots) {
-  return (long long)(7 * nx + rgf::kAdjTmp) * nm * nm * 16LL * slots + (long long)nx * nm * nm * 16LL * slots;
+  return rgf::adj_slot_blocks(nx, 0) * nm * nm * 16LL * slots + (long long)nx * nm * nm * 16LL * slots;
 }
 
 int rgf_adjoint_batched(const rgf_params* p, const rgf_batch* b, const double* w_tr, const double* w_n,
@@ -1004,24 +1005,37 @@ int rgf_adjoint_batched(const rgf_params* p, const rgf_batch* b, const double* w
   HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, v.fn, v.nth, 0));
   if (per_cu < 1) return fail(RGF_EHIP, "adjoint kernel does not fit on a CU");
   const long long nn = (long long)b->nm * b->nm;
-  const long long slot_bytes = (7LL * b->nx + rgf::kAdjTmp) * nn * 16LL;
   const long long dh_bytes = (long long)b->nx * nn * 16LL;
   long long slots = std::min<long long>(b->nE, (long long)prop.multiProcessorCount * per_cu);
   if (b->max_slots > 0) slots = std::min<long long>(slots, b->max_slots);
+  // The larger slot (R, R d and g S kept per slab: 5 fewer GEMMs per slab) is used
+  // when every slot still fits; otherwise the small slot, recomputing them.
+  const long long keep_bytes = rgf::adj_slot_blocks(b->nx, 1) * nn * 16LL;
+  const long long small_bytes = rgf::adj_slot_blocks(b->nx, 0) * nn * 16LL;
   std::lock_guard<std::mutex> lk(g_ws_mu);
   Buf& ws = g_ws[dev];
   Buf& dh = g_adj_dh[dev];
+  // LITEQT_ADJ_RECOMPUTE=1 forces the small slot (the path taken under memory pressure)
+  const char* rc_env = std::getenv("LITEQT_ADJ_RECOMPUTE");
+  int keep = (rc_env && rc_env[0] == '1') ? 0 : 1;
+  long long slot_bytes = keep ? keep_bytes : small_bytes;
   if ((size_t)(slots * slot_bytes) > ws.bytes || (size_t)(slots * dh_bytes) > dh.bytes) {
     size_t fr = 0, tot = 0;
     HIPCHK(hipMemGetInfo(&fr, &tot));
     const size_t avail = (size_t)((double)(fr + ws.bytes + dh.bytes) * 0.9);
+    if (keep && (long long)(avail / (size_t)(keep_bytes + dh_bytes)) < slots) {
+      keep = 0;
+      slot_bytes = small_bytes;
+    }
     const long long fit = (long long)(avail / (size_t)(slot_bytes + dh_bytes));
     if (fit < 1) return fail(RGF_ENOMEM, "not enough device memory for one adjoint slot");
     slots = std::min(slots, fit);
-    ws.release();
-    dh.release();
-    if (int rc = ws.ensure((size_t)(slots * slot_bytes))) return rc;
-    if (int rc = dh.ensure((size_t)(slots * dh_bytes))) return rc;
+    if ((size_t)(slots * slot_bytes) > ws.bytes || (size_t)(slots * dh_bytes) > dh.bytes) {
+      ws.release();
+      dh.release();
+      if (int rc = ws.ensure((size_t)(slots * slot_bytes))) return rc;
+      if (int rc = dh.ensure((size_t)(slots * dh_bytes))) return rc;
+    }
   }
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   HIPCHK(hipMemsetAsync(dh.p, 0, (size_t)(slots * dh_bytes), s));
@@ -1043,7 +1057,8 @@ int rgf_adjoint_batched(const rgf_params* p, const rgf_batch* b, const double* w
   a.dH = reinterpret_cast<double2*>(dh.p);
   a.info = b->info;
   a.ws = reinterpret_cast<double2*>(ws.p);
-  a.ws_slot = (7LL * b->nx + rgf::kAdjTmp) * nn;
+  a.ws_slot = slot_bytes / 16LL;
+  a.keep = keep;
   v.launch(a, (int)slots, s);
   HIPCHK(hipGetLastError());
   if (dH1i) {

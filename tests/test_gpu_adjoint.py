@@ -194,3 +194,35 @@ def test_offdiag_adjoint_density_gradient_matches_oracle_fd():
         Hm[:, :, l] -= h * np.eye(nm)
         fd = (Jo(Hp) - Jo(Hm)) / (2 * h)
         assert abs(fd - dU[l]) <= 1e-6 * max(1.0, abs(fd)), (l, fd, dU[l])
+
+
+@pytest.mark.parametrize("offdiag", [False, True])
+def test_adjoint_recompute_slot_is_bit_identical(offdiag, monkeypatch):
+    """The small workspace slot (products of the sweeps recomputed in the reverse
+    passes, the path taken under memory pressure) gives the same bits as the
+    default slot that keeps them."""
+    torch, rgf = _env()
+    dev = torch.device("cuda:0")
+    nm, nx = 64, 6
+    E = np.array([-0.25, 0.05, 0.3])
+    if offdiag:
+        Hii, H1i, sig = _od_case(nm, nx, 91, 0.0)
+        names = ("sigL", "sigR", "sigR_lo", "sigR_up", "sigL_up")
+        sd = {k: rgf.to_device_blocks(v, dev) for k, v in zip(names, sig)}
+    else:
+        Hii, H1i, sL, sR = _case(nm, nx, 93, scatter=True)
+        sd = {"sigL": rgf.to_device_blocks(sL, dev), "sigR": rgf.to_device_blocks(sR, dev)}
+    p = rgf.params(rgf.MODE_PHYSICAL, eta_dev=1e-4, eta_lead=1e-4, dec_tol=1e-12)
+    rng = np.random.default_rng(5)
+    w_tr = torch.tensor(rng.standard_normal(len(E)), device=dev)
+    w_n = torch.tensor(rng.standard_normal((len(E), nx)), device=dev)
+    args = (p, torch.tensor(E, dtype=torch.float64, device=dev), (300.0, 250.0), (0.1, -0.1),
+            rgf.to_device_blocks(Hii, dev), rgf.to_device_blocks(H1i, dev))
+    outs = []
+    for recompute in ("0", "1"):
+        monkeypatch.setenv("LITEQT_ADJ_RECOMPUTE", recompute)
+        J, dU, dH = rgf.adjoint_batched(*args, w_tr=w_tr, w_n=w_n, **sd)
+        torch.cuda.synchronize()
+        outs.append((J.clone(), dU.clone(), dH.clone()))
+    for a, b in zip(*outs):
+        assert torch.equal(a, b)
