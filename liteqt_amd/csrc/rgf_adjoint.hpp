@@ -113,10 +113,29 @@ __device__ void adjoint_energy(const AdjArgs& a, int e, double2* ws, double2* dH
   auto acc = [&](const double2* D) { Epi x = epi_store(); x.D1 = D; return x; };
   Epi neg = epi_store();
   neg.sacc = -1.0;
+  // Coupling operands of slab l: d = H1i_l + Sigma^R_{l,l-1} in dP, u = H1i_l^H +
+  // Sigma^R_{l-1,l} in uP.  Without the off-diagonal blocks the GEMMs read H1i_l
+  // itself: dP = H1i_l, and uP = H1i_l holding u^H (uC = 1 flips the GEMM op).
+  const double2* dP = Dd;
+  const double2* uP = Du;
+  int uC = 0;
   auto couple = [&](int l) {
-    wg_ew(n, Dd, 0, zero, Cb(l), one, sRlo(l), one, nullptr, one);
-    wg_ew(n, Du, 0, zero, sRup(l), one, nullptr, one, Cb(l), one);
+    if (a.sigR_lo) {
+      wg_ew(n, Dd, 0, zero, Cb(l), one, sRlo(l), one, nullptr, one);
+      dP = Dd;
+    } else {
+      dP = Cb(l);
+    }
+    if (a.sigR_up) {
+      wg_ew(n, Du, 0, zero, sRup(l), one, nullptr, one, Cb(l), one);
+      uP = Du;
+      uC = 0;
+    } else {
+      uP = Cb(l);
+      uC = 1;
+    }
   };
+  auto uo = [&](int op) { return op ^ uC; };
   int fails = 0, hitL = 0, hitR = 0;
   double Jv = 0.0;
 
@@ -153,14 +172,14 @@ __device__ void adjoint_energy(const AdjArgs& a, int e, double2* ws, double2* dH
   for (int l = 1; l < N; ++l) {
     const bool last = (l == N - 1);
     couple(l);
-    wg_gemm<T>(n, OP_N, Dd, OP_N, g(l - 1), T0, 0, st, lds);  // B = d g_{l-1}
+    wg_gemm<T>(n, OP_N, dP, OP_N, g(l - 1), T0, 0, st, lds);  // B = d g_{l-1}
     Epi ew = epi_store();
     ew.diag = z; ew.sacc = -1.0; ew.D1 = Hb(l); ew.s1 = mone; ew.D2 = sR(l); ew.s2 = mone;
     if (last) {
       wg_ew(n, T1, 0, zero, sR(l), one, SR, one, nullptr, one);  // sigR_N + Sigma_R
       ew.D2 = T1;
     }
-    wg_gemm<T>(n, OP_N, T0, OP_N, Du, W, 1, ew, lds);         // z - H - Sigma^R - B u
+    wg_gemm<T>(n, OP_N, T0, uo(OP_N), uP, W, 1, ew, lds);         // z - H - Sigma^R - B u
     if (wg_inverse<T, NMAX>(n, W, g(l), lds)) ++fails;
     Epi es = epi_store();
     es.D1 = sL(l);
@@ -172,8 +191,8 @@ __device__ void adjoint_energy(const AdjArgs& a, int e, double2* ws, double2* dH
       wg_ew(n, T2, 0, zero, SR, make_double2(-fR, 0), sL(l), one, SR, make_double2(fR, 0));
       es.D1 = T2;
     }
-    wg_gemm<T>(n, OP_N, Dd, OP_N, gl(l - 1), T1, 0, st, lds);  // Y = d g<_{l-1}
-    wg_gemm<T>(n, OP_N, T1, OP_C, Dd, B_(Sv, l), 0, es, lds);  // S
+    wg_gemm<T>(n, OP_N, dP, OP_N, gl(l - 1), T1, 0, st, lds);  // Y = d g<_{l-1}
+    wg_gemm<T>(n, OP_N, T1, OP_C, dP, B_(Sv, l), 0, es, lds);  // S
     double2* Mp = keep ? B_(Mv, l) : T0;  // g S
     wg_gemm<T>(n, OP_N, g(l), OP_N, B_(Sv, l), Mp, 0, st, lds);
     wg_gemm<T>(n, OP_N, Mp, OP_C, g(l), gl(l), 0, st, lds);
@@ -198,9 +217,9 @@ __device__ void adjoint_energy(const AdjArgs& a, int e, double2* ws, double2* dH
     double2* Rp = keep ? B_(Rv, l) : R;
     double2* Rdp = keep ? B_(Rdv, l) : Rd;
     double2* QGnp = keep ? B_(gbv, l) : QGn;
-    wg_gemm<T>(n, OP_N, g(l), OP_N, Du, Qp, 0, st, lds);             // Q = g u
+    wg_gemm<T>(n, OP_N, g(l), uo(OP_N), uP, Qp, 0, st, lds);             // Q = g u
     wg_gemm<T>(n, OP_N, Qp, OP_N, Gf(l + 1), Rp, 0, st, lds);        // R = Q G
-    wg_gemm<T>(n, OP_N, Rp, OP_N, Dd, Rdp, 0, st, lds);              // R d
+    wg_gemm<T>(n, OP_N, Rp, OP_N, dP, Rdp, 0, st, lds);              // R d
     wg_gemm<T>(n, OP_N, Rdp, OP_N, g(l), Gf(l), 0, acc(g(l)), lds);  // G_l = g + R d g
     wg_gemm<T>(n, OP_N, Qp, OP_N, Gnf(l + 1), QGnp, 0, st, lds);     // Q G<
     wg_gemm<T>(n, OP_N, QGnp, OP_C, Qp, T0, 0, acc(gl(l)), lds);     // g< + Q G< Q^H
@@ -227,9 +246,9 @@ __device__ void adjoint_energy(const AdjArgs& a, int e, double2* ws, double2* dH
     const double2* Rd = keep ? B_(Rdv, l) : Tm(12);
     const double2* QGn = keep ? B_(gbv, l) : Tm(13);
     if (!keep) {
-      wg_gemm<T>(n, OP_N, g(l), OP_N, Du, Tm(10), 0, st, lds);
+      wg_gemm<T>(n, OP_N, g(l), uo(OP_N), uP, Tm(10), 0, st, lds);
       wg_gemm<T>(n, OP_N, Q, OP_N, Gf(l + 1), Tm(11), 0, st, lds);
-      wg_gemm<T>(n, OP_N, R, OP_N, Dd, Tm(12), 0, st, lds);
+      wg_gemm<T>(n, OP_N, R, OP_N, dP, Tm(12), 0, st, lds);
       wg_gemm<T>(n, OP_N, Q, OP_N, Gnf(l + 1), Tm(13), 0, st, lds);
     }
     wg_ew(n, Wb, 0, zero, Lb, one, nullptr, one, Lb, mone);          // Wb = Lb - Lb^H
@@ -244,9 +263,9 @@ __device__ void adjoint_energy(const AdjArgs& a, int e, double2* ws, double2* dH
     wg_gemm<T>(n, OP_C, Rd, OP_N, Gb, B_(gbv, l), 0, acc(Gb), lds);  // gb_l = Gb + (R d)^H Gb
     wg_gemm<T>(n, OP_N, Gb, OP_C, g(l), Rdb, 0, acc(Rdb), lds);      // Rdb += Gb g^H
     if (s)
-      wg_gemm<T>(n, OP_N, Rdb, OP_C, Dd, Rb, 0, acc(Rb), lds);       // Rb += Rdb d^H
+      wg_gemm<T>(n, OP_N, Rdb, OP_C, dP, Rb, 0, acc(Rb), lds);       // Rb += Rdb d^H
     else
-      wg_gemm<T>(n, OP_N, Rdb, OP_C, Dd, Rb, 0, st, lds);
+      wg_gemm<T>(n, OP_N, Rdb, OP_C, dP, Rb, 0, st, lds);
     wg_gemm<T>(n, OP_C, R, OP_N, Rdb, Tbar, 0, acc(Tbar), lds);      // d-bar += R^H Rdb
     if (s) wg_gemm<T>(n, OP_N, gsb, OP_C, s, B_(gbv, l), 0, acc(B_(gbv, l)), lds);  // gb_l += gsb s^H
     wg_gemm<T>(n, OP_N, QGnb, OP_C, Gnf(l + 1), Qb, 0, acc(Qb), lds);  // Qb += QGnb G<^H
@@ -257,7 +276,7 @@ __device__ void adjoint_energy(const AdjArgs& a, int e, double2* ws, double2* dH
       wg_gemm<T>(n, OP_C, Q, OP_N, QGnb, Lb2, 0, dg, lds);           // Lb_{l+1} = Q^H QGnb + i w I
     }
     wg_gemm<T>(n, OP_C, Q, OP_N, Rb, Gb2, 0, st, lds);               // Gb_{l+1} = Q^H Rb
-    wg_gemm<T>(n, OP_N, Qb, OP_C, Du, B_(gbv, l), 0, acc(B_(gbv, l)), lds);  // gb_l += Qb u^H
+    wg_gemm<T>(n, OP_N, Qb, uo(OP_C), uP, B_(gbv, l), 0, acc(B_(gbv, l)), lds);  // gb_l += Qb u^H
     wg_gemm<T>(n, OP_C, Qb, OP_N, g(l), Tbar, 0, acc(Tbar), lds);   // u-bar^H = Qb^H g
     wg_gemm<T>(n, OP_C, Rd, OP_N, Wb, B_(glbv, l), 0, acc(Lb), lds); // glb_l = Lb + (R d)^H Wb (after the last use of Q)
     double2* t = Gb; Gb = Gb2; Gb2 = t;
@@ -290,22 +309,22 @@ __device__ void adjoint_energy(const AdjArgs& a, int e, double2* ws, double2* dH
       const double2* s = sLup(l);
       double2* Tbar = dH + (size_t)l * nn;
       couple(l);
-      wg_gemm<T>(n, OP_N, Dd, OP_N, g(l - 1), Bm, 0, st, lds);       // B = d g_{l-1}
-      wg_gemm<T>(n, OP_N, Ab, OP_C, Du, Bb, 0, neg, lds);            // Bb = -Ab u^H
+      wg_gemm<T>(n, OP_N, dP, OP_N, g(l - 1), Bm, 0, st, lds);       // B = d g_{l-1}
+      wg_gemm<T>(n, OP_N, Ab, uo(OP_C), uP, Bb, 0, neg, lds);            // Bb = -Ab u^H
       Epi na = epi_store();
       na.sacc = -1.0; na.D1 = Tbar;
       wg_gemm<T>(n, OP_C, Ab, OP_N, Bm, Tbar, 0, na, lds);           // u-bar^H = -Ab^H B
-      wg_gemm<T>(n, OP_N, Dd, OP_N, gl(l - 1), Y, 0, st, lds);       // Y = d g<_{l-1}
-      wg_gemm<T>(n, OP_N, Sb, OP_N, Dd, Yb, 0, st, lds);             // Yb = Sb d
+      wg_gemm<T>(n, OP_N, dP, OP_N, gl(l - 1), Y, 0, st, lds);       // Y = d g<_{l-1}
+      wg_gemm<T>(n, OP_N, Sb, OP_N, dP, Yb, 0, st, lds);             // Yb = Sb d
       wg_gemm<T>(n, OP_C, Sb, OP_N, Y, Tbar, 0, acc(Tbar), lds);     // d-bar += Sb^H Y
       wg_gemm<T>(n, OP_N, Yb, OP_C, gl(l - 1), Tbar, 0, acc(Tbar), lds);  // d-bar += Yb g<^H
-      wg_gemm<T>(n, OP_C, Dd, OP_N, Yb, B_(glbv, l - 1), 0, acc(B_(glbv, l - 1)), lds);
+      wg_gemm<T>(n, OP_C, dP, OP_N, Yb, B_(glbv, l - 1), 0, acc(B_(glbv, l - 1)), lds);
       if (s) {
         wg_ew(n, V, 0, zero, Sb, one, nullptr, one, Sb, mone);       // Vb = Sb - Sb^H
         wg_gemm<T>(n, OP_N, V, OP_C, s, Bb, 0, acc(Bb), lds);        // Bb += Vb s^H
       }
       wg_gemm<T>(n, OP_N, Bb, OP_C, g(l - 1), Tbar, 0, acc(Tbar), lds);  // d-bar += Bb g^H
-      wg_gemm<T>(n, OP_C, Dd, OP_N, Bb, B_(gbv, l - 1), 0, acc(B_(gbv, l - 1)), lds);
+      wg_gemm<T>(n, OP_C, dP, OP_N, Bb, B_(gbv, l - 1), 0, acc(B_(gbv, l - 1)), lds);
     }
   }
   if (threadIdx.x == 0) {
