@@ -680,12 +680,17 @@ __device__ __noinline__ int wg_inverse(int n, double2* W_, double2* out_, ldsd* 
     }
     __syncthreads();
     if (info) break;
-    // snapshot of the pivot rows (all columns) before they are updated
-    for (int idx = tid; idx < KP * n; idx += NTH) {
-      const int c = idx / n, j = idx % n;
-      const double2 v = W[(size_t)piv[k0 + c] * n + j];
-      Br[c * LDI + j] = v.x;
-      Bi[c * LDI + j] = v.y;
+    // snapshot of the pivot rows before they are updated: thread j copies column j
+    // of the KP rows, all its loads in flight at once
+    if (own) {
+      double2 v[KP];
+#pragma unroll
+      for (int c = 0; c < KP; ++c) v[c] = W[(size_t)piv[k0 + c] * n + tid];
+#pragma unroll
+      for (int c = 0; c < KP; ++c) {
+        Br[c * LDI + tid] = v[c].x;
+        Bi[c * LDI + tid] = v[c].y;
+      }
     }
     __syncthreads();
     if (tid < KP) Mr[tid * LDI + piv[k0 + tid]] -= 1.0;  // M = F - E

@@ -70,3 +70,20 @@ def test_zinv_singular_zero_and_info():
     torch.cuda.synchronize()
     assert info.cpu().numpy()[0] > 0
     assert np.all(Ainv.cpu().numpy() == 0)
+
+
+@pytest.mark.parametrize("n", [96, 128, 192, 256])
+def test_zinv_singular_after_strip_updates(n):
+    """A zero row is never a usable pivot, so the elimination runs through every
+    strip update and reports the last column; the output is zeroed."""
+    torch, rgf = _env()
+    rng = np.random.default_rng(7 + n)
+    A = rng.standard_normal((2, n, n)) + 1j * rng.standard_normal((2, n, n))
+    A[0, n // 2 + 3, :] = 0.0
+    Ainv, info = rgf.zinv_batched(_colmajor(torch, A))
+    torch.cuda.synchronize()
+    info = info.cpu().numpy()
+    assert info[0] == n and info[1] == 0
+    assert np.all(Ainv[0].cpu().numpy() == 0)
+    ref = np.linalg.inv(A[1])
+    assert np.linalg.norm(_back(Ainv)[1] - ref) / np.linalg.norm(ref) < 1e-11
