@@ -751,12 +751,14 @@ __device__ __noinline__ int wg_inverse(int n, double2* W_, double2* out_, ldsd* 
   ldsd* Lr = Mr;
   ldsd* Li = Mr + RB * ldr;
   for (int i0 = 0; i0 < n; i0 += RB) {
-    for (int rr = 0; rr < RB; ++rr) {
-      const gd2* src = W + (size_t)piv[i0 + rr] * n;
-      for (int c = tid; c < n; c += NTH) {
-        const double2 v = src[c];
-        Lr[rr * ldr + c] = v.x;
-        Li[rr * ldr + c] = v.y;
+    if (own) {  // thread c copies column c of the RB source rows, all loads in flight
+      double2 v[RB];
+#pragma unroll
+      for (int rr = 0; rr < RB; ++rr) v[rr] = W[(size_t)piv[i0 + rr] * n + tid];
+#pragma unroll
+      for (int rr = 0; rr < RB; ++rr) {
+        Lr[rr * ldr + tid] = v[rr].x;
+        Li[rr * ldr + tid] = v[rr].y;
       }
     }
     __syncthreads();
