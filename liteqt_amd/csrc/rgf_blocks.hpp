@@ -251,10 +251,19 @@ struct Cfg {
 // ROWMAJ=1: lane holds C[arow + ti*16 + 4r + l/16][bcol + tj*16 + l%16].
 // AKC / BKC: the panel is stored k-contiguous (A at Ar[(arow + m) * LDK + k], B at
 // Br[(bcol + m) * LDK + k]) instead of row-contiguous (A at Ar[k * lda + arow + m]).
-template <int TA, int TB, int ROWMAJ, int NKS, bool AKC = false, bool BKC = false>
+struct NoMid {
+  __device__ void operator()(int) const {}
+};
+
+// mid(g), g = 0..3, runs after the g-th group of MFMAs of the last k-step, pinned
+// there, so work of the caller (the LDS stores of the next panel) issues while
+// the MFMA pipe is still busy.
+template <int TA, int TB, int ROWMAJ, int NKS, bool AKC = false, bool BKC = false, typename Mid = NoMid>
 __device__ __forceinline__ void mma_steps(d4 (&cr)[TA][TB], d4 (&ci)[TA][TB],
                                           const ldsd* Ar, const ldsd* Ai, int lda, int arow,
-                                          const ldsd* Br, const ldsd* Bi, int ldb, int bcol) {
+                                          const ldsd* Br, const ldsd* Bi, int ldb, int bcol,
+                                          const Mid& mid = Mid()) {
+  constexpr bool kMid = !__is_same(Mid, NoMid);
   const int lane = threadIdx.x & 63;
   const int m = lane & 15, kq = lane >> 4;
   // fragments of k-step ks + 1 are read while the MFMAs of k-step ks run
@@ -282,30 +291,35 @@ __device__ __forceinline__ void mma_steps(d4 (&cr)[TA][TB], d4 (&ci)[TA][TB],
     double nai[TA];
 #pragma unroll
     for (int t = 0; t < TA; ++t) nai[t] = -ai[s][t];
+    const bool hook = kMid && ks == NKS - 1;
 #pragma unroll
     for (int ti = 0; ti < TA; ++ti)
 #pragma unroll
       for (int tj = 0; tj < TB; ++tj)
         cr[ti][tj] = ROWMAJ ? __builtin_amdgcn_mfma_f64_16x16x4f64(ar[s][ti], br[s][tj], cr[ti][tj], 0, 0, 0)
                             : __builtin_amdgcn_mfma_f64_16x16x4f64(br[s][tj], ar[s][ti], cr[ti][tj], 0, 0, 0);
+    if (hook) { __builtin_amdgcn_sched_barrier(0); mid(0); __builtin_amdgcn_sched_barrier(0); }
 #pragma unroll
     for (int ti = 0; ti < TA; ++ti)
 #pragma unroll
       for (int tj = 0; tj < TB; ++tj)
         ci[ti][tj] = ROWMAJ ? __builtin_amdgcn_mfma_f64_16x16x4f64(ar[s][ti], bi[s][tj], ci[ti][tj], 0, 0, 0)
                             : __builtin_amdgcn_mfma_f64_16x16x4f64(bi[s][tj], ar[s][ti], ci[ti][tj], 0, 0, 0);
+    if (hook) { __builtin_amdgcn_sched_barrier(0); mid(1); __builtin_amdgcn_sched_barrier(0); }
 #pragma unroll
     for (int ti = 0; ti < TA; ++ti)
 #pragma unroll
       for (int tj = 0; tj < TB; ++tj)
         cr[ti][tj] = ROWMAJ ? __builtin_amdgcn_mfma_f64_16x16x4f64(nai[ti], bi[s][tj], cr[ti][tj], 0, 0, 0)
                             : __builtin_amdgcn_mfma_f64_16x16x4f64(bi[s][tj], nai[ti], cr[ti][tj], 0, 0, 0);
+    if (hook) { __builtin_amdgcn_sched_barrier(0); mid(2); __builtin_amdgcn_sched_barrier(0); }
 #pragma unroll
     for (int ti = 0; ti < TA; ++ti)
 #pragma unroll
       for (int tj = 0; tj < TB; ++tj)
         ci[ti][tj] = ROWMAJ ? __builtin_amdgcn_mfma_f64_16x16x4f64(ai[s][ti], br[s][tj], ci[ti][tj], 0, 0, 0)
                             : __builtin_amdgcn_mfma_f64_16x16x4f64(br[s][tj], ai[s][ti], ci[ti][tj], 0, 0, 0);
+    if (hook) { __builtin_amdgcn_sched_barrier(0); mid(3); __builtin_amdgcn_sched_barrier(0); }
   }
 }
 
@@ -370,9 +384,10 @@ __device__ __forceinline__ void gemm_load_panel(const double2* Ao, const double2
   }
 }
 
+// part = -1: the whole panel; 0..3: the elements q with q % 4 == part
 template <int T, int OPA, int OPB>
 __device__ __forceinline__ void gemm_store_panel(ldsd* st, const double2 (&ra)[Cfg<T>::PERA],
-                                                 const double2 (&rb)[Cfg<T>::PERB]) {
+                                                 const double2 (&rb)[Cfg<T>::PERB], int part = -1) {
   constexpr int MR = Cfg<T>::MR, MC = Cfg<T>::MC, LDA = Cfg<T>::LDA, LDB = Cfg<T>::LDB, NTH = Cfg<T>::NTH;
   ldsd* Ar = st;
   ldsd* Ai = Ar + KP * LDA;
@@ -381,6 +396,7 @@ __device__ __forceinline__ void gemm_store_panel(ldsd* st, const double2 (&ra)[C
   const int tid = threadIdx.x;
 #pragma unroll
   for (int q = 0; q < Cfg<T>::PERA; ++q) {
+    if (part >= 0 && q % 4 != part) continue;
     const int idx = q * NTH + tid;
     int i, k;
     if (OPA == OP_N) { i = idx % MR; k = idx / MR; } else { k = idx % KP; i = idx / KP; }
@@ -391,6 +407,7 @@ __device__ __forceinline__ void gemm_store_panel(ldsd* st, const double2 (&ra)[C
   }
 #pragma unroll
   for (int q = 0; q < Cfg<T>::PERB; ++q) {
+    if (part >= 0 && q % 4 != part) continue;
     const int idx = q * NTH + tid;
     int j, k;
     if (OPB == OP_N) { k = idx % KP; j = idx / KP; } else { j = idx % MC; k = idx / MC; }
@@ -456,16 +473,16 @@ __device__ __noinline__ double wg_gemm_k(int n_, const double2* __restrict__ A_,
     for (int p = 0; p < npan; p += 2) {
       // stage 0 holds panel p; panel p + 1 goes to stage 1
       gemm_load_panel<T, OPA, OPB>(a_org(row0, (p + 1) * KP), b_org(col0, (p + 1) * KP), oa, ob, ra, rb);
-      mma_steps<TA, TB, CRM, KP / 4, AKC, BKC>(cr, ci, s0Ar, s0Ai, LDA, arow, s0Br, s0Bi, LDB, bcol);
-      gemm_store_panel<T, OPA, OPB>(lds + STAGE, ra, rb);
+      mma_steps<TA, TB, CRM, KP / 4, AKC, BKC>(cr, ci, s0Ar, s0Ai, LDA, arow, s0Br, s0Bi, LDB, bcol,
+                                              [&](int g) { gemm_store_panel<T, OPA, OPB>(lds + STAGE, ra, rb, g); });
       __syncthreads();
       // stage 1 holds panel p + 1; panel p + 2 (or panel 0 of the next tile) goes to stage 0
       const bool last = p + 2 == npan;
       gemm_load_panel<T, OPA, OPB>(last ? a_org(nrow0, 0) : a_org(row0, (p + 2) * KP),
                                    last ? b_org(ncol0, 0) : b_org(col0, (p + 2) * KP), oa, ob, ra, rb);
       mma_steps<TA, TB, CRM, KP / 4, AKC, BKC>(cr, ci, s0Ar + STAGE, s0Ai + STAGE, LDA, arow, s0Br + STAGE,
-                                              s0Bi + STAGE, LDB, bcol);
-      gemm_store_panel<T, OPA, OPB>(lds, ra, rb);
+                                              s0Bi + STAGE, LDB, bcol,
+                                              [&](int g) { gemm_store_panel<T, OPA, OPB>(lds, ra, rb, g); });
       __syncthreads();
     }
     // epilogue, in batches of one 16x16 tile (4 elements per thread); the D-term
