@@ -640,55 +640,98 @@ __device__ __noinline__ int wg_inverse(int n, double2* W_, double2* out_, ldsd* 
     double2 rv[KP];
 #pragma unroll
     for (int c2 = 0; c2 < KP; ++c2) rv[c2] = own ? W[(size_t)tid * n + k0 + c2] : make_double2(0, 0);
-    for (int c = 0; c < KP; ++c) {
-      // pivot: first maximum of |Re|+|Im| over unused rows (wave reduction, then one
-      // exchange of the 4 wave winners and their rows through LDS; Br/Bi hold the
-      // exchange area until the snapshot below, double-buffered by column parity)
-      double2 cand = make_double2(0, 0);
+    if (n <= 64) {
+      // all rows are in wave 0: the column steps run in that wave alone, the pivot
+      // row broadcast by readlane, no LDS exchange and no barrier per column
+      if (wave == 0) {
+        for (int c = 0; c < KP; ++c) {
+          double2 cand = make_double2(0, 0);
 #pragma unroll
-      for (int c2 = 0; c2 < KP; ++c2)
-        if (c2 == c) cand = rv[c2];
-      const bool valid = own && !myused;
-      const double myv = valid ? fabs(cand.x) + fabs(cand.y) : -1.0;
-      double best = wave_max(myv);
-      const unsigned long long hit = __ballot(valid && myv == best);  // lowest lane = lowest row
-      int bi = hit ? wave * 64 + __builtin_ctzll(hit) : 0x7fffffff;
-      ldsd* xrow = Br + (c & 1) * (4 * 2 * KP);  // [wave][c2] re, im
-      ldsd* xval = Bi + (c & 1) * 8;               // [wave] value, [4 + wave] row index
-      if (tid == bi) {
+          for (int c2 = 0; c2 < KP; ++c2)
+            if (c2 == c) cand = rv[c2];
+          const bool valid = own && !myused;
+          const double myv = valid ? fabs(cand.x) + fabs(cand.y) : -1.0;
+          const double best = wave_max(myv);
+          if (!(best > 0.0)) { info = k0 + c + 1; break; }
+          const int p = uni(__builtin_ctzll(__ballot(valid && myv == best)));  // lowest row
+          double2 pr[KP];
 #pragma unroll
-        for (int c2 = 0; c2 < KP; ++c2) {
-          xrow[wave * 2 * KP + 2 * c2] = rv[c2].x;
-          xrow[wave * 2 * KP + 2 * c2 + 1] = rv[c2].y;
+          for (int c2 = 0; c2 < KP; ++c2) pr[c2] = make_double2(lane_f64(rv[c2].x, p), lane_f64(rv[c2].y, p));
+          double2 pc = pr[0];
+#pragma unroll
+          for (int c2 = 1; c2 < KP; ++c2)
+            if (c2 == c) pc = pr[c2];
+          const double2 ainv = crecip(pc);
+          double2 rp[KP];
+#pragma unroll
+          for (int c2 = 0; c2 < KP; ++c2) rp[c2] = (c2 == c) ? ainv : cmul(pr[c2], ainv);
+          if (tid == p) {
+            myused = true;
+            used[p] = 1;
+            piv[k0 + c] = p;
+#pragma unroll
+            for (int c2 = 0; c2 < KP; ++c2) rv[c2] = rp[c2];
+          } else if (own) {
+            const double2 f = cand;
+#pragma unroll
+            for (int c2 = 0; c2 < KP; ++c2) rv[c2] = csub((c2 == c) ? make_double2(0, 0) : rv[c2], cmul(f, rp[c2]));
+          }
         }
+        if (tid == 0) redi[0] = info;
       }
-      if (lane == 0) { xval[wave] = best; xval[4 + wave] = (double)bi; }
       __syncthreads();
-      best = xval[0];
-      bi = (int)xval[4];
-      for (int w = 1; w < NTH / 64; ++w) {
-        const double ov = xval[w];
-        const int oi = (int)xval[4 + w];
-        if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
-      }
-      if (!(best > 0.0)) { info = k0 + c + 1; break; }
-      const int p = bi;
-      const ldsd* prow = xrow + (p >> 6) * 2 * KP;
-      const double2 ainv = crecip(make_double2(prow[2 * c], prow[2 * c + 1]));
-      double2 rp[KP];
-#pragma unroll
-      for (int c2 = 0; c2 < KP; ++c2)
-        rp[c2] = (c2 == c) ? ainv : cmul(make_double2(prow[2 * c2], prow[2 * c2 + 1]), ainv);
-      if (tid == p) {
-        myused = true;
-        used[p] = 1;
-        piv[k0 + c] = p;
-#pragma unroll
-        for (int c2 = 0; c2 < KP; ++c2) rv[c2] = rp[c2];
-      } else if (own) {
-        const double2 f = cand;
-#pragma unroll
-        for (int c2 = 0; c2 < KP; ++c2) rv[c2] = csub((c2 == c) ? make_double2(0, 0) : rv[c2], cmul(f, rp[c2]));
+      info = redi[0];
+    } else {
+      for (int c = 0; c < KP; ++c) {
+        // pivot: first maximum of |Re|+|Im| over unused rows (wave reduction, then one
+        // exchange of the 4 wave winners and their rows through LDS; Br/Bi hold the
+        // exchange area until the snapshot below, double-buffered by column parity)
+        double2 cand = make_double2(0, 0);
+  #pragma unroll
+        for (int c2 = 0; c2 < KP; ++c2)
+          if (c2 == c) cand = rv[c2];
+        const bool valid = own && !myused;
+        const double myv = valid ? fabs(cand.x) + fabs(cand.y) : -1.0;
+        double best = wave_max(myv);
+        const unsigned long long hit = __ballot(valid && myv == best);  // lowest lane = lowest row
+        int bi = hit ? wave * 64 + __builtin_ctzll(hit) : 0x7fffffff;
+        ldsd* xrow = Br + (c & 1) * (4 * 2 * KP);  // [wave][c2] re, im
+        ldsd* xval = Bi + (c & 1) * 8;               // [wave] value, [4 + wave] row index
+        if (tid == bi) {
+  #pragma unroll
+          for (int c2 = 0; c2 < KP; ++c2) {
+            xrow[wave * 2 * KP + 2 * c2] = rv[c2].x;
+            xrow[wave * 2 * KP + 2 * c2 + 1] = rv[c2].y;
+          }
+        }
+        if (lane == 0) { xval[wave] = best; xval[4 + wave] = (double)bi; }
+        __syncthreads();
+        best = xval[0];
+        bi = (int)xval[4];
+        for (int w = 1; w < NTH / 64; ++w) {
+          const double ov = xval[w];
+          const int oi = (int)xval[4 + w];
+          if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+        }
+        if (!(best > 0.0)) { info = k0 + c + 1; break; }
+        const int p = bi;
+        const ldsd* prow = xrow + (p >> 6) * 2 * KP;
+        const double2 ainv = crecip(make_double2(prow[2 * c], prow[2 * c + 1]));
+        double2 rp[KP];
+  #pragma unroll
+        for (int c2 = 0; c2 < KP; ++c2)
+          rp[c2] = (c2 == c) ? ainv : cmul(make_double2(prow[2 * c2], prow[2 * c2 + 1]), ainv);
+        if (tid == p) {
+          myused = true;
+          used[p] = 1;
+          piv[k0 + c] = p;
+  #pragma unroll
+          for (int c2 = 0; c2 < KP; ++c2) rv[c2] = rp[c2];
+        } else if (own) {
+          const double2 f = cand;
+  #pragma unroll
+          for (int c2 = 0; c2 < KP; ++c2) rv[c2] = csub((c2 == c) ? make_double2(0, 0) : rv[c2], cmul(f, rp[c2]));
+        }
       }
     }
     if (!info && own) {
