@@ -255,7 +255,12 @@ int rgf_adjoint_batched(const rgf_params *p, const rgf_batch *b, const double *w
                         const double *w_n, const double *lead_H00, double *J, double *dU,
                         double *dH1i, void *stream);
 
-/* Device workspace of rgf_adjoint_batched for `slots` energies in flight. */
+/* Device workspace of rgf_adjoint_batched for `slots` energies in flight with the
+ * small slot (7*nx + 48 blocks per energy, plus nx coupling-adjoint blocks).  When
+ * every slot fits in 90% of free device memory the engine takes the larger slot
+ * (10*nx + 48 blocks), which keeps three more products per slab and saves 5 GEMMs per
+ * slab; results are bit-identical.  The environment variable LITEQT_ADJ_RECOMPUTE=1
+ * forces the small slot. */
 long long rgf_adjoint_workspace_bytes(int nm, int nx, int slots);
 
 /* Batched block operations of the engine, one workgroup per matrix, device
