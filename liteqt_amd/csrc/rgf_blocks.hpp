@@ -417,6 +417,11 @@ __device__ __forceinline__ void gemm_store_panel(ldsd* st, const double2 (&ra)[C
   }
 }
 
+template <int M>
+struct EpiMode {
+  static constexpr int value = M;
+};
+
 template <int T, int OPA, int OPB, int CRM>
 __device__ __noinline__ double wg_gemm_k(int n_, const double2* __restrict__ A_, const double2* __restrict__ B_,
                                          void* Cout_, Epi ep_, ldsd* lds) {
@@ -433,6 +438,7 @@ __device__ __noinline__ double wg_gemm_k(int n_, const double2* __restrict__ A_,
   ep.D2 = uni(ep_.D2); ep.s2 = uni(ep_.s2);
   ep.D3 = uni(ep_.D3); ep.s3 = uni(ep_.s3);
   ep.real_out = uni(ep_.real_out); ep.abs_sum = uni(ep_.abs_sum);
+  const int emode = (ep.D2 || ep.D3 || ep.real_out || ep.abs_sum) ? 2 : (ep.D1 ? 1 : 0);
   ldsd* red = lds + 2 * STAGE;
   const ldsd* s0Ar = lds;
   const ldsd* s0Ai = s0Ar + KP * LDA;
@@ -486,52 +492,68 @@ __device__ __noinline__ double wg_gemm_k(int n_, const double2* __restrict__ A_,
       __syncthreads();
     }
     // epilogue, in batches of one 16x16 tile (4 elements per thread); the D-term
-    // loads of the next batch are issued before the current batch is combined
+    // loads of the next batch are issued before the current batch is combined.
+    // The variant is chosen once per tile (no per-element tests of the epilogue
+    // fields): 0 = scale (+ diagonal) and store, 1 = plus s1*D1, 2 = general.
     const size_t cm0 = row0 + (size_t)col0 * n, tr0 = col0 + (size_t)row0 * n;
     auto di = [&](int ti, int r) { return CRM ? ti * 16 + 4 * r : ti * 16; };
     auto dj = [&](int tj, int r) { return CRM ? tj * 16 : tj * 16 + 4 * r; };
-    auto load_d = [&](int b, double2 (&d1)[4], double2 (&d2)[4], double2 (&d3)[4]) {
-      const int ti = b / TB, tj = b % TB;
-      if (ep.D1) {
+    auto epilogue = [&](auto mode) {
+      constexpr int MODE = decltype(mode)::value;
+      auto load_d = [&](int b, double2 (&d1)[4], double2 (&d2)[4], double2 (&d3)[4]) {
+        const int ti = b / TB, tj = b % TB;
+        if (MODE == 1 || (MODE == 2 && ep.D1)) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) d1[r] = ldgb(ep.D1 + cm0 + di(ti, r) + (size_t)dj(tj, r) * n, 16u * t_cm);
-      }
-      if (ep.D2) {
+          for (int r = 0; r < 4; ++r) d1[r] = ldgb(ep.D1 + cm0 + di(ti, r) + (size_t)dj(tj, r) * n, 16u * t_cm);
+        }
+        if (MODE == 2 && ep.D2) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) d2[r] = ldgb(ep.D2 + cm0 + di(ti, r) + (size_t)dj(tj, r) * n, 16u * t_cm);
-      }
-      if (ep.D3) {
+          for (int r = 0; r < 4; ++r) d2[r] = ldgb(ep.D2 + cm0 + di(ti, r) + (size_t)dj(tj, r) * n, 16u * t_cm);
+        }
+        if (MODE == 2 && ep.D3) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) d3[r] = ldgb(ep.D3 + tr0 + dj(tj, r) + (size_t)di(ti, r) * n, 16u * t_tr);
+          for (int r = 0; r < 4; ++r) d3[r] = ldgb(ep.D3 + tr0 + dj(tj, r) + (size_t)di(ti, r) * n, 16u * t_tr);
+        }
+      };
+      double2 n1[4], n2[4], n3[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) n1[r] = n2[r] = n3[r] = make_double2(0, 0);
+      if (MODE > 0) load_d(0, n1, n2, n3);
+#pragma unroll
+      for (int b = 0; b < TA * TB; ++b) {
+        const int ti = b / TB, tj = b % TB;
+        double2 c1[4], c2[4], c3[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) { c1[r] = n1[r]; c2[r] = n2[r]; c3[r] = n3[r]; }
+        if (MODE > 0 && b + 1 < TA * TB) load_d(b + 1, n1, n2, n3);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = row0 + it + di(ti, r), j = col0 + jt + dj(tj, r);
+          double2 v = make_double2(ep.sacc * cr[ti][tj][r], ep.sacc * ci[ti][tj][r]);
+          const double2 vd = cadd(v, ep.diag);
+          v.x = (i == j) ? vd.x : v.x;
+          v.y = (i == j) ? vd.y : v.y;
+          if (MODE == 1) v = cadd(v, cmul(ep.s1, c1[r]));
+          if (MODE == 2) {
+            if (ep.D1) v = cadd(v, cmul(ep.s1, c1[r]));
+            if (ep.D2) v = cadd(v, cmul(ep.s2, c2[r]));
+            if (ep.D3) v = cadd(v, cmul(ep.s3, cconj(c3[r])));
+            if (ep.abs_sum) asum += sqrt(v.y * v.y + v.x * v.x);
+          }
+          const size_t u = CRM ? tr0 + dj(tj, r) + (size_t)di(ti, r) * n : cm0 + di(ti, r) + (size_t)dj(tj, r) * n;
+          if (MODE == 2 && ep.real_out)
+            stgb_re((double*)Cout + u, 8u * t_out, v.x);
+          else
+            stgb((double2*)Cout + u, 16u * t_out, v);
+        }
       }
     };
-    double2 n1[4], n2[4], n3[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) n1[r] = n2[r] = n3[r] = make_double2(0, 0);
-    load_d(0, n1, n2, n3);
-#pragma unroll
-    for (int b = 0; b < TA * TB; ++b) {
-      const int ti = b / TB, tj = b % TB;
-      double2 c1[4], c2[4], c3[4];
-#pragma unroll
-      for (int r = 0; r < 4; ++r) { c1[r] = n1[r]; c2[r] = n2[r]; c3[r] = n3[r]; }
-      if (b + 1 < TA * TB) load_d(b + 1, n1, n2, n3);
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int i = row0 + it + di(ti, r), j = col0 + jt + dj(tj, r);
-        double2 v = make_double2(ep.sacc * cr[ti][tj][r], ep.sacc * ci[ti][tj][r]);
-        if (i == j) v = cadd(v, ep.diag);
-        if (ep.D1) v = cadd(v, cmul(ep.s1, c1[r]));
-        if (ep.D2) v = cadd(v, cmul(ep.s2, c2[r]));
-        if (ep.D3) v = cadd(v, cmul(ep.s3, cconj(c3[r])));
-        if (ep.abs_sum) asum += sqrt(v.y * v.y + v.x * v.x);
-        const size_t u = CRM ? tr0 + dj(tj, r) + (size_t)di(ti, r) * n : cm0 + di(ti, r) + (size_t)dj(tj, r) * n;
-        if (ep.real_out)
-          stgb_re((double*)Cout + u, 8u * t_out, v.x);
-        else
-          stgb((double2*)Cout + u, 16u * t_out, v);
-      }
-    }
+    if (emode == 0)
+      epilogue(EpiMode<0>());
+    else if (emode == 1)
+      epilogue(EpiMode<1>());
+    else
+      epilogue(EpiMode<2>());
   }
   double s = 0.0;
   if (ep.abs_sum) s = block_sum(asum, red);
