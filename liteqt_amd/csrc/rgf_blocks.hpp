@@ -798,22 +798,32 @@ __device__ __noinline__ int wg_inverse(int n, double2* W_, double2* out_, ldsd* 
           }
       mma_steps<TA, TB, 1, KP / 4>(cr, ci, Mr, Mi, LDI, row0 + arow, Br, Bi, LDI, col0 + bcol);
       const bool panel = col0 <= k0 && k0 < col0 + MC;  // the panel's columns take F
+      if (!panel) {
 #pragma unroll
-      for (int ti = 0; ti < TA; ++ti)
+        for (int ti = 0; ti < TA; ++ti)
 #pragma unroll
-        for (int tj = 0; tj < TB; ++tj)
+          for (int tj = 0; tj < TB; ++tj)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int i = row0 + arow + ti * 16 + 4 * r + kq;
-            const int j = col0 + bcol + tj * 16 + m;
-            double2 v = make_double2(cr[ti][tj][r], ci[ti][tj][r]);
-            if (panel && j >= k0 && j < k0 + KP) {
-              const int c = j - k0;
-              v = make_double2(Mr[c * LDI + i], Mi[c * LDI + i]);
-              if (piv[j] == i) v.x += 1.0;
+            for (int r = 0; r < 4; ++r)
+              stgb(W_ + (size_t)(row0 + ti * 16 + 4 * r) * n + col0 + tj * 16, t_off,
+                   make_double2(cr[ti][tj][r], ci[ti][tj][r]));
+      } else {
+#pragma unroll
+        for (int ti = 0; ti < TA; ++ti)
+#pragma unroll
+          for (int tj = 0; tj < TB; ++tj)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int i = row0 + arow + ti * 16 + 4 * r + kq;
+              const int j = col0 + bcol + tj * 16 + m;
+              const bool inp = j >= k0 && j < k0 + KP;
+              const int c = inp ? j - k0 : 0;
+              double2 f = make_double2(Mr[c * LDI + i], Mi[c * LDI + i]);
+              f.x += (piv[k0 + c] == i) ? 1.0 : 0.0;
+              const double2 v = inp ? f : make_double2(cr[ti][tj][r], ci[ti][tj][r]);
+              stgb(W_ + (size_t)(row0 + ti * 16 + 4 * r) * n + col0 + tj * 16, t_off, v);
             }
-            stgb(W_ + (size_t)(row0 + ti * 16 + 4 * r) * n + col0 + tj * 16, t_off, v);
-          }
+      }
     }
     __syncthreads();
   }
