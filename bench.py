@@ -46,14 +46,18 @@ def build_inputs(cfg):
 
 
 def kernel_source_hash():
-    """Hash of the HIP engine sources: a PMC traffic figure is reported only for the
-    kernel it was measured on."""
+    """Hash of the HIP engine sources without comments and whitespace: a PMC traffic
+    figure is reported only for the kernel code it was measured on."""
     import hashlib
+    import re
     h = hashlib.sha256()
     csrc = os.path.join(ROOT, "liteqt_amd", "csrc")
     for f in sorted(os.listdir(csrc)):
         if f.endswith((".hpp", ".hip")) and f != "rgf_plain.hip":
-            h.update(open(os.path.join(csrc, f), "rb").read())
+            t = open(os.path.join(csrc, f)).read()
+            t = re.sub(r"/\*.*?\*/", " ", t, flags=re.S)
+            t = re.sub(r"//[^\n]*", " ", t)
+            h.update(" ".join(t.split()).encode())
     return h.hexdigest()[:16]
 
 

@@ -625,10 +625,12 @@ __device__ __forceinline__ double wg_imtrace(int n, const double2* X, ldsd* red)
 // the inverse is written column-major to out.  Returns 0, or k+1 when column k
 // has no nonzero pivot, in which case out is zeroed (src/green_rgf.f90:4429-4437).
 //
-// Panel step (16 columns K, pivot rows P): the panel is swept in LDS, giving
-// F_P = A_PK^-1 and F_R = -A_RK A_PK^-1; the other columns J are updated by one
-// MFMA GEMM W[:,J] += (F - E) W[P,J] with E the pivot incidence, and columns K
-// take F.  After all panels, inv(i, q) = W[piv[i]][pinv[q]].
+// Panel step (16 columns K, pivot rows P): the panel is swept with one row per
+// thread in registers, giving F_P = A_PK^-1 and F_R = -A_RK A_PK^-1; the other
+// columns J are updated by one MFMA GEMM W[:,J] += (F - E) W[P,J] with E the pivot
+// incidence (accumulating onto the loaded tiles of W), and columns K take F.
+// out (a separate n x n buffer) receives inv(i, q) = W[piv[i]][pinv[q]] after the
+// last panel.
 template <int T, int NMAX>
 struct InvLds {
   static constexpr int LDI = NMAX + PAD;
