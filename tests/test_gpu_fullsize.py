@@ -1,0 +1,90 @@
+"""GPU: the benchmark's random device at its full size (nm = 256, nx = 200 slabs),
+checked through properties that hold at any size, on a few energies (the CPU oracle
+takes minutes per energy at this size; it is compared at smaller sizes in
+test_gpu_parity.py).  Physical mode:
+  * equilibrium (mu_L = mu_R, T_L = T_R) with eta_dev = 0 (the leads are the only
+    open channels; an absorbing eta in the device enters G - G^H but not G^<):
+    G^<_ll = -f(E) (G_ll - G_ll^H) on every slab, the fluctuation-dissipation identity
+    that the recursion keeps block by block, to rounding relative to f |G_ll| (the
+    disordered 200-slab device localises its states, so deep slabs have an
+    exponentially small G - G^H); electron densities Im Tr G^<_ll >= 0;
+  * f_L = 1, f_R = 0 with eta_dev = 0 (no absorption in the device): the current
+    spectra at the two contacts agree (tr = tre = T(E)) and 0 <= T(E) <= nm.
+Tolerances are relative and written below."""
+import numpy as np
+import pytest
+
+from liteqt_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+NM, NX = 256, 200
+E = np.array([-0.6, -0.1, 0.3, 0.8])
+BOLTZ = 8.61734e-05
+# per slab, |G^< + f (G - G^H)| relative to f |G|: rounding accumulated over 200 slabs
+# at eta_lead = 1e-5; the CPU oracle's own residual on this device at these energies
+# reaches 2.3e-10 (measured once, 134 s on 4 threads)
+TOL_IDENTITY = 1e-9
+TOL_CONSERVATION = 1e-9  # |tr - tre| / max(1, T)
+
+
+def _env():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.fail("no HIP device visible")
+    from liteqt_amd import rgf
+    return torch, rgf
+
+
+def _device(torch, rgf, dev):
+    Hii, H1i = configs.random_device(NM, NX, 2)  # the benchmark's device
+    return rgf.to_device_blocks(Hii, dev), rgf.to_device_blocks(H1i, dev)
+
+
+def test_full_size_equilibrium_identity_on_every_slab():
+    torch, rgf = _env()
+    dev = torch.device("cuda:0")
+    Hd, Cd = _device(torch, rgf, dev)
+    nE = len(E)
+    T, mu = 300.0, 0.05
+    p = rgf.params(rgf.MODE_PHYSICAL, eta_dev=0.0, eta_lead=1e-5, dec_tol=1e-12)
+    out = {"ldos": torch.zeros(nE, NX, NM, NM, dtype=torch.complex128, device=dev),
+           "ndens": torch.zeros(nE, NX, NM, NM, dtype=torch.complex128, device=dev),
+           "dens_n": torch.zeros(nE, NX, dtype=torch.float64, device=dev),
+           "info": torch.zeros(4 * nE, dtype=torch.int32, device=dev)}
+    rgf.solve_batched(p, torch.tensor(E, dtype=torch.float64, device=dev), (T, T), (mu, mu), Hd, Cd, out=out)
+    torch.cuda.synchronize()
+    info = out["info"].cpu().numpy().reshape(nE, 4)
+    assert np.all(info[:, 3] == 0), info
+    f = 1.0 / (1.0 + np.exp((E - mu) / (BOLTZ * T)))
+    worst = 0.0
+    for e in range(nE):
+        G, Gn = out["ldos"][e], out["ndens"][e]
+        A = f[e] * (G - G.transpose(-1, -2).conj())
+        err = torch.linalg.matrix_norm(Gn + A) / (f[e] * torch.linalg.matrix_norm(G))
+        worst = max(worst, float(err.max()))
+    print(f"equilibrium identity: worst relative error over {nE} energies x {NX} slabs = {worst:.2e}")
+    assert worst < TOL_IDENTITY
+    dn = out["dens_n"].cpu().numpy()
+    assert np.all(np.isfinite(dn)) and dn.min() >= -1e-12 * np.abs(dn).max()
+
+
+def test_full_size_current_conservation_and_transmission_bounds():
+    torch, rgf = _env()
+    dev = torch.device("cuda:0")
+    Hd, Cd = _device(torch, rgf, dev)
+    nE = len(E)
+    p = rgf.params(rgf.MODE_PHYSICAL, eta_dev=0.0, eta_lead=1e-5, dec_tol=1e-12)
+    out = {"tr": torch.zeros(nE, dtype=torch.float64, device=dev),
+           "tre": torch.zeros(nE, dtype=torch.float64, device=dev),
+           "info": torch.zeros(4 * nE, dtype=torch.int32, device=dev)}
+    # f_L = 1, f_R = 0: the chemical potentials sit 100 eV above / below the energies
+    rgf.solve_batched(p, torch.tensor(E, dtype=torch.float64, device=dev), (300.0, 300.0), (100.0, -100.0),
+                      Hd, Cd, out=out)
+    torch.cuda.synchronize()
+    tr, tre = out["tr"].cpu().numpy(), out["tre"].cpu().numpy()
+    print("T(E):", tr, "left-contact spectrum:", tre)
+    assert np.all(np.isfinite(tr)) and np.all(tr >= -1e-9) and np.all(tr <= NM)
+    cons = np.abs(tr - tre) / np.maximum(1.0, np.abs(tr))
+    print(f"current conservation: worst relative |tr - tre| = {cons.max():.2e}")
+    assert cons.max() < TOL_CONSERVATION
