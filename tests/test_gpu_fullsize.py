@@ -9,8 +9,9 @@ test_gpu_parity.py).  Physical mode:
     disordered 200-slab device localises its states, so deep slabs have an
     exponentially small G - G^H); electron densities Im Tr G^<_ll >= 0;
   * f_L = 1, f_R = 0 with eta_dev = 0 (no absorption in the device): the current
-    spectra at the two contacts agree (tr = tre = T(E)) and 0 <= T(E) <= nm.
-Tolerances are relative and written below."""
+    spectra at the two contacts agree (tr = tre = T(E)) and 0 <= T(E) <= nm;
+and the adjoint benchmark's device (nm = 128, nx = 300) against central differences of
+the GPU forward solve.  Tolerances are relative and written below."""
 import numpy as np
 import pytest
 
@@ -88,3 +89,49 @@ def test_full_size_current_conservation_and_transmission_bounds():
     cons = np.abs(tr - tre) / np.maximum(1.0, np.abs(tr))
     print(f"current conservation: worst relative |tr - tre| = {cons.max():.2e}")
     assert cons.max() < TOL_CONSERVATION
+
+
+def test_full_size_adjoint_matches_gpu_central_differences():
+    """The adjoint benchmark's device at full size (nm = 128, nx = 300, Fermi leads
+    mu = +-0.1 eV, kT = 0.0259 eV): J = sum_E dE tr(E) from the adjoint kernel equals J
+    from the forward solve, and its directional derivative along a random potential on
+    8 interior slabs equals the central difference (h = 1e-6 eV) of two forward solves
+    (the edge slabs, which also define the leads, are not perturbed)."""
+    torch, rgf = _env()
+    dev = torch.device("cuda:0")
+    nm, nx = 128, 300
+    Hii, H1i = configs.random_device(nm, nx, 4)  # the adjoint benchmark's device
+    Hd, Cd = rgf.to_device_blocks(Hii, dev), rgf.to_device_blocks(H1i, dev)
+    E = np.array([-0.08, -0.02, 0.03, 0.07])
+    nE = len(E)
+    T = 0.0259 / BOLTZ
+    temp, mu = (T, T), (0.1, -0.1)
+    p = rgf.params(rgf.MODE_PHYSICAL, eta_dev=1e-5, eta_lead=1e-5, dec_tol=1e-12)
+    Et = torch.tensor(E, dtype=torch.float64, device=dev)
+    w = torch.full((nE,), 0.01, dtype=torch.float64, device=dev)
+    rng = np.random.default_rng(11)
+    slabs = rng.choice(np.arange(1, nx - 1), 8, replace=False)
+    d = np.zeros(nx)
+    d[slabs] = rng.standard_normal(8)
+    J, dU, _ = rgf.adjoint_batched(p, Et, temp, mu, Hd, Cd, w_tr=w, want_dH=False)
+    torch.cuda.synchronize()
+    j_adj = float(J.sum())
+    deriv = float((dU.sum(0).cpu().numpy() * d).sum())
+
+    def j_forward(s):
+        Hs = Hd.clone()
+        for l in slabs:
+            Hs[l].diagonal().add_(s * d[l])
+        tr = torch.zeros(nE, dtype=torch.float64, device=dev)
+        rgf.solve_batched(p, Et, temp, mu, Hs, Cd, out={"tr": tr})
+        torch.cuda.synchronize()
+        return float((w * tr).sum())
+
+    h = 1e-6
+    j0 = j_forward(0.0)
+    fd = (j_forward(h) - j_forward(-h)) / (2 * h)
+    print(f"J adjoint {j_adj:.15e} forward {j0:.15e}; dJ/ds adjoint {deriv:.10e} central difference {fd:.10e}")
+    assert abs(j_adj - j0) <= 1e-10 * abs(j0)
+    # the O(h^2) truncation is ~1e-12 relative; the rounding of J over 2h bounds the
+    # quotient (measured agreement 1.5e-8 relative)
+    assert abs(fd - deriv) <= 1e-6 * abs(deriv)
