@@ -135,3 +135,42 @@ def test_full_size_adjoint_matches_gpu_central_differences():
     # the O(h^2) truncation is ~1e-12 relative; the rounding of J over 2h bounds the
     # quotient (measured agreement 1.5e-8 relative)
     assert abs(fd - deriv) <= 1e-6 * abs(deriv)
+
+
+def test_full_size_offdiag_sigma_equilibrium_identity():
+    """The SCBA benchmark's device at full size (nm = 64, nx = 100) with block-tridiagonal
+    scattering Sigma of size ~1e-3 whose lesser part uses the leads' Fermi function,
+    Sigma^< = -f(E) (Sigma^R - Sigma^A), so the whole system is in equilibrium
+    (eta_dev = 0): on every slab G^<_ll = -f (G_ll - G_ll^H), and on every bond
+    (1 - f) G^<_{l,l+1} + f G^>_{l,l+1} = 0.  Errors are relative to the size of the terms
+    that cancel; the bar is the one of the nm = 256 identity above."""
+    from tests.dense_ref import fermi
+    torch, rgf = _env()
+    dev = torch.device("cuda:0")
+    nm, nx = 64, 100
+    Hii, H1i = configs.random_device(nm, nx, 5)  # the SCBA benchmark's device
+    blocks = configs.scattering_blocks(nm, nx, 6, scale=1e-3)
+    Ev = np.array([-0.15, -0.05, 0.05, 0.15])
+    nE, T, mu = len(Ev), 300.0, 0.0
+    f = fermi(Ev, mu, BOLTZ, T)
+    per_e = [configs.offdiag_self_energies(blocks, f[e]) for e in range(nE)]
+    names = ("sigL", "sigR", "sigR_lo", "sigR_up", "sigL_up")
+    sd = {k: rgf.to_device_blocks(np.stack([s[i] for s in per_e], axis=-1), dev) for i, k in enumerate(names)}
+    p = rgf.params(rgf.MODE_PHYSICAL, eta_dev=0.0, eta_lead=1e-4, dec_tol=1e-12)
+    out = {k: torch.zeros(nE, nx, nm, nm, dtype=torch.complex128, device=dev) for k in ("ldos", "ndens", "GLi1", "GGi1")}
+    out["info"] = torch.zeros(4 * nE, dtype=torch.int32, device=dev)
+    rgf.solve_batched(p, torch.tensor(Ev, dtype=torch.float64, device=dev), (T, T), (mu, mu),
+                      rgf.to_device_blocks(Hii, dev), rgf.to_device_blocks(H1i, dev), out=out, **sd)
+    torch.cuda.synchronize()
+    assert np.all(out["info"].cpu().numpy().reshape(nE, 4)[:, 3] == 0)
+    wd = wo = 0.0
+    for e in range(nE):
+        G, Gn = out["ldos"][e], out["ndens"][e]
+        r = torch.linalg.matrix_norm(Gn + f[e] * (G - G.transpose(-1, -2).conj())) / (f[e] * torch.linalg.matrix_norm(G))
+        wd = max(wd, float(r.max()))
+        Gl, Gg = out["GLi1"][e, : nx - 1], out["GGi1"][e, : nx - 1]
+        num = torch.linalg.matrix_norm((1.0 - f[e]) * Gl + f[e] * Gg)
+        den = (1.0 - f[e]) * torch.linalg.matrix_norm(Gl) + f[e] * torch.linalg.matrix_norm(Gg)
+        wo = max(wo, float((num / den).max()))
+    print(f"equilibrium with block-tridiagonal Sigma: diagonal {wd:.2e}, bonds {wo:.2e}")
+    assert wd < TOL_IDENTITY and wo < TOL_IDENTITY
