@@ -174,3 +174,37 @@ def test_full_size_offdiag_sigma_equilibrium_identity():
         wo = max(wo, float((num / den).max()))
     print(f"equilibrium with block-tridiagonal Sigma: diagonal {wd:.2e}, bonds {wo:.2e}")
     assert wd < TOL_IDENTITY and wo < TOL_IDENTITY
+
+
+def test_full_size_uneven_devices_one_launch_equals_per_device_launches():
+    """Devices with the shapes of the many-device workload (nm from 64 to 256, nx from
+    50 to 400 slabs, the largest slots of that workload) in one persistent launch give
+    the same bits as one rgf_solve_batched launch per device (tile variants fixed: no
+    autotuner), and every energy's decimations converge with no singular slab."""
+    torch, rgf = _env()
+    dev = torch.device("cuda:0")
+    specs = [(64, 400, 3), (96, 250, 2), (128, 180, 2), (192, 90, 2), (256, 50, 2)]
+    p = rgf.params(rgf.MODE_PHYSICAL, eta_dev=1e-5, eta_lead=1e-5, dec_tol=1e-12)
+    devs = []
+    for j, (nm, nx, nE) in enumerate(specs):
+        Hii, H1i = configs.random_device(nm, nx, 500 + j)
+        E = np.linspace(-0.4, 0.4, nE) + 0.013 * j
+        out = {"tr": torch.zeros(nE, dtype=torch.float64, device=dev),
+               "tre": torch.zeros(nE, dtype=torch.float64, device=dev),
+               "dens_n": torch.zeros(nE, nx, dtype=torch.float64, device=dev),
+               "info": torch.zeros(nE, 4, dtype=torch.int32, device=dev)}
+        devs.append({"E": torch.tensor(E, dtype=torch.float64, device=dev), "temp": (300.0, 300.0),
+                     "mu": (0.1, -0.1), "Hii": rgf.to_device_blocks(Hii, dev), "H1i": rgf.to_device_blocks(H1i, dev),
+                     "out": out})
+    tiles, rep = rgf.solve_devices(p, devs, autotune=False)
+    torch.cuda.synchronize()
+    assert rep["items"] == sum(s[2] for s in specs)
+    for j, d in enumerate(devs):
+        o = {k: v.clone().zero_() for k, v in d["out"].items()}
+        rgf.solve_batched(p, d["E"], d["temp"], d["mu"], d["Hii"], d["H1i"], out=o)
+        torch.cuda.synchronize()
+        for k in o:
+            assert torch.equal(d["out"][k], o[k]), (j, k)
+        info = o["info"].cpu().numpy()
+        assert np.all(info[:, 2] == 0) and np.all(info[:, 3] == 0), (j, info)
+        assert np.all(np.isfinite(o["tr"].cpu().numpy())) and np.all(np.isfinite(o["dens_n"].cpu().numpy()))
