@@ -6,8 +6,10 @@ scale of those terms (the error a faithful evaluation of the same formula can
 make), not against their possibly tiny result:
 
   * G^>_ll = G^<_ll + G_ll - G_ll^H (green_rgf.f90:4056, 4148).  Below the Fermi
-    level 1 - f -> 0 and |G^>| is ~(1-f) times the operands, so the scale is
-    |G^<_ll| + |G_ll - G_ll^H|.
+    level 1 - f -> 0 and |G^>| is ~(1-f) times the operands; in a band gap
+    G_ll is nearly Hermitian (|G - G^H| can fall to 1e-15 |G| in the middle of
+    a device), so the scale is that of the operands, |G^<_ll| + 2 |G_ll| (any
+    G_ll within the ldos bar gives a G^>_ll within this one).
   * GGi1 (compat) = g^H H^H G^>_{nx,nx} inherits the amplification
     amp = (|G^<_nn| + |G_nn - G_nn^H|) / |G^>_nn| of its G^> factor; GGi1
     (physical) = G^<_{l,l+1} + G_{l,l+1} - G_{l+1,l}^H has the scale
@@ -26,14 +28,10 @@ def _n(a):
     return float(np.linalg.norm(a))
 
 
-def _herm_diff(a):
-    return a - np.conj(a.T)
-
-
 def amp_gp(ref, l):
     """Cancellation amplification of G^>_ll = G^<_ll + G_ll - G_ll^H."""
     gp = _n(ref["pdens"][..., l])
-    scale = _n(ref["ndens"][..., l]) + _n(_herm_diff(ref["ldos"][..., l]))
+    scale = _n(ref["ndens"][..., l]) + 2 * _n(ref["ldos"][..., l])
     return max(1.0, scale / gp) if gp > 0 else float("inf")
 
 
@@ -46,7 +44,7 @@ def block_errors(gpu, ref, H1i, mode):
                 yield k, l, _n(gpu[k][..., l] - ref[k][..., l]), TOL * _n(ref[k][..., l])
     if "pdens" in gpu:
         for l in range(nx):
-            scale = _n(ref["ndens"][..., l]) + _n(_herm_diff(ref["ldos"][..., l]))
+            scale = _n(ref["ndens"][..., l]) + 2 * _n(ref["ldos"][..., l])
             yield "pdens", l, _n(gpu["pdens"][..., l] - ref["pdens"][..., l]), TOL * scale
     for k in ("GRi1", "GLi1"):
         if k in gpu:

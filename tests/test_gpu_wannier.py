@@ -1,0 +1,64 @@
+"""GPU: the reference's example device (examples/cnt_8-0: a Wannier (8,0) nanotube,
+slabs of 4 cells = 128 orbitals, 18 slabs, the potential of pot_dat, temperatures and
+chemical potentials of its input file) through liteqt_amd/wannier.py and the engine,
+against the CPU oracle on energies of the example's grid; and the pristine tube's
+transmission against the number of propagating modes of its bands."""
+import os
+
+import numpy as np
+import pytest
+
+from liteqt_amd import configs, wannier
+from oracle import pyoracle as orc
+from tests import parity
+
+pytestmark = pytest.mark.gpu
+
+EX = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "cnt_8-0")
+NS, LENGTH = 4, 18                        # input: cells per slab, device length in slabs
+TEMP, MU = (300.0, 300.0), (-3.0, -3.6)   # input: temperatures, mu_s mu_d
+
+
+def _rgf():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.fail("no HIP device visible")
+    from liteqt_amd import rgf
+    return rgf
+
+
+def test_example_device_matches_oracle():
+    rgf = _rgf()
+    w = wannier.load_ham_dat(os.path.join(EX, "ham_dat"))
+    pot = wannier.read_pot(os.path.join(EX, "pot_dat"), LENGTH)
+    Hii, H1i = wannier.device_blocks(w, NS, LENGTH, pot, potscale=1.0)
+    grid = configs.energy_grid(-15.0, 5.0, 2000)  # the example's emin, emax, num_en
+    for ie in (420, 940, 1210, 1650):
+        g = rgf.rgf_rs(TEMP, grid[ie], MU, Hii, H1i)
+        r = orc.rgf(orc.params(orc.COMPAT), TEMP, grid[ie], MU, Hii, H1i)
+        parity.check(g, r, H1i, "compat", f"cnt_8-0 E={grid[ie]:.3f}")
+
+
+def _modes(w, E, rmax, nk=2001):
+    """Right-moving propagating modes at E of h(k) = sum_{|R|<=rmax} H(R) e^{-ikR}:
+    band crossings of E over the Brillouin zone / 2."""
+    ks = np.linspace(-np.pi, np.pi, nk)
+    Hs = [w.Hr[:, :, x - w.xmin, 0] for x in range(-rmax, rmax + 1)]
+    bands = np.array([np.linalg.eigvalsh(sum(h * np.exp(-1j * k * x) for h, x in zip(Hs, range(-rmax, rmax + 1))))
+                      for k in ks])
+    s = np.sign(bands - E)
+    return int(np.sum(s[1:] != s[:-1]) // 2)
+
+
+def test_pristine_tube_transmission_counts_modes():
+    """Hermitian hoppings up to NS cells (what a slab and its neighbours hold exactly),
+    no potential, f_L = 1, f_R = 0: T(E) is the number of propagating modes."""
+    from tests.test_wannier import _hermitian, _truncated
+    rgf = _rgf()
+    w = _truncated(_hermitian(wannier.load_ham_dat(os.path.join(EX, "ham_dat"))), NS)
+    Hii, H1i = wannier.device_blocks(w, NS, 6)
+    p = rgf.params(rgf.MODE_PHYSICAL, eta_dev=0.0, eta_lead=1e-7, dec_tol=1e-14, dec_nmax=200)
+    for E in (-9.3, -6.1, -4.4, -1.7, 1.2):
+        M = _modes(w, E, NS)
+        g = rgf.rgf_rs((300.0, 300.0), E, (100.0, -100.0), Hii, H1i, p=p, offdiag=False)
+        assert abs(g["tr"] - M) < 1e-4 and abs(g["tre"] - M) < 1e-4, (E, g["tr"], g["tre"], M)
