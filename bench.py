@@ -53,7 +53,7 @@ def kernel_source_hash():
     h = hashlib.sha256()
     csrc = os.path.join(ROOT, "liteqt_amd", "csrc")
     for f in sorted(os.listdir(csrc)):
-        if f.endswith((".hpp", ".hip")) and f != "rgf_plain.hip":
+        if f.endswith((".hpp", ".hip")) and f not in ("rgf_plain.hip", "gw_step.hip"):
             t = open(os.path.join(csrc, f)).read()
             t = re.sub(r"/\*.*?\*/", " ", t, flags=re.S)
             t = re.sub(r"//[^\n]*", " ", t)

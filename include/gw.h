@@ -1,0 +1,69 @@
+/* GW self-energy step of the one-dimensional SCBA driver, on MI355X (gfx950).
+ *
+ * Replaces the loop bodies between two RGF solves of green_rgf_solve_gw_1D
+ * (src/green_rgf.f90:3186-3290): the polarization
+ *   P^<_ij(w)  = dE_P sum_E G^<_ij(E) G^>_ji(E - w)
+ *   P^>_ij(w)  = dE_P sum_E G^>_ij(E) G^<_ji(E - w)
+ *   P^r_ij(w)  = dE_P sum_E [G^<_ij(E) conj(G^r_ij(E - w)) + G^r_ij(E) G^<_ji(E - w)]
+ * (src/green_rgf.f90:3190-3221) and the self-energy
+ *   Sigma^x_ij(E) = dE_S sum_w G^x_ij(E - w) W^x_ij(w)   (x = <, >)
+ *   Sigma^r_ij(E) = dE_S sum_w [G^<_ij W^r_ij + G^r_ij W^<_ij + G^r_ij W^r_ij]
+ * with Im Sigma^r replaced by Im(Sigma^> - Sigma^<)/2, the linear mixing into the
+ * running self-energy and the copy of slab 3 into slabs 1-2 and of slab nx-2 into
+ * slabs nx-1..nx (src/green_rgf.f90:3233-3290).
+ *
+ * All arrays are device pointers to complex double (interleaved re, im) in the
+ * reference's Fortran layout (nm, nm, nx, nen).  As the reference does before this
+ * step (src/green_rgf.f90:3186-3188), only the imaginary parts of G^<, G^>, G^r are
+ * used (their real parts are taken as zero).  Frequencies w = nop * (E2 - E1) with
+ * |nop| <= nopmax are stored at energy index nop + nen/2 (1-based); P is zero at the
+ * other indices and for |i - j| > ndiag.  Sigma sums use the reference's index
+ * condition (E > max(w, E_1) and E - w < E_nen: the last energy of G never enters and
+ * Sigma_new(E_1) = 0).
+ *
+ * Calls on one stream run in order; the arrays must not overlap each other.
+ * Return codes are those of rgf.h (0 ok, 1 invalid argument, 2 HIP error,
+ * 4 unsupported size); gw_last_error() gives the message (per thread).
+ */
+#ifndef LITEQT_GW_H
+#define LITEQT_GW_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int nm;           /* orbitals per slab */
+  int nx;           /* slabs (>= 5 for the self-energy step: the edge copy needs slabs 3 and nx-2) */
+  int nen;          /* energies (uniform grid) */
+  int ndiag;        /* band half-width: pairs with |i - j| <= ndiag (reference: 1 or nb*ns) */
+  int nopmax;       /* largest |nop| (reference 1D driver: nen/2 - 10) */
+  double dE_P[2];   /* complex prefactor of P (reference: (0, -(E2-E1)/(2 pi) * spindeg)) */
+  double dE_S[2];   /* complex prefactor of Sigma (reference: (0, (E2-E1)/(2 pi))) */
+  double alpha_mix; /* Sigma_gw += alpha_mix * (Sigma_new - Sigma_gw) */
+} gw_params;
+
+/* gw_params of green_rgf_solve_gw_1D for an energy grid of nen points spaced dE. */
+void gw_params_reference(gw_params* p, int nm, int nx, int nen, int ndiag, double dE, double spindeg,
+                         double alpha_mix);
+
+/* P^<, P^>, P^r from G^<, G^>, G^r (src/green_rgf.f90:3190-3221).  nen <= 3072. */
+int gw_polarization(const gw_params* p, const double* Gl, const double* Gg, const double* Gr, double* Pl,
+                    double* Pg, double* Pr, void* stream);
+
+/* Sigma^<, Sigma^>, Sigma^r (the running self-energy, updated in place) from G and
+ * W^<, W^>, W^r (src/green_rgf.f90:3233-3290).  nen <= 2040, nx >= 5. */
+int gw_selfenergy(const gw_params* p, const double* Gl, const double* Gg, const double* Gr, const double* Wl,
+                  const double* Wg, const double* Wr, double* Sl, double* Sg, double* Sr, void* stream);
+
+/* Valid (ie, nop) pairs summed per band element (algorithmic work): for P and for Sigma. */
+long long gw_pairs_polarization(int nen, int nopmax);
+long long gw_pairs_selfenergy(int nen, int nopmax);
+
+const char* gw_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,485 @@
+// GW self-energy step of the one-dimensional SCBA driver (green_rgf_solve_gw_1D,
+// src/green_rgf.f90:3186-3290) for gfx950: polarization P^{<,>,r} and self-energy
+// Sigma^{<,>,r} as energy correlations of the band elements (i, j, slab), mixing and
+// edge copy.  C ABI in include/gw.h.
+//
+// Work per band element is a 1D correlation of length nen: O(nen^2) FP64 FMAs against
+// O(nen) data, so one workgroup takes one element, stages its energy series in LDS
+// and runs the correlation from LDS at the FP64 FMA rate.  The reference zeroes the
+// real parts of G before this step (src/green_rgf.f90:3186-3188), so the G series are
+// real (their imaginary parts): P^< = dE_P * (-sum a d) etc. are sums of real
+// products, Sigma sums real G times complex W.
+//
+// Lane mapping: each lane owns 4 consecutive outputs and slides a 4-value register
+// window along the shifted ("Y") series, so one LDS read per series and step feeds 4
+// outputs; the other series ("X") is read at the same address by the whole wave
+// (broadcast).  The Y series are stored zero-padded and 4-way de-interleaved
+// (padded index p at (p % 4) * Q + p / 4), so the lanes' reads, 4 elements apart,
+// fall on consecutive words.  Zero padding also takes the place of every index
+// bound of the reference's loops: a term outside them reads a zero.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+
+#include "../../include/gw.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                        \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess) return fail(2, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+constexpr int NTH = 256;   // threads per workgroup: 4 waves x 64 lanes x 4 outputs = 1024 outputs per pass
+constexpr int PADY = 260;  // zero padding of a Y series on each side (a wave's window reaches 258 past its ends)
+
+struct GwArgs {
+  int nm, nx, nen, nd, nopmax, half;  // nd = min(ndiag, nm - 1), half = nen / 2
+  int slab0;                          // first slab of the grid
+  double dr, di;                      // complex prefactor
+  double alpha;
+  const double* Gl;
+  const double* Gg;
+  const double* Gr;
+  const double* Wl;
+  const double* Wg;
+  const double* Wr;
+  double* Ol;  // P or Sigma outputs
+  double* Og;
+  double* Or;
+};
+
+__device__ __forceinline__ int dpos(int p, int q) { return (p & 3) * q + (p >> 2); }
+
+// Element (i, j, slab) of a workgroup: blockIdx.x = (slab' * nm + i) * (2 nd + 1) + (j - i + nd).
+__device__ __forceinline__ bool element(const GwArgs& a, int& i, int& j, int& ix) {
+  const int band = 2 * a.nd + 1;
+  long long b = blockIdx.x;
+  const int dj = (int)(b % band);
+  b /= band;
+  i = (int)(b % a.nm);
+  ix = a.slab0 + (int)(b / a.nm);
+  j = i + dj - a.nd;
+  return j >= 0 && j < a.nm;
+}
+
+template <int NENMAX>
+struct PolLds {
+  static constexpr int XL = NENMAX + 4;
+  static constexpr int YL = (NENMAX + 2 * PADY + 3) / 4 * 4;
+  static constexpr int DOUBLES = 3 * XL + 3 * YL;
+};
+
+// P^<_ij(w) = dE_P sum_E G^<_ij(E) G^>_ji(E-w), P^>_ij(w) = dE_P sum_E G^>_ij(E) G^<_ji(E-w),
+// P^r_ij(w) = dE_P sum_E [G^<_ij(E) conj(G^r_ij(E-w)) + G^r_ij(E) G^<_ji(E-w)]
+// (src/green_rgf.f90:3199-3218).  With G = i g: G^< G^> = -a d, G^< conj(G^r) +
+// G^r G^<_ji = a f - c e (a, b, c = g^<_ij, g^>_ij, g^r_ij; d, e = g^>_ji, g^<_ji; f = c).
+template <int NENMAX>
+__global__ __launch_bounds__(NTH) void k_polarization(GwArgs a) {
+  using L = PolLds<NENMAX>;
+  __shared__ double lds[L::DOUBLES];
+  int i, j, ix;
+  if (!element(a, i, j, ix)) return;  // uniform over the workgroup, before any barrier
+  const int nen = a.nen, tid = threadIdx.x;
+  const size_t ES = (size_t)a.nm * a.nm * a.nx;  // energy stride (complex elements)
+  const size_t eij = (size_t)i + (size_t)a.nm * j + (size_t)a.nm * a.nm * ix;
+  const size_t eji = (size_t)j + (size_t)a.nm * i + (size_t)a.nm * a.nm * ix;
+  double* xa = lds;
+  double* xb = xa + L::XL;
+  double* xc = xb + L::XL;
+  double* yd = xc + L::XL;
+  double* ye = yd + L::YL;
+  double* yf = ye + L::YL;
+  constexpr int Q = L::YL / 4;
+  for (int p = tid; p < L::XL; p += NTH) {
+    double va = 0.0, vb = 0.0, vc = 0.0;
+    if (p < nen) {
+      va = a.Gl[2 * (eij + ES * p) + 1];
+      vb = a.Gg[2 * (eij + ES * p) + 1];
+      vc = a.Gr[2 * (eij + ES * p) + 1];
+    }
+    xa[p] = va;
+    xb[p] = vb;
+    xc[p] = vc;
+  }
+  for (int p = tid; p < L::YL; p += NTH) {
+    const int k = p - PADY;
+    double vd = 0.0, ve = 0.0, vf = 0.0;
+    if (k >= 0 && k < nen) {
+      vd = a.Gg[2 * (eji + ES * k) + 1];
+      ve = a.Gl[2 * (eji + ES * k) + 1];
+      vf = a.Gr[2 * (eij + ES * k) + 1];
+    }
+    const int q = dpos(p, Q);
+    yd[q] = vd;
+    ye[q] = ve;
+    yf[q] = vf;
+  }
+  __syncthreads();
+  const int nout = 2 * a.nopmax + 1;
+  const int lo_iop = a.half - 1 - a.nopmax;  // 0-based energy index of nop = -nopmax
+  // indices outside the frequency window hold zero (the reference zeroes P first)
+  for (int p = tid; p < nen; p += NTH) {
+    if (p < lo_iop || p >= lo_iop + nout) {
+      const size_t o = 2 * (eij + ES * p);
+      a.Ol[o] = 0.0; a.Ol[o + 1] = 0.0;
+      a.Og[o] = 0.0; a.Og[o + 1] = 0.0;
+      a.Or[o] = 0.0; a.Or[o + 1] = 0.0;
+    }
+  }
+  const int wave = tid >> 6, lane = tid & 63;
+  for (int ow = 256 * wave; ow < nout; ow += 1024) {
+    const int nop_lo = ow - a.nopmax;
+    const int nop_hi = min(ow + 255, nout - 1) - a.nopmax;
+    const int elo = max(nop_lo, 0);                 // union over the wave of the
+    const int ehi = min(nen - 1, nen - 1 + nop_hi); // valid energies of its outputs
+    const int nb = nop_lo + 4 * lane;               // this lane's outputs: nop = nb .. nb+3
+    double rd[4], re[4], rf[4];
+#pragma unroll
+    for (int s = 1; s < 4; ++s) {  // slots of energies elo-1, elo-2, elo-3
+      const int q = dpos(elo - s - nb + PADY, Q);
+      rd[4 - s] = yd[q];
+      re[4 - s] = ye[q];
+      rf[4 - s] = yf[q];
+    }
+    // Y(e + s - nb) for e = elo + 4 it sits at off[s] + it (the residue mod 4 of the
+    // padded index is fixed per s, its quotient grows by one per iteration)
+    int off[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) off[s] = dpos(elo + s - nb + PADY, Q);
+    double sl[4] = {0, 0, 0, 0}, sg[4] = {0, 0, 0, 0}, sr[4] = {0, 0, 0, 0};
+    for (int e = elo, it = 0; e <= ehi; e += 4, ++it) {
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int m = e + s;
+        rd[s] = yd[off[s] + it];
+        re[s] = ye[off[s] + it];
+        rf[s] = yf[off[s] + it];
+        const double va = xa[m], vb = xb[m], vc = xc[m];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {  // output nb + r reads Y(m - nb - r), slot (s - r) & 3
+          const int q = (s - r) & 3;
+          sl[r] = fma(va, rd[q], sl[r]);
+          sg[r] = fma(vb, re[q], sg[r]);
+          sr[r] = fma(va, rf[q], sr[r]);
+          sr[r] = fma(-vc, re[q], sr[r]);
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int o = ow + 4 * lane + r;
+      if (o < nout) {
+        const size_t w = 2 * (eij + ES * (size_t)(lo_iop + o));
+        a.Ol[w] = -sl[r] * a.dr; a.Ol[w + 1] = -sl[r] * a.di;
+        a.Og[w] = -sg[r] * a.dr; a.Og[w + 1] = -sg[r] * a.di;
+        a.Or[w] = sr[r] * a.dr;  a.Or[w + 1] = sr[r] * a.di;
+      }
+    }
+  }
+}
+
+template <int NENMAX>
+struct SigLds {
+  static constexpr int WL = NENMAX + 4;  // W over o = nop + nopmax (nout <= nen - 1), zero tail
+  static constexpr int YL = (NENMAX + 2 * PADY + 3) / 4 * 4;
+  static constexpr int DOUBLES = 3 * 2 * WL + 3 * YL;
+};
+
+// Sigma_new^x(E) = dE_S sum_w G^x(E-w) W^x(w) over the reference's terms (E_ie with
+// ie > max(nop, 1) and ie < nen + nop: G's index ie - nop in 1..nen-1, output ie = 1
+// empty), Im Sigma_new^r = Im(Sigma_new^> - Sigma_new^<)/2, mixing and edge copy
+// (src/green_rgf.f90:3247-3290).  With G = i g: (i g) W = (-g Im W, g Re W).
+template <int NENMAX>
+__global__ __launch_bounds__(NTH) void k_selfenergy(GwArgs a) {
+  using L = SigLds<NENMAX>;
+  __shared__ double lds[L::DOUBLES];
+  int i, j, ix;
+  if (!element(a, i, j, ix)) return;
+  const int nen = a.nen, tid = threadIdx.x;
+  const size_t ES = (size_t)a.nm * a.nm * a.nx;
+  const size_t SS = (size_t)a.nm * a.nm;  // slab stride
+  const size_t eij = (size_t)i + (size_t)a.nm * j + SS * ix;
+  double* wl = lds;
+  double* wg = wl + 2 * L::WL;
+  double* wr = wg + 2 * L::WL;
+  double* yl = wr + 2 * L::WL;
+  double* yg = yl + L::YL;
+  double* yr = yg + L::YL;
+  constexpr int Q = L::YL / 4;
+  const int nout = 2 * a.nopmax + 1;
+  const int lo_iop = a.half - 1 - a.nopmax;
+  for (int p = tid; p < L::WL; p += NTH) {
+    double2 vl = make_double2(0, 0), vg = vl, vr = vl;
+    if (p < nout) {
+      const size_t o = eij + ES * (size_t)(lo_iop + p);
+      vl = reinterpret_cast<const double2*>(a.Wl)[o];
+      vg = reinterpret_cast<const double2*>(a.Wg)[o];
+      vr = reinterpret_cast<const double2*>(a.Wr)[o];
+    }
+    wl[2 * p] = vl.x; wl[2 * p + 1] = vl.y;
+    wg[2 * p] = vg.x; wg[2 * p + 1] = vg.y;
+    wr[2 * p] = vr.x; wr[2 * p + 1] = vr.y;
+  }
+  for (int p = tid; p < L::YL; p += NTH) {
+    const int k = p - PADY;
+    double vl = 0.0, vg = 0.0, vr = 0.0;
+    if (k >= 0 && k < nen - 1) {  // the last energy of G never enters
+      vl = a.Gl[2 * (eij + ES * k) + 1];
+      vg = a.Gg[2 * (eij + ES * k) + 1];
+      vr = a.Gr[2 * (eij + ES * k) + 1];
+    }
+    const int q = dpos(p, Q);
+    yl[q] = vl;
+    yg[q] = vg;
+    yr[q] = vr;
+  }
+  __syncthreads();
+  const int wave = tid >> 6, lane = tid & 63;
+  const int last = a.nx - 3;
+  for (int ew = 256 * wave; ew < nen; ew += 1024) {
+    const int e_hi = min(ew + 255, nen - 1);
+    const int nlo = max(-a.nopmax, ew - (nen - 2));
+    const int nhi = min(a.nopmax, e_hi);
+    const int eb = ew + 4 * lane;  // this lane's outputs: e = eb .. eb+3
+    const int c = eb - nlo;        // step t reads G(c - t)
+    double gl[4], gg[4], gr[4];
+#pragma unroll
+    for (int s = 1; s < 4; ++s) {  // slots of steps -1, -2, -3: G(c+1), G(c+2), G(c+3)
+      const int q = dpos(c + s + PADY, Q);
+      gl[4 - s] = yl[q];
+      gg[4 - s] = yg[q];
+      gr[4 - s] = yr[q];
+    }
+    // G(c - t) for t = 4 it + s sits at off[s] - it
+    int off[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) off[s] = dpos(c - s + PADY, Q);
+    double lr[4] = {0, 0, 0, 0}, li[4] = {0, 0, 0, 0}, gre[4] = {0, 0, 0, 0}, gim[4] = {0, 0, 0, 0};
+    double rr[4] = {0, 0, 0, 0}, ri[4] = {0, 0, 0, 0};
+    for (int t0 = 0, it = 0; t0 <= nhi - nlo; t0 += 4, ++it) {
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int o = nlo + t0 + s + a.nopmax;  // W index of nop = nlo + t0 + s
+        gl[s] = yl[off[s] - it];
+        gg[s] = yg[off[s] - it];
+        gr[s] = yr[off[s] - it];
+        const double wlr = wl[2 * o], wli = wl[2 * o + 1];
+        const double wgr = wg[2 * o], wgi = wg[2 * o + 1];
+        const double wrr = wr[2 * o], wri = wr[2 * o + 1];
+        const double wsr = wlr + wrr, wsi = wli + wri;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {  // output eb + r reads G(c + r - t), slot (s - r) & 3
+          const int q = (s - r) & 3;
+          lr[r] = fma(-gl[q], wli, lr[r]);
+          li[r] = fma(gl[q], wlr, li[r]);
+          gre[r] = fma(-gg[q], wgi, gre[r]);
+          gim[r] = fma(gg[q], wgr, gim[r]);
+          rr[r] = fma(-gl[q], wri, rr[r]);
+          rr[r] = fma(-gr[q], wsi, rr[r]);
+          ri[r] = fma(gl[q], wrr, ri[r]);
+          ri[r] = fma(gr[q], wsr, ri[r]);
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int e = eb + r;
+      if (e >= nen) continue;
+      const bool first = (e == 0);  // ie = 1 has no term
+      const double2 al = first ? make_double2(0, 0) : make_double2(lr[r], li[r]);
+      const double2 ag = first ? make_double2(0, 0) : make_double2(gre[r], gim[r]);
+      const double2 ar = first ? make_double2(0, 0) : make_double2(rr[r], ri[r]);
+      const double2 nl = make_double2(al.x * a.dr - al.y * a.di, al.x * a.di + al.y * a.dr);
+      const double2 ng = make_double2(ag.x * a.dr - ag.y * a.di, ag.x * a.di + ag.y * a.dr);
+      const double2 nr = make_double2(ar.x * a.dr - ar.y * a.di, (ng.y - nl.y) / 2.0);
+      const size_t o = eij + ES * (size_t)e;
+      double2* Sl = reinterpret_cast<double2*>(a.Ol);
+      double2* Sg = reinterpret_cast<double2*>(a.Og);
+      double2* Sr = reinterpret_cast<double2*>(a.Or);
+      const double2 ol = Sl[o], og = Sg[o], orr = Sr[o];
+      const double2 ml = make_double2(ol.x + a.alpha * (nl.x - ol.x), ol.y + a.alpha * (nl.y - ol.y));
+      const double2 mg = make_double2(og.x + a.alpha * (ng.x - og.x), og.y + a.alpha * (ng.y - og.y));
+      const double2 mr = make_double2(orr.x + a.alpha * (nr.x - orr.x), orr.y + a.alpha * (nr.y - orr.y));
+      Sl[o] = ml; Sg[o] = mg; Sr[o] = mr;
+      if (ix == 2) {  // slabs 1-2 (1-based) take slab 3
+        Sl[o - 2 * SS] = ml; Sg[o - 2 * SS] = mg; Sr[o - 2 * SS] = mr;
+        Sl[o - SS] = ml; Sg[o - SS] = mg; Sr[o - SS] = mr;
+      }
+      if (ix == last) {  // slabs nx-1..nx take slab nx-2
+        Sl[o + SS] = ml; Sg[o + SS] = mg; Sr[o + SS] = mr;
+        Sl[o + 2 * SS] = ml; Sg[o + 2 * SS] = mg; Sr[o + 2 * SS] = mr;
+      }
+    }
+  }
+}
+
+// P outside the band (|i - j| > nd): zero.
+__global__ __launch_bounds__(256) void k_pol_offband(GwArgs a) {
+  const long long n = (long long)a.nm * a.nm * a.nx * a.nen;
+  for (long long idx = blockIdx.x * 256LL + threadIdx.x; idx < n; idx += (long long)gridDim.x * 256) {
+    const int i = (int)(idx % a.nm), j = (int)((idx / a.nm) % a.nm);
+    if (abs(i - j) <= a.nd) continue;
+    reinterpret_cast<double2*>(a.Ol)[idx] = make_double2(0, 0);
+    reinterpret_cast<double2*>(a.Og)[idx] = make_double2(0, 0);
+    reinterpret_cast<double2*>(a.Or)[idx] = make_double2(0, 0);
+  }
+}
+
+// Sigma outside the band: Sigma_new = 0, so Sigma_gw += alpha (0 - Sigma_gw), on slabs
+// 3..nx-2 (1-based), with the same edge copy.
+__global__ __launch_bounds__(256) void k_sig_offband(GwArgs a) {
+  const size_t SS = (size_t)a.nm * a.nm;
+  const int ns = a.nx - 4;
+  const long long n = (long long)SS * ns * a.nen;
+  const int last = a.nx - 3;
+  for (long long idx = blockIdx.x * 256LL + threadIdx.x; idx < n; idx += (long long)gridDim.x * 256) {
+    const long long ij = idx % (long long)SS;
+    const long long r = idx / (long long)SS;
+    const int ix = 2 + (int)(r % ns);
+    const long long e = r / ns;
+    const int i = (int)(ij % a.nm), j = (int)(ij / a.nm);
+    if (abs(i - j) <= a.nd) continue;
+    const size_t o = (size_t)ij + SS * ix + SS * a.nx * (size_t)e;
+    double2* S[3] = {reinterpret_cast<double2*>(a.Ol), reinterpret_cast<double2*>(a.Og),
+                     reinterpret_cast<double2*>(a.Or)};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double2 v = S[k][o];
+      const double2 m = make_double2(v.x + a.alpha * (0.0 - v.x), v.y + a.alpha * (0.0 - v.y));
+      S[k][o] = m;
+      if (ix == 2) { S[k][o - 2 * SS] = m; S[k][o - SS] = m; }
+      if (ix == last) { S[k][o + SS] = m; S[k][o + 2 * SS] = m; }
+    }
+  }
+}
+
+int check(const gw_params* p, int min_nx) {
+  if (!p) return fail(1, "gw: params are required");
+  if (p->nm < 1 || p->nx < min_nx || p->nen < 2)
+    return fail(1, "gw: need nm >= 1, nx >= " + std::to_string(min_nx) + ", nen >= 2");
+  if (p->ndiag < 0) return fail(1, "gw: ndiag must be >= 0");
+  if (p->nopmax < 0 || p->nopmax > p->nen / 2 - 1)
+    return fail(1, "gw: need 0 <= nopmax <= nen/2 - 1 (frequency index nop + nen/2 inside 1..nen)");
+  const long long band = 2LL * std::min(p->ndiag, p->nm - 1) + 1;
+  if ((long long)p->nm * band * p->nx > 2147483647LL) return fail(4, "gw: too many band elements for one launch");
+  return 0;
+}
+
+GwArgs args(const gw_params* p) {
+  GwArgs a{};
+  a.nm = p->nm;
+  a.nx = p->nx;
+  a.nen = p->nen;
+  a.nd = std::min(p->ndiag, p->nm - 1);
+  a.nopmax = p->nopmax;
+  a.half = p->nen / 2;
+  a.alpha = p->alpha_mix;
+  return a;
+}
+
+int offband_grid(long long n) { return (int)std::max(1LL, std::min<long long>((n + 255) / 256, 8192)); }
+
+}  // namespace
+
+extern "C" {
+
+void gw_params_reference(gw_params* p, int nm, int nx, int nen, int ndiag, double dE, double spindeg,
+                         double alpha_mix) {
+  const double pi = 3.14159265358979323846;
+  p->nm = nm;
+  p->nx = nx;
+  p->nen = nen;
+  p->ndiag = ndiag;
+  p->nopmax = nen / 2 - 10;
+  p->dE_P[0] = 0.0;
+  p->dE_P[1] = -1.0 * dE / 2.0 / pi * spindeg;  // dcmplx(0, -1*(En(2)-En(1))/2/pi) * spindeg
+  p->dE_S[0] = 0.0;
+  p->dE_S[1] = dE / 2.0 / pi;                   // dcmplx(0, (En(2)-En(1))/2/pi)
+  p->alpha_mix = alpha_mix;
+}
+
+int gw_polarization(const gw_params* p, const double* Gl, const double* Gg, const double* Gr, double* Pl,
+                    double* Pg, double* Pr, void* stream) {
+  g_err.clear();
+  if (int rc = check(p, 1)) return rc;
+  if (p->nen > 3072) return fail(4, "gw_polarization: nen > 3072 (the energy series of an element live in LDS)");
+  if (!Gl || !Gg || !Gr || !Pl || !Pg || !Pr) return fail(1, "gw_polarization: null array");
+  GwArgs a = args(p);
+  a.slab0 = 0;
+  a.dr = p->dE_P[0];
+  a.di = p->dE_P[1];
+  a.Gl = Gl; a.Gg = Gg; a.Gr = Gr;
+  a.Ol = Pl; a.Og = Pg; a.Or = Pr;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  const long long nel = (long long)a.nm * (2 * a.nd + 1) * a.nx;
+  if (a.nd < a.nm - 1) {
+    hipLaunchKernelGGL(k_pol_offband, dim3(offband_grid((long long)a.nm * a.nm * a.nx * a.nen)), dim3(256), 0, s, a);
+    HIPCHK(hipGetLastError());
+  }
+  if (a.nen <= 1024)
+    hipLaunchKernelGGL(k_polarization<1024>, dim3((unsigned)nel), dim3(NTH), 0, s, a);
+  else if (a.nen <= 2048)
+    hipLaunchKernelGGL(k_polarization<2048>, dim3((unsigned)nel), dim3(NTH), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_polarization<3072>, dim3((unsigned)nel), dim3(NTH), 0, s, a);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int gw_selfenergy(const gw_params* p, const double* Gl, const double* Gg, const double* Gr, const double* Wl,
+                  const double* Wg, const double* Wr, double* Sl, double* Sg, double* Sr, void* stream) {
+  g_err.clear();
+  if (int rc = check(p, 5)) return rc;
+  if (p->nen > 2040) return fail(4, "gw_selfenergy: nen > 2040 (the energy series of an element live in LDS)");
+  if (!Gl || !Gg || !Gr || !Wl || !Wg || !Wr || !Sl || !Sg || !Sr) return fail(1, "gw_selfenergy: null array");
+  GwArgs a = args(p);
+  a.slab0 = 2;
+  a.dr = p->dE_S[0];
+  a.di = p->dE_S[1];
+  a.Gl = Gl; a.Gg = Gg; a.Gr = Gr;
+  a.Wl = Wl; a.Wg = Wg; a.Wr = Wr;
+  a.Ol = Sl; a.Og = Sg; a.Or = Sr;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  const long long nel = (long long)a.nm * (2 * a.nd + 1) * (a.nx - 4);
+  if (a.nd < a.nm - 1) {
+    hipLaunchKernelGGL(k_sig_offband, dim3(offband_grid((long long)a.nm * a.nm * (a.nx - 4) * a.nen)), dim3(256), 0,
+                       s, a);
+    HIPCHK(hipGetLastError());
+  }
+  if (a.nen <= 1024)
+    hipLaunchKernelGGL(k_selfenergy<1024>, dim3((unsigned)nel), dim3(NTH), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_selfenergy<2040>, dim3((unsigned)nel), dim3(NTH), 0, s, a);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+long long gw_pairs_polarization(int nen, int nopmax) {
+  long long n = 0;
+  for (int nop = -nopmax; nop <= nopmax; ++nop) n += nen - (nop < 0 ? -nop : nop);
+  return n;
+}
+
+long long gw_pairs_selfenergy(int nen, int nopmax) {
+  long long n = 0;
+  for (int e = 1; e < nen; ++e) {
+    const int lo = std::max(-nopmax, e - (nen - 2)), hi = std::min(nopmax, e);
+    if (hi >= lo) n += hi - lo + 1;
+  }
+  return n;
+}
+
+const char* gw_last_error(void) { return g_err.c_str(); }
+
+}  // extern "C"
