@@ -301,6 +301,85 @@ def devices_cost(rgf, torch, dev, ndev, e_div, compare, seed=3):
             "tile_T_by_nm": {str(nm): int(tiles[[s[0] for s in shapes].index(nm)]) for nm in sizes}}
 
 
+def _gw_leg(gw, torch, nm, nx, nen, ndiag, steps, warmup):
+    """Time gw.polarization and gw.selfenergy (events on the launch stream) for one
+    device size; returns ms per call, band elements and algorithmic FLOPs."""
+    dE = 20.0 / (nen - 1)
+    p = gw.params_reference(nm, nx, nen, ndiag, dE, 2.0, 0.5)
+    g = torch.Generator(device="cuda:0").manual_seed(7)
+
+    def rnd():
+        return 0.1 * torch.randn((nen, nx, nm, nm), dtype=torch.complex128, device="cuda:0", generator=g)
+
+    G, W, S = [rnd() for _ in range(3)], [rnd() for _ in range(3)], [rnd() for _ in range(3)]
+    P = [torch.empty_like(G[0]) for _ in range(3)]
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    tp = ts = 0.0
+    for it in range(warmup + steps):
+        ev[0].record()
+        gw.polarization(p, *G, *P)
+        ev[1].record()
+        gw.selfenergy(p, *G, *W, *S)
+        ev[2].record()
+        torch.cuda.synchronize()
+        if it >= warmup:
+            tp += ev[0].elapsed_time(ev[1]) / steps
+            ts += ev[1].elapsed_time(ev[2]) / steps
+    nd = min(ndiag, nm - 1)
+    per_slab = nm * (2 * nd + 1) - nd * (nd + 1)  # pairs (i, j) with |i - j| <= nd
+    L = gw.lib()
+    pairs_p = L.gw_pairs_polarization(nen, p.nopmax) * per_slab * nx
+    pairs_s = L.gw_pairs_selfenergy(nen, p.nopmax) * per_slab * (nx - 4)
+    del G, W, S, P
+    torch.cuda.empty_cache()
+    return {"nm": nm, "nx": nx, "nen": nen, "ndiag": ndiag, "nopmax": p.nopmax, "ms_polarization": tp,
+            "ms_selfenergy": ts, "band_elements": per_slab * nx, "pairs_polarization": pairs_p,
+            "pairs_selfenergy": pairs_s, "flops_polarization": 8.0 * pairs_p, "flops_selfenergy": 16.0 * pairs_s}
+
+
+def gw_bench(args):
+    """The GW step of green_rgf_solve_gw_1D (src/green_rgf.f90:3186-3290) on one GPU:
+    the reference example's size (examples/cnt_8-0: nm = 32 * 4 = 128, 18 slabs,
+    2000 energies, nopmax = nen/2 - 10) with ndiag = 1 (its ldiag setting), and a
+    full-band device (ndiag = nm - 1, the driver's other setting) of one cell (nm = 32),
+    5 slabs, 2000 energies, where only the FP64 correlation kernels run."""
+    import torch
+    from liteqt_amd import gw
+    from oracle import gw_oracle
+    ex = _gw_leg(gw, torch, 128, 18, 2000, 1, args.steps, args.warmup)
+    full = _gw_leg(gw, torch, 32, 5, 2000, 31, args.steps, args.warmup)
+    ms = ex["ms_polarization"] + ex["ms_selfenergy"]
+    pairs = ex["pairs_polarization"] + ex["pairs_selfenergy"]
+    fp = full["flops_polarization"] / (full["ms_polarization"] * 1e-3) / 1e12
+    fs = full["flops_selfenergy"] / (full["ms_selfenergy"] * 1e-3) / 1e12
+    res = {
+        "metric": "GW self-energy step (polarization + self-energy): band-element energy pairs per second",
+        "value": pairs / (ms * 1e-3), "unit": "pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 (complex G, W, Sigma)", "data": "synthetic",
+        "config": {"workload": "cnt_8-0 GW step, ldiag", **{k: ex[k] for k in ("nm", "nx", "nen", "ndiag", "nopmax")}},
+        "example_leg": ex,
+        "full_band_leg": full,
+        "roofline": {"bound": "fp64", "unit": "TFLOP/s", "peak": PEAK_FP64_MFMA,
+                     "achieved": fp, "frac": fp / PEAK_FP64_MFMA,
+                     "kernel": "k_polarization<2048> (full band nm=32, 5 slabs, nen=2000)",
+                     "selfenergy_achieved": fs, "selfenergy_frac": fs / PEAK_FP64_MFMA, "traffic": None},
+    }
+    if not args.no_cpu:
+        # the numpy oracle on a bounded sample: 16 band elements (4 x 4, one slab), 2000 energies
+        rng = np.random.default_rng(1)
+        G = [rng.standard_normal((4, 4, 1, 2000)) + 1j * rng.standard_normal((4, 4, 1, 2000)) for _ in range(3)]
+        nopmax, dE_P, _ = gw_oracle.reference_params(2000, 0.01, 2.0)
+        t0 = time.perf_counter()
+        gw_oracle.polarization(*G, 3, nopmax, dE_P)
+        dt = time.perf_counter() - t0
+        n = 16 * gw.lib().gw_pairs_polarization(2000, nopmax)
+        res["cpu_baseline"] = {"value": n / dt, "unit": "pairs/s", "cores": 1, "kind": "port",
+                               "sample": "polarization of 16 band elements (4x4, one slab), 2000 energies"}
+        res["gpu_over_cpu"] = full["pairs_polarization"] / (full["ms_polarization"] * 1e-3) / res["cpu_baseline"]["value"]
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -323,6 +402,8 @@ def main():
                     help="divide each device's energy count (U[256,2048]) by this")
     ap.add_argument("--plain-energies", type=int, default=4,
                     help="energies timed on the plain per-energy GPU path (0: skip)")
+    ap.add_argument("--gw", action="store_true",
+                    help="only measure the GW self-energy step (polarization + self-energy correlations)")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -341,6 +422,10 @@ def main():
         print(json.dumps({"error": "this RGF build runs on one GPU; multi-GPU energy sharding is a design note only",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+
+    if args.gw:
+        gw_bench(args)
+        return
 
     import torch
     from liteqt_amd import rgf
