@@ -10,13 +10,14 @@
 // real (their imaginary parts): P^< = dE_P * (-sum a d) etc. are sums of real
 // products, Sigma sums real G times complex W.
 //
-// Lane mapping: each lane owns 4 consecutive outputs and slides a 4-value register
-// window along the shifted ("Y") series, so one LDS read per series and step feeds 4
-// outputs; the other series ("X") is read at the same address by the whole wave
-// (broadcast).  The Y series are stored zero-padded and 4-way de-interleaved
-// (padded index p at (p % 4) * Q + p / 4), so the lanes' reads, 4 elements apart,
-// fall on consecutive words.  Zero padding also takes the place of every index
-// bound of the reference's loops: a term outside them reads a zero.
+// Lane mapping: each lane owns R consecutive outputs (8 in the polarization, 4 in the
+// self-energy) and slides an R-value register window along the shifted ("Y") series,
+// so one LDS read per series and step feeds R outputs; the other series ("X") is read
+// at the same address by the whole wave (broadcast).  The Y series are stored
+// zero-padded and R-way de-interleaved (padded index p at (p % R) * Q + p / R), so the
+// lanes' reads, R elements apart, fall on consecutive words.  Zero padding also takes
+// the place of every index bound of the reference's loops: a term outside them reads
+// a zero.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -40,8 +41,12 @@ int fail(int code, const std::string& msg) {
     if (e_ != hipSuccess) return fail(2, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-constexpr int NTH = 256;   // threads per workgroup: 4 waves x 64 lanes x 4 outputs = 1024 outputs per pass
-constexpr int PADY = 260;  // zero padding of a Y series on each side (a wave's window reaches 258 past its ends)
+// polarization: 4 waves x 64 lanes x 8 outputs = 2048 outputs per pass
+constexpr int NTH_P = 256, RP = 8, SP = 64 * RP;
+constexpr int PADP = SP;   // a wave's window reaches SP - 1 below its first output's series start
+// self-energy: 8 waves x 64 lanes x 4 outputs = 2048 outputs per pass (two waves per SIMD)
+constexpr int NTH_S = 512, RS = 4, SS_ = 64 * RS;
+constexpr int PADY = 260;  // zero padding of a self-energy Y series on each side (reach 258)
 
 struct GwArgs {
   int nm, nx, nen, nd, nopmax, half;  // nd = min(ndiag, nm - 1), half = nen / 2
@@ -59,7 +64,8 @@ struct GwArgs {
   double* Or;
 };
 
-__device__ __forceinline__ int dpos(int p, int q) { return (p & 3) * q + (p >> 2); }
+template <int R>
+__device__ __forceinline__ int dpos(int p, int q) { return (p % R) * q + p / R; }  // p >= 0, R a power of 2
 
 // Element (i, j, slab) of a workgroup: blockIdx.x = (slab' * nm + i) * (2 nd + 1) + (j - i + nd).
 __device__ __forceinline__ bool element(const GwArgs& a, int& i, int& j, int& ix) {
@@ -75,8 +81,8 @@ __device__ __forceinline__ bool element(const GwArgs& a, int& i, int& j, int& ix
 
 template <int NENMAX>
 struct PolLds {
-  static constexpr int XL = NENMAX + 4;
-  static constexpr int YL = (NENMAX + 2 * PADY + 3) / 4 * 4;
+  static constexpr int XL = NENMAX + RP;                              // X read up to nen + RP - 2
+  static constexpr int YL = (NENMAX + PADP + SP + RP + RP - 1) / RP * RP;  // Y index up to nen + SP + RP - 3
   static constexpr int DOUBLES = 3 * XL + 3 * YL;
 };
 
@@ -85,7 +91,7 @@ struct PolLds {
 // (src/green_rgf.f90:3199-3218).  With G = i g: G^< G^> = -a d, G^< conj(G^r) +
 // G^r G^<_ji = a f - c e (a, b, c = g^<_ij, g^>_ij, g^r_ij; d, e = g^>_ji, g^<_ji; f = c).
 template <int NENMAX>
-__global__ __launch_bounds__(NTH) void k_polarization(GwArgs a) {
+__global__ __launch_bounds__(NTH_P) void k_polarization(GwArgs a) {
   using L = PolLds<NENMAX>;
   __shared__ double lds[L::DOUBLES];
   int i, j, ix;
@@ -100,8 +106,8 @@ __global__ __launch_bounds__(NTH) void k_polarization(GwArgs a) {
   double* yd = xc + L::XL;
   double* ye = yd + L::YL;
   double* yf = ye + L::YL;
-  constexpr int Q = L::YL / 4;
-  for (int p = tid; p < L::XL; p += NTH) {
+  constexpr int Q = L::YL / RP;
+  for (int p = tid; p < L::XL; p += NTH_P) {
     double va = 0.0, vb = 0.0, vc = 0.0;
     if (p < nen) {
       va = a.Gl[2 * (eij + ES * p) + 1];
@@ -112,15 +118,15 @@ __global__ __launch_bounds__(NTH) void k_polarization(GwArgs a) {
     xb[p] = vb;
     xc[p] = vc;
   }
-  for (int p = tid; p < L::YL; p += NTH) {
-    const int k = p - PADY;
+  for (int p = tid; p < L::YL; p += NTH_P) {
+    const int k = p - PADP;
     double vd = 0.0, ve = 0.0, vf = 0.0;
     if (k >= 0 && k < nen) {
       vd = a.Gg[2 * (eji + ES * k) + 1];
       ve = a.Gl[2 * (eji + ES * k) + 1];
       vf = a.Gr[2 * (eij + ES * k) + 1];
     }
-    const int q = dpos(p, Q);
+    const int q = dpos<RP>(p, Q);
     yd[q] = vd;
     ye[q] = ve;
     yf[q] = vf;
@@ -129,7 +135,7 @@ __global__ __launch_bounds__(NTH) void k_polarization(GwArgs a) {
   const int nout = 2 * a.nopmax + 1;
   const int lo_iop = a.half - 1 - a.nopmax;  // 0-based energy index of nop = -nopmax
   // indices outside the frequency window hold zero (the reference zeroes P first)
-  for (int p = tid; p < nen; p += NTH) {
+  for (int p = tid; p < nen; p += NTH_P) {
     if (p < lo_iop || p >= lo_iop + nout) {
       const size_t o = 2 * (eij + ES * p);
       a.Ol[o] = 0.0; a.Ol[o + 1] = 0.0;
@@ -138,37 +144,39 @@ __global__ __launch_bounds__(NTH) void k_polarization(GwArgs a) {
     }
   }
   const int wave = tid >> 6, lane = tid & 63;
-  for (int ow = 256 * wave; ow < nout; ow += 1024) {
+  for (int ow = SP * wave; ow < nout; ow += 4 * SP) {
     const int nop_lo = ow - a.nopmax;
-    const int nop_hi = min(ow + 255, nout - 1) - a.nopmax;
+    const int nop_hi = min(ow + SP - 1, nout - 1) - a.nopmax;
     const int elo = max(nop_lo, 0);                 // union over the wave of the
     const int ehi = min(nen - 1, nen - 1 + nop_hi); // valid energies of its outputs
-    const int nb = nop_lo + 4 * lane;               // this lane's outputs: nop = nb .. nb+3
-    double rd[4], re[4], rf[4];
+    const int nb = nop_lo + RP * lane;              // this lane's outputs: nop = nb .. nb+RP-1
+    double rd[RP], re[RP], rf[RP];
 #pragma unroll
-    for (int s = 1; s < 4; ++s) {  // slots of energies elo-1, elo-2, elo-3
-      const int q = dpos(elo - s - nb + PADY, Q);
-      rd[4 - s] = yd[q];
-      re[4 - s] = ye[q];
-      rf[4 - s] = yf[q];
+    for (int s = 1; s < RP; ++s) {  // slots of energies elo-1 .. elo-RP+1
+      const int q = dpos<RP>(elo - s - nb + PADP, Q);
+      rd[RP - s] = yd[q];
+      re[RP - s] = ye[q];
+      rf[RP - s] = yf[q];
     }
-    // Y(e + s - nb) for e = elo + 4 it sits at off[s] + it (the residue mod 4 of the
+    // Y(e + s - nb) for e = elo + RP it sits at off[s] + it (the residue mod RP of the
     // padded index is fixed per s, its quotient grows by one per iteration)
-    int off[4];
+    int off[RP];
 #pragma unroll
-    for (int s = 0; s < 4; ++s) off[s] = dpos(elo + s - nb + PADY, Q);
-    double sl[4] = {0, 0, 0, 0}, sg[4] = {0, 0, 0, 0}, sr[4] = {0, 0, 0, 0};
-    for (int e = elo, it = 0; e <= ehi; e += 4, ++it) {
+    for (int s = 0; s < RP; ++s) off[s] = dpos<RP>(elo + s - nb + PADP, Q);
+    double sl[RP], sg[RP], sr[RP];
 #pragma unroll
-      for (int s = 0; s < 4; ++s) {
+    for (int r = 0; r < RP; ++r) sl[r] = sg[r] = sr[r] = 0.0;
+    for (int e = elo, it = 0; e <= ehi; e += RP, ++it) {
+#pragma unroll
+      for (int s = 0; s < RP; ++s) {
         const int m = e + s;
         rd[s] = yd[off[s] + it];
         re[s] = ye[off[s] + it];
         rf[s] = yf[off[s] + it];
         const double va = xa[m], vb = xb[m], vc = xc[m];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {  // output nb + r reads Y(m - nb - r), slot (s - r) & 3
-          const int q = (s - r) & 3;
+        for (int r = 0; r < RP; ++r) {  // output nb + r reads Y(m - nb - r), slot (s - r) mod RP
+          const int q = (s - r) & (RP - 1);
           sl[r] = fma(va, rd[q], sl[r]);
           sg[r] = fma(vb, re[q], sg[r]);
           sr[r] = fma(va, rf[q], sr[r]);
@@ -177,8 +185,8 @@ __global__ __launch_bounds__(NTH) void k_polarization(GwArgs a) {
       }
     }
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int o = ow + 4 * lane + r;
+    for (int r = 0; r < RP; ++r) {
+      const int o = ow + RP * lane + r;
       if (o < nout) {
         const size_t w = 2 * (eij + ES * (size_t)(lo_iop + o));
         a.Ol[w] = -sl[r] * a.dr; a.Ol[w + 1] = -sl[r] * a.di;
@@ -201,7 +209,7 @@ struct SigLds {
 // empty), Im Sigma_new^r = Im(Sigma_new^> - Sigma_new^<)/2, mixing and edge copy
 // (src/green_rgf.f90:3247-3290).  With G = i g: (i g) W = (-g Im W, g Re W).
 template <int NENMAX>
-__global__ __launch_bounds__(NTH) void k_selfenergy(GwArgs a) {
+__global__ __launch_bounds__(NTH_S) void k_selfenergy(GwArgs a) {
   using L = SigLds<NENMAX>;
   __shared__ double lds[L::DOUBLES];
   int i, j, ix;
@@ -219,7 +227,7 @@ __global__ __launch_bounds__(NTH) void k_selfenergy(GwArgs a) {
   constexpr int Q = L::YL / 4;
   const int nout = 2 * a.nopmax + 1;
   const int lo_iop = a.half - 1 - a.nopmax;
-  for (int p = tid; p < L::WL; p += NTH) {
+  for (int p = tid; p < L::WL; p += NTH_S) {
     double2 vl = make_double2(0, 0), vg = vl, vr = vl;
     if (p < nout) {
       const size_t o = eij + ES * (size_t)(lo_iop + p);
@@ -231,7 +239,7 @@ __global__ __launch_bounds__(NTH) void k_selfenergy(GwArgs a) {
     wg[2 * p] = vg.x; wg[2 * p + 1] = vg.y;
     wr[2 * p] = vr.x; wr[2 * p + 1] = vr.y;
   }
-  for (int p = tid; p < L::YL; p += NTH) {
+  for (int p = tid; p < L::YL; p += NTH_S) {
     const int k = p - PADY;
     double vl = 0.0, vg = 0.0, vr = 0.0;
     if (k >= 0 && k < nen - 1) {  // the last energy of G never enters
@@ -239,7 +247,7 @@ __global__ __launch_bounds__(NTH) void k_selfenergy(GwArgs a) {
       vg = a.Gg[2 * (eij + ES * k) + 1];
       vr = a.Gr[2 * (eij + ES * k) + 1];
     }
-    const int q = dpos(p, Q);
+    const int q = dpos<RS>(p, Q);
     yl[q] = vl;
     yg[q] = vg;
     yr[q] = vr;
@@ -247,8 +255,8 @@ __global__ __launch_bounds__(NTH) void k_selfenergy(GwArgs a) {
   __syncthreads();
   const int wave = tid >> 6, lane = tid & 63;
   const int last = a.nx - 3;
-  for (int ew = 256 * wave; ew < nen; ew += 1024) {
-    const int e_hi = min(ew + 255, nen - 1);
+  for (int ew = SS_ * wave; ew < nen; ew += 8 * SS_) {
+    const int e_hi = min(ew + SS_ - 1, nen - 1);
     const int nlo = max(-a.nopmax, ew - (nen - 2));
     const int nhi = min(a.nopmax, e_hi);
     const int eb = ew + 4 * lane;  // this lane's outputs: e = eb .. eb+3
@@ -256,7 +264,7 @@ __global__ __launch_bounds__(NTH) void k_selfenergy(GwArgs a) {
     double gl[4], gg[4], gr[4];
 #pragma unroll
     for (int s = 1; s < 4; ++s) {  // slots of steps -1, -2, -3: G(c+1), G(c+2), G(c+3)
-      const int q = dpos(c + s + PADY, Q);
+      const int q = dpos<RS>(c + s + PADY, Q);
       gl[4 - s] = yl[q];
       gg[4 - s] = yg[q];
       gr[4 - s] = yr[q];
@@ -264,7 +272,7 @@ __global__ __launch_bounds__(NTH) void k_selfenergy(GwArgs a) {
     // G(c - t) for t = 4 it + s sits at off[s] - it
     int off[4];
 #pragma unroll
-    for (int s = 0; s < 4; ++s) off[s] = dpos(c - s + PADY, Q);
+    for (int s = 0; s < 4; ++s) off[s] = dpos<RS>(c - s + PADY, Q);
     double lr[4] = {0, 0, 0, 0}, li[4] = {0, 0, 0, 0}, gre[4] = {0, 0, 0, 0}, gim[4] = {0, 0, 0, 0};
     double rr[4] = {0, 0, 0, 0}, ri[4] = {0, 0, 0, 0};
     for (int t0 = 0, it = 0; t0 <= nhi - nlo; t0 += 4, ++it) {
@@ -413,7 +421,7 @@ int gw_polarization(const gw_params* p, const double* Gl, const double* Gg, cons
                     double* Pg, double* Pr, void* stream) {
   g_err.clear();
   if (int rc = check(p, 1)) return rc;
-  if (p->nen > 3072) return fail(4, "gw_polarization: nen > 3072 (the energy series of an element live in LDS)");
+  if (p->nen > 2560) return fail(4, "gw_polarization: nen > 2560 (the energy series of an element live in LDS)");
   if (!Gl || !Gg || !Gr || !Pl || !Pg || !Pr) return fail(1, "gw_polarization: null array");
   GwArgs a = args(p);
   a.slab0 = 0;
@@ -428,11 +436,11 @@ int gw_polarization(const gw_params* p, const double* Gl, const double* Gg, cons
     HIPCHK(hipGetLastError());
   }
   if (a.nen <= 1024)
-    hipLaunchKernelGGL(k_polarization<1024>, dim3((unsigned)nel), dim3(NTH), 0, s, a);
+    hipLaunchKernelGGL(k_polarization<1024>, dim3((unsigned)nel), dim3(NTH_P), 0, s, a);
   else if (a.nen <= 2048)
-    hipLaunchKernelGGL(k_polarization<2048>, dim3((unsigned)nel), dim3(NTH), 0, s, a);
+    hipLaunchKernelGGL(k_polarization<2048>, dim3((unsigned)nel), dim3(NTH_P), 0, s, a);
   else
-    hipLaunchKernelGGL(k_polarization<3072>, dim3((unsigned)nel), dim3(NTH), 0, s, a);
+    hipLaunchKernelGGL(k_polarization<2560>, dim3((unsigned)nel), dim3(NTH_P), 0, s, a);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -458,9 +466,9 @@ int gw_selfenergy(const gw_params* p, const double* Gl, const double* Gg, const 
     HIPCHK(hipGetLastError());
   }
   if (a.nen <= 1024)
-    hipLaunchKernelGGL(k_selfenergy<1024>, dim3((unsigned)nel), dim3(NTH), 0, s, a);
+    hipLaunchKernelGGL(k_selfenergy<1024>, dim3((unsigned)nel), dim3(NTH_S), 0, s, a);
   else
-    hipLaunchKernelGGL(k_selfenergy<2040>, dim3((unsigned)nel), dim3(NTH), 0, s, a);
+    hipLaunchKernelGGL(k_selfenergy<2040>, dim3((unsigned)nel), dim3(NTH_S), 0, s, a);
   HIPCHK(hipGetLastError());
   return 0;
 }

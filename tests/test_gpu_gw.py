@@ -30,7 +30,7 @@ def _close(got, ref, tag):
 
 
 @pytest.mark.parametrize("nm,nx,nen,ndiag,wide", [(6, 3, 64, 1, False), (5, 2, 301, 9, True), (4, 2, 1100, 1, False),
-                                                  (3, 1, 2048, 2, True), (2, 1, 3072, 1, False)])
+                                                  (3, 1, 2048, 2, True), (2, 1, 2560, 1, False)])
 def test_polarization_matches_oracle(nm, nx, nen, ndiag, wide):
     import torch
     gw = _gw()
