@@ -51,6 +51,7 @@ constexpr int PADY = 260;  // zero padding of a self-energy Y series on each sid
 struct GwArgs {
   int nm, nx, nen, nd, nopmax, half;  // nd = min(ndiag, nm - 1), half = nen / 2
   int slab0;                          // first slab of the grid
+  long long nel;                      // logical blocks (band positions x slabs)
   double dr, di;                      // complex prefactor
   double alpha;
   const double* Gl;
@@ -67,16 +68,23 @@ struct GwArgs {
 template <int R>
 __device__ __forceinline__ int dpos(int p, int q) { return (p % R) * q + p / R; }  // p >= 0, R a power of 2
 
-// Element (i, j, slab) of a workgroup: blockIdx.x = (slab' * nm + i) * (2 nd + 1) + (j - i + nd).
+// Element (i, j, slab) of a workgroup.  Logical block b = (slab' * nm + j) * (2 nd + 1) +
+// (i - j + nd): consecutive blocks are consecutive rows i of one column j, whose energy
+// series are adjacent in memory.  Workgroups are dealt round-robin to the 8 XCDs (each
+// with its own L2), so the grid (a multiple of 8) gives XCD x the contiguous logical range
+// [x * per, (x + 1) * per): the ~32 workgroups an XCD runs at once read neighbouring
+// elements and share the cache lines of their strided series.
 __device__ __forceinline__ bool element(const GwArgs& a, int& i, int& j, int& ix) {
   const int band = 2 * a.nd + 1;
-  long long b = blockIdx.x;
-  const int dj = (int)(b % band);
+  const long long per = gridDim.x / 8;
+  long long b = (long long)(blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (b >= a.nel) return false;
+  const int di = (int)(b % band);
   b /= band;
-  i = (int)(b % a.nm);
+  j = (int)(b % a.nm);
   ix = a.slab0 + (int)(b / a.nm);
-  j = i + dj - a.nd;
-  return j >= 0 && j < a.nm;
+  i = j + di - a.nd;
+  return i >= 0 && i < a.nm;
 }
 
 template <int NENMAX>
@@ -107,29 +115,57 @@ __global__ __launch_bounds__(NTH_P) void k_polarization(GwArgs a) {
   double* ye = yd + L::YL;
   double* yf = ye + L::YL;
   constexpr int Q = L::YL / RP;
-  for (int p = tid; p < L::XL; p += NTH_P) {
-    double va = 0.0, vb = 0.0, vc = 0.0;
-    if (p < nen) {
-      va = a.Gl[2 * (eij + ES * p) + 1];
-      vb = a.Gg[2 * (eij + ES * p) + 1];
-      vc = a.Gr[2 * (eij + ES * p) + 1];
+  // staging: every load of a thread is issued before its LDS stores (one HBM latency
+  // per group instead of one per energy block)
+  {
+    constexpr int NI = (L::XL + NTH_P - 1) / NTH_P;
+    double va[NI], vb[NI], vc[NI];
+#pragma unroll
+    for (int u = 0; u < NI; ++u) {
+      const int p = tid + u * NTH_P;
+      va[u] = vb[u] = vc[u] = 0.0;
+      if (p < nen) {
+        va[u] = a.Gl[2 * (eij + ES * p) + 1];
+        vb[u] = a.Gg[2 * (eij + ES * p) + 1];
+        vc[u] = a.Gr[2 * (eij + ES * p) + 1];
+      }
     }
-    xa[p] = va;
-    xb[p] = vb;
-    xc[p] = vc;
+#pragma unroll
+    for (int u = 0; u < NI; ++u) {
+      const int p = tid + u * NTH_P;
+      if (p < L::XL) {
+        xa[p] = va[u];
+        xb[p] = vb[u];
+        xc[p] = vc[u];
+      }
+    }
   }
-  for (int p = tid; p < L::YL; p += NTH_P) {
-    const int k = p - PADP;
-    double vd = 0.0, ve = 0.0, vf = 0.0;
-    if (k >= 0 && k < nen) {
-      vd = a.Gg[2 * (eji + ES * k) + 1];
-      ve = a.Gl[2 * (eji + ES * k) + 1];
-      vf = a.Gr[2 * (eij + ES * k) + 1];
+  {
+    constexpr int NI = (L::YL + NTH_P - 1) / NTH_P;
+    double vd[NI], ve[NI];
+#pragma unroll
+    for (int u = 0; u < NI; ++u) {
+      const int k = tid + u * NTH_P - PADP;
+      vd[u] = ve[u] = 0.0;
+      if (k >= 0 && k < nen) {
+        vd[u] = a.Gg[2 * (eji + ES * k) + 1];
+        ve[u] = a.Gl[2 * (eji + ES * k) + 1];
+      }
     }
-    const int q = dpos<RP>(p, Q);
-    yd[q] = vd;
-    ye[q] = ve;
-    yf[q] = vf;
+#pragma unroll
+    for (int u = 0; u < NI; ++u) {
+      const int p = tid + u * NTH_P;
+      if (p < L::YL) {
+        const int q = dpos<RP>(p, Q);
+        yd[q] = vd[u];
+        ye[q] = ve[u];
+      }
+    }
+  }
+  __syncthreads();
+  for (int p = tid; p < L::YL; p += NTH_P) {  // the shifted copy of g^r_ij comes from LDS
+    const int k = p - PADP;
+    yf[dpos<RP>(p, Q)] = (k >= 0 && k < nen) ? xc[k] : 0.0;
   }
   __syncthreads();
   const int nout = 2 * a.nopmax + 1;
@@ -227,30 +263,53 @@ __global__ __launch_bounds__(NTH_S) void k_selfenergy(GwArgs a) {
   constexpr int Q = L::YL / 4;
   const int nout = 2 * a.nopmax + 1;
   const int lo_iop = a.half - 1 - a.nopmax;
-  for (int p = tid; p < L::WL; p += NTH_S) {
-    double2 vl = make_double2(0, 0), vg = vl, vr = vl;
-    if (p < nout) {
-      const size_t o = eij + ES * (size_t)(lo_iop + p);
-      vl = reinterpret_cast<const double2*>(a.Wl)[o];
-      vg = reinterpret_cast<const double2*>(a.Wg)[o];
-      vr = reinterpret_cast<const double2*>(a.Wr)[o];
+  {  // staging: every load of a thread is issued before its LDS stores
+    constexpr int NI = (L::WL + NTH_S - 1) / NTH_S;
+    double2 vl[NI], vg[NI], vr[NI];
+#pragma unroll
+    for (int u = 0; u < NI; ++u) {
+      const int p = tid + u * NTH_S;
+      vl[u] = vg[u] = vr[u] = make_double2(0, 0);
+      if (p < nout) {
+        const size_t o = eij + ES * (size_t)(lo_iop + p);
+        vl[u] = reinterpret_cast<const double2*>(a.Wl)[o];
+        vg[u] = reinterpret_cast<const double2*>(a.Wg)[o];
+        vr[u] = reinterpret_cast<const double2*>(a.Wr)[o];
+      }
     }
-    wl[2 * p] = vl.x; wl[2 * p + 1] = vl.y;
-    wg[2 * p] = vg.x; wg[2 * p + 1] = vg.y;
-    wr[2 * p] = vr.x; wr[2 * p + 1] = vr.y;
+#pragma unroll
+    for (int u = 0; u < NI; ++u) {
+      const int p = tid + u * NTH_S;
+      if (p < L::WL) {
+        wl[2 * p] = vl[u].x; wl[2 * p + 1] = vl[u].y;
+        wg[2 * p] = vg[u].x; wg[2 * p + 1] = vg[u].y;
+        wr[2 * p] = vr[u].x; wr[2 * p + 1] = vr[u].y;
+      }
+    }
   }
-  for (int p = tid; p < L::YL; p += NTH_S) {
-    const int k = p - PADY;
-    double vl = 0.0, vg = 0.0, vr = 0.0;
-    if (k >= 0 && k < nen - 1) {  // the last energy of G never enters
-      vl = a.Gl[2 * (eij + ES * k) + 1];
-      vg = a.Gg[2 * (eij + ES * k) + 1];
-      vr = a.Gr[2 * (eij + ES * k) + 1];
+  {
+    constexpr int NI = (L::YL + NTH_S - 1) / NTH_S;
+    double vl[NI], vg[NI], vr[NI];
+#pragma unroll
+    for (int u = 0; u < NI; ++u) {
+      const int k = tid + u * NTH_S - PADY;
+      vl[u] = vg[u] = vr[u] = 0.0;
+      if (k >= 0 && k < nen - 1) {  // the last energy of G never enters
+        vl[u] = a.Gl[2 * (eij + ES * k) + 1];
+        vg[u] = a.Gg[2 * (eij + ES * k) + 1];
+        vr[u] = a.Gr[2 * (eij + ES * k) + 1];
+      }
     }
-    const int q = dpos<RS>(p, Q);
-    yl[q] = vl;
-    yg[q] = vg;
-    yr[q] = vr;
+#pragma unroll
+    for (int u = 0; u < NI; ++u) {
+      const int p = tid + u * NTH_S;
+      if (p < L::YL) {
+        const int q = dpos<RS>(p, Q);
+        yl[q] = vl[u];
+        yg[q] = vg[u];
+        yr[q] = vr[u];
+      }
+    }
   }
   __syncthreads();
   const int wave = tid >> 6, lane = tid & 63;
@@ -380,7 +439,7 @@ int check(const gw_params* p, int min_nx) {
   if (p->nopmax < 0 || p->nopmax > p->nen / 2 - 1)
     return fail(1, "gw: need 0 <= nopmax <= nen/2 - 1 (frequency index nop + nen/2 inside 1..nen)");
   const long long band = 2LL * std::min(p->ndiag, p->nm - 1) + 1;
-  if ((long long)p->nm * band * p->nx > 2147483647LL) return fail(4, "gw: too many band elements for one launch");
+  if ((long long)p->nm * band * p->nx > 2147483640LL) return fail(4, "gw: too many band elements for one launch");
   return 0;
 }
 
@@ -395,6 +454,8 @@ GwArgs args(const gw_params* p) {
   a.alpha = p->alpha_mix;
   return a;
 }
+
+unsigned grid8(long long nel) { return (unsigned)((nel + 7) / 8 * 8); }
 
 int offband_grid(long long n) { return (int)std::max(1LL, std::min<long long>((n + 255) / 256, 8192)); }
 
@@ -431,16 +492,17 @@ int gw_polarization(const gw_params* p, const double* Gl, const double* Gg, cons
   a.Ol = Pl; a.Og = Pg; a.Or = Pr;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   const long long nel = (long long)a.nm * (2 * a.nd + 1) * a.nx;
+  a.nel = nel;
   if (a.nd < a.nm - 1) {
     hipLaunchKernelGGL(k_pol_offband, dim3(offband_grid((long long)a.nm * a.nm * a.nx * a.nen)), dim3(256), 0, s, a);
     HIPCHK(hipGetLastError());
   }
   if (a.nen <= 1024)
-    hipLaunchKernelGGL(k_polarization<1024>, dim3((unsigned)nel), dim3(NTH_P), 0, s, a);
+    hipLaunchKernelGGL(k_polarization<1024>, dim3(grid8(nel)), dim3(NTH_P), 0, s, a);
   else if (a.nen <= 2048)
-    hipLaunchKernelGGL(k_polarization<2048>, dim3((unsigned)nel), dim3(NTH_P), 0, s, a);
+    hipLaunchKernelGGL(k_polarization<2048>, dim3(grid8(nel)), dim3(NTH_P), 0, s, a);
   else
-    hipLaunchKernelGGL(k_polarization<2560>, dim3((unsigned)nel), dim3(NTH_P), 0, s, a);
+    hipLaunchKernelGGL(k_polarization<2560>, dim3(grid8(nel)), dim3(NTH_P), 0, s, a);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -460,15 +522,16 @@ int gw_selfenergy(const gw_params* p, const double* Gl, const double* Gg, const 
   a.Ol = Sl; a.Og = Sg; a.Or = Sr;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   const long long nel = (long long)a.nm * (2 * a.nd + 1) * (a.nx - 4);
+  a.nel = nel;
   if (a.nd < a.nm - 1) {
     hipLaunchKernelGGL(k_sig_offband, dim3(offband_grid((long long)a.nm * a.nm * (a.nx - 4) * a.nen)), dim3(256), 0,
                        s, a);
     HIPCHK(hipGetLastError());
   }
   if (a.nen <= 1024)
-    hipLaunchKernelGGL(k_selfenergy<1024>, dim3((unsigned)nel), dim3(NTH_S), 0, s, a);
+    hipLaunchKernelGGL(k_selfenergy<1024>, dim3(grid8(nel)), dim3(NTH_S), 0, s, a);
   else
-    hipLaunchKernelGGL(k_selfenergy<2040>, dim3((unsigned)nel), dim3(NTH_S), 0, s, a);
+    hipLaunchKernelGGL(k_selfenergy<2040>, dim3(grid8(nel)), dim3(NTH_S), 0, s, a);
   HIPCHK(hipGetLastError());
   return 0;
 }
