@@ -141,3 +141,5 @@ def test_example_sized_step_on_sampled_elements():
         assert np.linalg.norm(got - want) <= TOL * np.linalg.norm(want), k
     # edge copy of the last interior slab
     assert torch.equal(S[0][:, 17], S[0][:, 15]) and torch.equal(S[2][:, 16], S[2][:, 15])
+    del G, W, S
+    torch.cuda.empty_cache()  # ~85 GB back to the device for the tests that follow
