@@ -62,3 +62,23 @@ def test_pristine_tube_transmission_counts_modes():
         M = _modes(w, E, NS)
         g = rgf.rgf_rs((300.0, 300.0), E, (100.0, -100.0), Hii, H1i, p=p, offdiag=False)
         assert abs(g["tr"] - M) < 1e-4 and abs(g["tre"] - M) < 1e-4, (E, g["tr"], g["tre"], M)
+
+
+def test_engine_bond_current_equals_its_transmission():
+    """The engine's G^<_ll (physical mode, f_L = 1, f_R = 0) through the reference's bond
+    current density (liteqt_amd/writers.py, src/green_rgf.f90:4602-4640) on a tube with
+    hoppings between neighbouring cells only: every bond carries the engine's T(E)."""
+    from liteqt_amd import writers
+    from tests.test_wannier import _hermitian, _truncated
+    rgf = _rgf()
+    w = _truncated(_hermitian(wannier.load_ham_dat(os.path.join(EX, "ham_dat"))), 1)
+    Hii, H1i = wannier.device_blocks(w, NS, 5)
+    p = rgf.params(rgf.MODE_PHYSICAL, eta_dev=0.0, eta_lead=1e-7, dec_tol=1e-14, dec_nmax=300)
+    E = np.array([-9.3, -4.4, -1.7])
+    g = rgf.rgf_rs_energies(p, (300.0, 300.0), E, (100.0, -100.0), Hii, H1i, outputs=("ndens",))
+    tot, etot, jd = writers.calc_block_current(Hii, g["ndens"], E, 1.0, w.nb, NS)
+    scale = (E[1] - E[0]) * writers.E0 / writers.TPI / writers.HBAR * writers.E0
+    for k in range(len(E)):
+        bonds = jd[:, :, :, k].sum(axis=(0, 1)) / scale
+        assert np.abs(bonds - g["tr"][k]).max() < 1e-8, (E[k], g["tr"][k], bonds)
+        assert abs(g["tr"][k] - _modes(w, E[k], 1)) < 1e-6

@@ -1,0 +1,103 @@
+"""Output files of the RGF drivers (liteqt_amd/writers.py): Fortran E18.4 editing against
+what ROCm's flang prints for the same values, the file names and line layout of the
+spectrum / current / transmission writers, and the bond current density against the
+Landauer transmission (with hoppings between neighbouring cells only, the current through
+every bond of a coherent device equals T(E) (f_L - f_R))."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from liteqt_amd import wannier, writers
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FLANG = os.environ.get("FC", "/opt/rocm/lib/llvm/bin/flang")
+
+
+def _values():
+    rng = np.random.default_rng(11)
+    v = [0.0, -0.0, 1.0, -1.0, 1.0625, -1.0625, 1.1875, 9.99995, 0.99995, 100.625, 1e-120, -3.5e150, 2.5e-5,
+         12345.6789, 0.15, 0.25, 0.35, 1e99, 9.9996e98, 1e-99, 1e-100, 5e-324]
+    v += list(rng.standard_normal(40) * 10.0 ** rng.integers(-30, 30, 40))
+    return v
+
+
+def test_fortran_e_matches_flang(tmp_path):
+    if not os.path.exists(FLANG):
+        pytest.skip("no Fortran compiler (ROCm flang) in this image")
+    vals = _values()
+    (tmp_path / "v.txt").write_text("\n".join(repr(float(x)) for x in vals) + "\n")
+    (tmp_path / "p.f90").write_text(
+        "program p\n  real(8) :: v(%d)\n  open(10, file='v.txt')\n  read(10, *) v\n"
+        "  write(*, '(2E18.4)') v\nend program\n" % len(vals))
+    subprocess.run([FLANG, "-O0", "p.f90", "-o", "p"], cwd=tmp_path, check=True, capture_output=True)
+    out = subprocess.run(["./p"], cwd=tmp_path, check=True, capture_output=True, text=True).stdout
+    want = out.splitlines()
+    got = "".join(writers._line(vals[k:k + 2]) for k in range(0, len(vals), 2)).splitlines()
+    assert got == want
+
+
+def test_spectrum_file_layout(tmp_path):
+    rng = np.random.default_rng(2)
+    nb, ns, length, nen = 2, 3, 2, 3
+    G = rng.standard_normal((nb * ns, nb * ns, length, nen)) + 1j * rng.standard_normal((nb * ns, nb * ns, length, nen))
+    en = np.array([-1.0, -0.5, 0.25])
+    ds = str(tmp_path / "ldos")
+    writers.write_spectrum(ds, 7, G, en, nb, ns, 0.4, (1.0, -2.0))
+    lines = open(ds + "0007.dat").read().split("\n")
+    assert len(lines) == nen * (length * ns + 1) + 1 and lines[-1] == ""
+    n = 0
+    for ie in range(nen):
+        for j in range(length):
+            for k in range(ns):
+                blk = G[k * nb:(k + 1) * nb, k * nb:(k + 1) * nb, j, ie]
+                t = np.trace(blk)
+                want = writers._line((j * 0.4 * ns + k * 0.4, en[ie], t.real, -2.0 * t.imag)).rstrip("\n")
+                assert lines[n] == want and len(lines[n]) == 72
+                n += 1
+        assert lines[n] == ""
+        n += 1
+    writers.write_spectrum(ds, 7, G, en, nb, ns, 0.4, (1.0, -2.0), append=True)
+    assert len(open(ds + "0007.dat").read().split("\n")) == 2 * nen * (length * ns + 1) + 1
+
+
+def test_current_and_transmission_files(tmp_path):
+    rng = np.random.default_rng(3)
+    cur = rng.standard_normal((2, 2, 5, 4))
+    en = np.linspace(-1, 1, 4)
+    writers.write_current_spectrum(str(tmp_path / "Jdens"), 12, cur, en, 0.5)
+    lines = open(tmp_path / "Jdens_0012.dat").read().split("\n")
+    assert lines[0] == writers._line((0.0, en[0], cur[:, :, 0, 0].sum())).rstrip("\n")
+    assert lines[5] == "" and len(lines) == 4 * 6 + 1
+    tr = rng.standard_normal(4)
+    writers.write_transmission_spectrum(str(tmp_path / "trR"), 0, tr, en)
+    lines = open(tmp_path / "trR_0000.dat").read().splitlines()
+    assert lines == [writers._line((en[k], tr[k])).rstrip("\n") for k in range(4)]
+    with pytest.raises(ValueError):
+        writers.string4(10000)
+
+
+def test_bond_current_equals_the_transmission():
+    """Hermitian hoppings between neighbouring cells only (the reference's bond current
+    takes the term between cells ib and ib+1), no potential, f_L = 1, f_R = 0: the current
+    through every bond of every slab is T(E), the number of propagating modes."""
+    from oracle import pyoracle as orc
+    from tests.test_wannier import _hermitian, _truncated
+    from tests.test_gpu_wannier import _modes
+    ns = 4
+    w = _truncated(_hermitian(wannier.load_ham_dat(os.path.join(HERE, "golden", "cnt_8-0", "ham_dat"))), 1)
+    Hii, H1i = wannier.device_blocks(w, ns, 3)
+    po = orc.params(orc.PHYSICAL, eta_dev=0.0, eta_lead=1e-7, dec_tol=1e-14, dec_nmax=300)
+    Es = (-4.4, -1.7)
+    outs = [orc.rgf(po, (300.0, 300.0), E, (100.0, -100.0), Hii, H1i, offdiag=False) for E in Es]
+    Gl = np.stack([o["ndens"] for o in outs], axis=-1)
+    en = np.array([Es[0], Es[0] + 0.01])  # only E2 - E1 enters the scale
+    tot, etot, jd = writers.calc_block_current(Hii, Gl, en, 2.0, w.nb, ns)
+    scale = 0.01 * writers.E0 / writers.TPI / writers.HBAR * writers.E0 * 2.0
+    for k, E in enumerate(Es):
+        M = _modes(w, E, 1, nk=1001)
+        bonds = jd[:, :, :, k].sum(axis=(0, 1)) / scale
+        assert np.abs(bonds - M).max() < 1e-9, (E, M, bonds)
+    assert np.allclose(tot, jd.sum(axis=3), rtol=0, atol=1e-30)
+    assert np.allclose(etot, jd[..., 0] * en[0] + jd[..., 1] * en[1], rtol=1e-14, atol=0)
