@@ -50,18 +50,37 @@ def fortran_e(x, w=18, d=4):
     return s.rjust(w)
 
 
+def parse_fortran_e(field):
+    """Inverse of fortran_e for reading the files back (`0.1234-101` has no `E`)."""
+    t = field.strip()
+    k = max(t.rfind("+"), t.rfind("-"))
+    if k > 0 and t[k - 1] not in "Ee":
+        t = t[:k] + "E" + t[k:]
+    return float(t)
+
+
 def _line(vals, w=18, d=4):
     return "".join(fortran_e(v, w, d) for v in vals) + "\n"
 
 
-def write_spectrum(dataset, i, G, en, nb, ns, lx, coeff, append=False):
-    """G: complex (nb*ns, nb*ns, length, nen)."""
+def spectrum_traces(G, nb, ns):
+    """Traces of the nb x nb diagonal blocks of each cell: complex (ns, length, nen) from
+    G (nb*ns, nb*ns, length, nen)."""
     G = np.asarray(G)
     nm, _, length, nen = G.shape
     if nm != nb * ns:
         raise ValueError("G's block size must be nb*ns")
-    diag = np.einsum("iijk->ijk", G).reshape(ns, nb, length, nen)  # (cell k, orbital, slab, E)
-    tr = diag.sum(axis=1)                                            # (k, j, E)
+    return np.einsum("iijk->ijk", G).reshape(ns, nb, length, nen).sum(axis=1)
+
+
+def write_spectrum(dataset, i, G, en, nb, ns, lx, coeff, append=False):
+    """G: complex (nb*ns, nb*ns, length, nen)."""
+    write_spectrum_traces(dataset, i, spectrum_traces(G, nb, ns), en, lx, coeff, append)
+
+
+def write_spectrum_traces(dataset, i, tr, en, lx, coeff, append=False):
+    """write_spectrum from the cell traces (ns, length, nen) of spectrum_traces."""
+    ns, length, nen = tr.shape
     with open(f"{dataset}{string4(i)}.dat", "a" if append else "w") as f:
         for ie in range(nen):
             for j in range(length):
@@ -73,9 +92,12 @@ def write_spectrum(dataset, i, G, en, nb, ns, lx, coeff, append=False):
 
 def write_current_spectrum(dataset, i, cur, en, lx):
     """cur: real (nb, nb, length, nen) bond current densities."""
-    cur = np.asarray(cur)
-    _, _, length, nen = cur.shape
-    tot = cur.sum(axis=(0, 1))
+    write_current_totals(dataset, i, np.asarray(cur).sum(axis=(0, 1)), en, lx)
+
+
+def write_current_totals(dataset, i, tot, en, lx):
+    """write_current_spectrum from the per-bond sums tot (length, nen)."""
+    length, nen = tot.shape
     with open(f"{dataset}_{string4(i)}.dat", "w") as f:
         for ie in range(nen):
             for j in range(length):
@@ -89,15 +111,16 @@ def write_transmission_spectrum(dataset, i, tr, en):
             f.write(_line((en[ie], float(np.real(tr[ie])))))
 
 
-def calc_block_current(H, G_lesser, en, spindeg, nb, ns):
+def calc_block_current(H, G_lesser, en, spindeg, nb, ns, dE=None):
     """(tot_cur, tot_ecur, jdens): real (nb, nb, nx*ns), (nb, nb, nx*ns), (nb, nb, nx*ns, nen)
-    from H (nm, nm, nx) and G^< (nm, nm, nx, nen), nm = nb*ns, ns >= 2."""
+    from H (nm, nm, nx) and G^< (nm, nm, nx, nen), nm = nb*ns, ns >= 2.  dE: the grid
+    spacing (default en[1] - en[0]; give it when en is a chunk of the grid)."""
     H = np.asarray(H)
     Gl = np.asarray(G_lesser)
     nm, _, nx, nen = Gl.shape
     if nm != nb * ns or ns < 2:
         raise ValueError("need nm = nb*ns and ns >= 2 (the reference leaves the bond undefined for ns = 1)")
-    scale = (en[1] - en[0]) * E0 / TPI / HBAR * E0 * float(spindeg)
+    scale = ((en[1] - en[0]) if dE is None else dE) * E0 / TPI / HBAR * E0 * float(spindeg)
     jdens = np.zeros((nb, nb, nx * ns, nen))
     for ix in range(nx):
         for ib in range(ns - 1):

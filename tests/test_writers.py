@@ -36,6 +36,9 @@ def test_fortran_e_matches_flang(tmp_path):
     want = out.splitlines()
     got = "".join(writers._line(vals[k:k + 2]) for k in range(0, len(vals), 2)).splitlines()
     assert got == want
+    back = [writers.parse_fortran_e(ln[k:k + 18]) for ln in want for k in (0, 18)]
+    for x, y in zip(vals, back):  # the files read back to 4 digits
+        assert abs(y - x) <= 5e-4 * abs(x) or abs(x) < 1e-300
 
 
 def test_spectrum_file_layout(tmp_path):
