@@ -391,35 +391,41 @@ __global__ __launch_bounds__(NTH_S) void k_selfenergy(GwArgs a) {
   }
 }
 
-// P outside the band (|i - j| > nd): zero.
+// P outside the band (|i - j| > nd): zero.  blockIdx.x covers the nm^2 pairs of a slab
+// (32-bit index math), blockIdx.y strides over the nx * nen (slab, energy) rows.
 __global__ __launch_bounds__(256) void k_pol_offband(GwArgs a) {
-  const long long n = (long long)a.nm * a.nm * a.nx * a.nen;
-  for (long long idx = blockIdx.x * 256LL + threadIdx.x; idx < n; idx += (long long)gridDim.x * 256) {
-    const int i = (int)(idx % a.nm), j = (int)((idx / a.nm) % a.nm);
-    if (abs(i - j) <= a.nd) continue;
-    reinterpret_cast<double2*>(a.Ol)[idx] = make_double2(0, 0);
-    reinterpret_cast<double2*>(a.Og)[idx] = make_double2(0, 0);
-    reinterpret_cast<double2*>(a.Or)[idx] = make_double2(0, 0);
+  const int nn = a.nm * a.nm;
+  const int ij = blockIdx.x * 256 + threadIdx.x;
+  if (ij >= nn) return;
+  const int i = ij % a.nm, j = ij / a.nm;
+  if (abs(i - j) <= a.nd) return;
+  const long long rows = (long long)a.nx * a.nen;
+  for (long long r = blockIdx.y; r < rows; r += gridDim.y) {
+    const size_t o = (size_t)r * nn + ij;
+    reinterpret_cast<double2*>(a.Ol)[o] = make_double2(0, 0);
+    reinterpret_cast<double2*>(a.Og)[o] = make_double2(0, 0);
+    reinterpret_cast<double2*>(a.Or)[o] = make_double2(0, 0);
   }
 }
 
 // Sigma outside the band: Sigma_new = 0, so Sigma_gw += alpha (0 - Sigma_gw), on slabs
-// 3..nx-2 (1-based), with the same edge copy.
+// 3..nx-2 (1-based), with the same edge copy.  Rows are (energy, interior slab) pairs.
 __global__ __launch_bounds__(256) void k_sig_offband(GwArgs a) {
-  const size_t SS = (size_t)a.nm * a.nm;
+  const int nn = a.nm * a.nm;
+  const int ij = blockIdx.x * 256 + threadIdx.x;
+  if (ij >= nn) return;
+  const int i = ij % a.nm, j = ij / a.nm;
+  if (abs(i - j) <= a.nd) return;
+  const size_t SS = (size_t)nn;
   const int ns = a.nx - 4;
-  const long long n = (long long)SS * ns * a.nen;
   const int last = a.nx - 3;
-  for (long long idx = blockIdx.x * 256LL + threadIdx.x; idx < n; idx += (long long)gridDim.x * 256) {
-    const long long ij = idx % (long long)SS;
-    const long long r = idx / (long long)SS;
+  const long long rows = (long long)ns * a.nen;
+  double2* S[3] = {reinterpret_cast<double2*>(a.Ol), reinterpret_cast<double2*>(a.Og),
+                   reinterpret_cast<double2*>(a.Or)};
+  for (long long r = blockIdx.y; r < rows; r += gridDim.y) {
     const int ix = 2 + (int)(r % ns);
     const long long e = r / ns;
-    const int i = (int)(ij % a.nm), j = (int)(ij / a.nm);
-    if (abs(i - j) <= a.nd) continue;
     const size_t o = (size_t)ij + SS * ix + SS * a.nx * (size_t)e;
-    double2* S[3] = {reinterpret_cast<double2*>(a.Ol), reinterpret_cast<double2*>(a.Og),
-                     reinterpret_cast<double2*>(a.Or)};
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       const double2 v = S[k][o];
@@ -436,6 +442,7 @@ int check(const gw_params* p, int min_nx) {
   if (p->nm < 1 || p->nx < min_nx || p->nen < 2)
     return fail(1, "gw: need nm >= 1, nx >= " + std::to_string(min_nx) + ", nen >= 2");
   if (p->ndiag < 0) return fail(1, "gw: ndiag must be >= 0");
+  if (p->nm > 4096) return fail(4, "gw: nm > 4096");
   if (p->nopmax < 0 || p->nopmax > p->nen / 2 - 1)
     return fail(1, "gw: need 0 <= nopmax <= nen/2 - 1 (frequency index nop + nen/2 inside 1..nen)");
   const long long band = 2LL * std::min(p->ndiag, p->nm - 1) + 1;
@@ -457,7 +464,12 @@ GwArgs args(const gw_params* p) {
 
 unsigned grid8(long long nel) { return (unsigned)((nel + 7) / 8 * 8); }
 
-int offband_grid(long long n) { return (int)std::max(1LL, std::min<long long>((n + 255) / 256, 8192)); }
+// (pair blocks, row stride) of the off-band kernels: ~16k workgroups in flight
+dim3 offband_grid(int nm, long long rows) {
+  const unsigned gx = (unsigned)((nm * nm + 255) / 256);
+  const long long gy = std::max(1LL, std::min<long long>(rows, std::max(1LL, 16384LL / gx)));
+  return dim3(gx, (unsigned)std::min<long long>(gy, 65535));
+}
 
 }  // namespace
 
@@ -494,7 +506,7 @@ int gw_polarization(const gw_params* p, const double* Gl, const double* Gg, cons
   const long long nel = (long long)a.nm * (2 * a.nd + 1) * a.nx;
   a.nel = nel;
   if (a.nd < a.nm - 1) {
-    hipLaunchKernelGGL(k_pol_offband, dim3(offband_grid((long long)a.nm * a.nm * a.nx * a.nen)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_pol_offband, offband_grid(a.nm, (long long)a.nx * a.nen), dim3(256), 0, s, a);
     HIPCHK(hipGetLastError());
   }
   if (a.nen <= 1024)
@@ -524,8 +536,7 @@ int gw_selfenergy(const gw_params* p, const double* Gl, const double* Gg, const 
   const long long nel = (long long)a.nm * (2 * a.nd + 1) * (a.nx - 4);
   a.nel = nel;
   if (a.nd < a.nm - 1) {
-    hipLaunchKernelGGL(k_sig_offband, dim3(offband_grid((long long)a.nm * a.nm * (a.nx - 4) * a.nen)), dim3(256), 0,
-                       s, a);
+    hipLaunchKernelGGL(k_sig_offband, offband_grid(a.nm, (long long)(a.nx - 4) * a.nen), dim3(256), 0, s, a);
     HIPCHK(hipGetLastError());
   }
   if (a.nen <= 1024)
