@@ -23,21 +23,36 @@ def _rand(rng, shape):
     return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
 
 
-def _close(got, ref, tag):
+def _close(got, ref, tag, refs=None):
+    """Per slab err <= TOL * (largest norm among the step's outputs on that slab): P^r
+    vanishes identically on the diagonal at nop = 0 (a c - c a per term), where the GPU's
+    two FMAs leave rounding noise of that scale."""
+    refs = [ref] if refs is None else refs
     for ix in range(ref.shape[2]):
         err = np.linalg.norm(got[:, :, ix] - ref[:, :, ix])
-        assert err <= TOL * np.linalg.norm(ref[:, :, ix]), (tag, ix, err)
+        scale = max(np.linalg.norm(r[:, :, ix]) for r in refs)
+        assert err <= TOL * scale, (tag, ix, err)
+
+
+def _nopmax(nen, wide):
+    """nen/2 - 10 (the 1D driver), nen/2 - 1 (the widest window) or a given value."""
+    if wide is True:
+        return nen // 2 - 1
+    if wide is False:
+        return nen // 2 - 10
+    return wide
 
 
 @pytest.mark.parametrize("nm,nx,nen,ndiag,wide", [(6, 3, 64, 1, False), (5, 2, 301, 9, True), (4, 2, 1100, 1, False),
-                                                  (3, 1, 2048, 2, True), (2, 1, 2560, 1, False)])
+                                                  (3, 1, 2048, 2, True), (2, 1, 2560, 1, False),
+                                                  (4, 2, 33, 0, 0), (3, 2, 2, 1, 0), (5, 1, 7, 1, True)])
 def test_polarization_matches_oracle(nm, nx, nen, ndiag, wide):
     import torch
     gw = _gw()
     print(f"polarization nm={nm} nx={nx} nen={nen} ndiag={ndiag}", flush=True)
     rng = np.random.default_rng(nen + nm)
     G = [_rand(rng, (nm, nm, nx, nen)) for _ in range(3)]
-    nopmax = nen // 2 - 1 if wide else nen // 2 - 10
+    nopmax = _nopmax(nen, wide)
     dE_P = complex(0.0, -0.013)
     p = gw.params(nm, nx, nen, ndiag, nopmax, dE_P, 0.0, 0.0)
     Gd = [gw.to_device(g) for g in G]
@@ -50,13 +65,15 @@ def test_polarization_matches_oracle(nm, nx, nen, ndiag, wide):
     win[np.arange(-nopmax, nopmax + 1) + nen // 2 - 1] = True
     for name, d, r in zip(("P<", "P>", "Pr"), Pd, ref):
         got = gw.to_host(d)
-        _close(got, r, name)
+        _close(got, r, name, ref)
         assert np.all(got[~m] == 0), name
         assert np.all(got[..., ~win] == 0), name
 
 
 @pytest.mark.parametrize("nm,nx,nen,ndiag,wide,alpha", [(6, 5, 64, 1, False, 0.3), (5, 7, 301, 9, True, 0.7),
-                                                        (3, 6, 1100, 1, False, 1.0), (2, 5, 2040, 1, True, 0.25)])
+                                                        (3, 6, 1100, 1, False, 1.0), (2, 5, 2040, 1, True, 0.25),
+                                                        (3, 5, 33, 0, 0, 0.5), (2, 5, 4, 1, 1, 0.5),
+                                                        (4, 9, 9, 2, True, 0.0)])
 def test_selfenergy_matches_oracle(nm, nx, nen, ndiag, wide, alpha):
     gw = _gw()
     print(f"selfenergy nm={nm} nx={nx} nen={nen} ndiag={ndiag}", flush=True)
@@ -65,7 +82,7 @@ def test_selfenergy_matches_oracle(nm, nx, nen, ndiag, wide, alpha):
     G = [_rand(rng, (nm, nm, nx, nen)) for _ in range(3)]
     W = [_rand(rng, (nm, nm, nx, nen)) for _ in range(3)]
     S = [_rand(rng, (nm, nm, nx, nen)) for _ in range(3)]
-    nopmax = nen // 2 - 1 if wide else nen // 2 - 10
+    nopmax = _nopmax(nen, wide)
     dE_S = complex(0.0, 0.021)
     p = gw.params(nm, nx, nen, ndiag, nopmax, 0.0, dE_S, alpha)
     Sd = [gw.to_device(s) for s in S]
@@ -73,7 +90,7 @@ def test_selfenergy_matches_oracle(nm, nx, nen, ndiag, wide, alpha):
     torch.cuda.synchronize()
     ref = orc.selfenergy_step(*G, *W, *S, ndiag, nopmax, dE_S, alpha)
     for name, d, r in zip(("S<", "S>", "Sr"), Sd, ref):
-        _close(gw.to_host(d), r, name)
+        _close(gw.to_host(d), r, name, ref)
 
 
 def test_example_sized_step_on_sampled_elements():
