@@ -51,7 +51,8 @@ constexpr int PADY = 260;  // zero padding of a self-energy Y series on each sid
 struct GwArgs {
   int nm, nx, nen, nd, nopmax, half;  // nd = min(ndiag, nm - 1), half = nen / 2
   int slab0;                          // first slab of the grid
-  long long nel;                      // logical blocks (band positions x slabs)
+  long long nel;                      // logical blocks (band positions or tile slots x slabs)
+  int tiled;                          // 8 x 4 (i, j) tiles (dense bands) instead of column order
   double dr, di;                      // complex prefactor
   double alpha;
   const double* Gl;
@@ -74,11 +75,24 @@ __device__ __forceinline__ int dpos(int p, int q) { return (p % R) * q + p / R; 
 // with its own L2), so the grid (a multiple of 8) gives XCD x the contiguous logical range
 // [x * per, (x + 1) * per): the ~32 workgroups an XCD runs at once read neighbouring
 // elements and share the cache lines of their strided series.
+// Dense bands (nd >= 8) use 8 x 4 tiles of (i, j) instead, i fastest: the 32 workgroups
+// an XCD runs at once share the lines of the ij series 8 ways and, in the polarization,
+// those of the transposed ji series 4 ways.
 __device__ __forceinline__ bool element(const GwArgs& a, int& i, int& j, int& ix) {
   const int band = 2 * a.nd + 1;
   const long long per = gridDim.x / 8;
   long long b = (long long)(blockIdx.x % 8) * per + blockIdx.x / 8;
   if (b >= a.nel) return false;
+  if (a.tiled) {
+    const int ti = (a.nm + 7) / 8;
+    const int w = (int)(b % 32);
+    b /= 32;
+    const int t = (int)(b % ((long long)ti * ((a.nm + 3) / 4)));
+    ix = a.slab0 + (int)(b / ((long long)ti * ((a.nm + 3) / 4)));
+    i = (t % ti) * 8 + (w % 8);
+    j = (t / ti) * 4 + (w / 8);
+    return i < a.nm && j < a.nm && abs(i - j) <= a.nd;
+  }
   const int di = (int)(b % band);
   b /= band;
   j = (int)(b % a.nm);
@@ -446,7 +460,9 @@ int check(const gw_params* p, int min_nx) {
   if (p->nopmax < 0 || p->nopmax > p->nen / 2 - 1)
     return fail(1, "gw: need 0 <= nopmax <= nen/2 - 1 (frequency index nop + nen/2 inside 1..nen)");
   const long long band = 2LL * std::min(p->ndiag, p->nm - 1) + 1;
-  if ((long long)p->nm * band * p->nx > 2147483640LL) return fail(4, "gw: too many band elements for one launch");
+  const long long tiles = 32LL * ((p->nm + 7) / 8) * ((p->nm + 3) / 4);
+  if (std::max((long long)p->nm * band, tiles) * p->nx > 2147483640LL)
+    return fail(4, "gw: too many band elements for one launch");
   return 0;
 }
 
@@ -503,7 +519,9 @@ int gw_polarization(const gw_params* p, const double* Gl, const double* Gg, cons
   a.Gl = Gl; a.Gg = Gg; a.Gr = Gr;
   a.Ol = Pl; a.Og = Pg; a.Or = Pr;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-  const long long nel = (long long)a.nm * (2 * a.nd + 1) * a.nx;
+  a.tiled = a.nd >= 8;
+  const long long per_slab = a.tiled ? 32LL * ((a.nm + 7) / 8) * ((a.nm + 3) / 4) : (long long)a.nm * (2 * a.nd + 1);
+  const long long nel = per_slab * a.nx;
   a.nel = nel;
   if (a.nd < a.nm - 1) {
     hipLaunchKernelGGL(k_pol_offband, offband_grid(a.nm, (long long)a.nx * a.nen), dim3(256), 0, s, a);
@@ -533,7 +551,9 @@ int gw_selfenergy(const gw_params* p, const double* Gl, const double* Gg, const 
   a.Wl = Wl; a.Wg = Wg; a.Wr = Wr;
   a.Ol = Sl; a.Og = Sg; a.Or = Sr;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-  const long long nel = (long long)a.nm * (2 * a.nd + 1) * (a.nx - 4);
+  a.tiled = a.nd >= 8;
+  const long long per_slab = a.tiled ? 32LL * ((a.nm + 7) / 8) * ((a.nm + 3) / 4) : (long long)a.nm * (2 * a.nd + 1);
+  const long long nel = per_slab * (a.nx - 4);
   a.nel = nel;
   if (a.nd < a.nm - 1) {
     hipLaunchKernelGGL(k_sig_offband, offband_grid(a.nm, (long long)(a.nx - 4) * a.nen), dim3(256), 0, s, a);

@@ -45,7 +45,8 @@ def _nopmax(nen, wide):
 
 @pytest.mark.parametrize("nm,nx,nen,ndiag,wide", [(6, 3, 64, 1, False), (5, 2, 301, 9, True), (4, 2, 1100, 1, False),
                                                   (3, 1, 2048, 2, True), (2, 1, 2560, 1, False),
-                                                  (4, 2, 33, 0, 0), (3, 2, 2, 1, 0), (5, 1, 7, 1, True)])
+                                                  (4, 2, 33, 0, 0), (3, 2, 2, 1, 0), (5, 1, 7, 1, True),
+                                                  (12, 2, 200, 11, False), (20, 1, 64, 9, True)])
 def test_polarization_matches_oracle(nm, nx, nen, ndiag, wide):
     import torch
     gw = _gw()
@@ -73,7 +74,8 @@ def test_polarization_matches_oracle(nm, nx, nen, ndiag, wide):
 @pytest.mark.parametrize("nm,nx,nen,ndiag,wide,alpha", [(6, 5, 64, 1, False, 0.3), (5, 7, 301, 9, True, 0.7),
                                                         (3, 6, 1100, 1, False, 1.0), (2, 5, 2040, 1, True, 0.25),
                                                         (3, 5, 33, 0, 0, 0.5), (2, 5, 4, 1, 1, 0.5),
-                                                        (4, 9, 9, 2, True, 0.0)])
+                                                        (4, 9, 9, 2, True, 0.0), (10, 5, 100, 9, False, 0.5),
+                                                        (13, 6, 40, 8, True, 0.3)])
 def test_selfenergy_matches_oracle(nm, nx, nen, ndiag, wide, alpha):
     gw = _gw()
     print(f"selfenergy nm={nm} nx={nx} nen={nen} ndiag={ndiag}", flush=True)
