@@ -47,7 +47,7 @@ typedef struct {
 void gw_params_reference(gw_params* p, int nm, int nx, int nen, int ndiag, double dE, double spindeg,
                          double alpha_mix);
 
-/* P^<, P^>, P^r from G^<, G^>, G^r (src/green_rgf.f90:3190-3221).  nen <= 2560. */
+/* P^<, P^>, P^r from G^<, G^>, G^r (src/green_rgf.f90:3190-3221).  nen <= 3072. */
 int gw_polarization(const gw_params* p, const double* Gl, const double* Gg, const double* Gr, double* Pl,
                     double* Pg, double* Pr, void* stream);
 

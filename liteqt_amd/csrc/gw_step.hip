@@ -10,8 +10,7 @@
 // real (their imaginary parts): P^< = dE_P * (-sum a d) etc. are sums of real
 // products, Sigma sums real G times complex W.
 //
-// Lane mapping: each lane owns R consecutive outputs (8 in the polarization, 4 in the
-// self-energy) and slides an R-value register window along the shifted ("Y") series,
+// Lane mapping: each lane owns R = 4 consecutive outputs and slides an R-value register window along the shifted ("Y") series,
 // so one LDS read per series and step feeds R outputs; the other series ("X") is read
 // at the same address by the whole wave (broadcast).  The Y series are stored
 // zero-padded and R-way de-interleaved (padded index p at (p % R) * Q + p / R), so the
@@ -41,8 +40,8 @@ int fail(int code, const std::string& msg) {
     if (e_ != hipSuccess) return fail(2, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-// polarization: 4 waves x 64 lanes x 8 outputs = 2048 outputs per pass
-constexpr int NTH_P = 256, RP = 8, SP = 64 * RP;
+// polarization: 8 waves x 64 lanes x 4 outputs = 2048 outputs per pass (two waves per SIMD)
+constexpr int NTH_P = 512, RP = 4, SP = 64 * RP;
 constexpr int PADP = SP;   // a wave's window reaches SP - 1 below its first output's series start
 // self-energy: 8 waves x 64 lanes x 4 outputs = 2048 outputs per pass (two waves per SIMD)
 constexpr int NTH_S = 512, RS = 4, SS_ = 64 * RS;
@@ -194,7 +193,7 @@ __global__ __launch_bounds__(NTH_P) void k_polarization(GwArgs a) {
     }
   }
   const int wave = tid >> 6, lane = tid & 63;
-  for (int ow = SP * wave; ow < nout; ow += 4 * SP) {
+  for (int ow = SP * wave; ow < nout; ow += (NTH_P / 64) * SP) {
     const int nop_lo = ow - a.nopmax;
     const int nop_hi = min(ow + SP - 1, nout - 1) - a.nopmax;
     const int elo = max(nop_lo, 0);                 // union over the wave of the
@@ -510,7 +509,7 @@ int gw_polarization(const gw_params* p, const double* Gl, const double* Gg, cons
                     double* Pg, double* Pr, void* stream) {
   g_err.clear();
   if (int rc = check(p, 1)) return rc;
-  if (p->nen > 2560) return fail(4, "gw_polarization: nen > 2560 (the energy series of an element live in LDS)");
+  if (p->nen > 3072) return fail(4, "gw_polarization: nen > 3072 (the energy series of an element live in LDS)");
   if (!Gl || !Gg || !Gr || !Pl || !Pg || !Pr) return fail(1, "gw_polarization: null array");
   GwArgs a = args(p);
   a.slab0 = 0;
@@ -532,7 +531,7 @@ int gw_polarization(const gw_params* p, const double* Gl, const double* Gg, cons
   else if (a.nen <= 2048)
     hipLaunchKernelGGL(k_polarization<2048>, dim3(grid8(nel)), dim3(NTH_P), 0, s, a);
   else
-    hipLaunchKernelGGL(k_polarization<2560>, dim3(grid8(nel)), dim3(NTH_P), 0, s, a);
+    hipLaunchKernelGGL(k_polarization<3072>, dim3(grid8(nel)), dim3(NTH_P), 0, s, a);
   HIPCHK(hipGetLastError());
   return 0;
 }

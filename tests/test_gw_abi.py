@@ -50,7 +50,7 @@ def test_pair_counts():
 
 
 @pytest.mark.parametrize("fn,kw,msg", [
-    ("gw_polarization", dict(nen=2562), "nen > 2560"),
+    ("gw_polarization", dict(nen=3074), "nen > 3072"),
     ("gw_polarization", dict(nopmax=1000), "nopmax"),
     ("gw_polarization", dict(ndiag=-1), "ndiag"),
     ("gw_selfenergy", dict(nen=2042), "nen > 2040"),
