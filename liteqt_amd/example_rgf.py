@@ -65,8 +65,8 @@ def run(example_dir, out_dir, nen=None, chunk=250):
     for e0 in range(0, nE, chunk):
         sl = slice(e0, min(nE, e0 + chunk))
         g = rgf.rgf_rs_energies(p, cfg["temp"], en[sl], cfg["mu"], Hii, H1i, outputs=("ndens", "pdens", "ldos"))
-        # the driver keeps only the imaginary parts of G before writing (src/green_rgf.f90:3186-3188
-        # comes after the writes, so the files see the full G)
+        # the driver writes these files before it zeroes the real parts of G
+        # (src/green_rgf.f90:3181-3188), so they see the full G
         tr_r[..., sl] = writers.spectrum_traces(g["ldos"], nb, ns)
         tr_l[..., sl] = writers.spectrum_traces(g["ndens"], nb, ns)
         tr_g[..., sl] = writers.spectrum_traces(g["pdens"], nb, ns)
