@@ -21,7 +21,10 @@
  * condition (E > max(w, E_1) and E - w < E_nen: the last energy of G never enters and
  * Sigma_new(E_1) = 0).
  *
- * Calls on one stream run in order; the arrays must not overlap each other.
+ * Calls on one stream run in order; the arrays must not overlap each other.  For dense
+ * bands (ndiag >= 8) gw_polarization gathers the band's Im G series into a per-device
+ * workspace it allocates (nm (2 ndiag + 1) nx 3 nen doubles, kept until the process
+ * ends): polarization calls that overlap on one GPU (different streams) race on it.
  * Return codes are those of rgf.h (0 ok, 1 invalid argument, 2 HIP error,
  * 4 unsupported size); gw_last_error() gives the message (per thread).
  */

@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <string>
 
 #include "../../include/gw.h"
@@ -63,7 +64,47 @@ struct GwArgs {
   double* Ol;  // P or Sigma outputs
   double* Og;
   double* Or;
+  double* X;   // polarization: Im G^<, Im G^>, Im G^r of every band element, energy-contiguous
 };
+
+// Index (in series of nen doubles) of series s (0: G^<, 1: G^>, 2: G^r) of band element
+// (i, j, slab ix) in the polarization's gathered copy.
+__device__ __forceinline__ size_t xseries(const GwArgs& a, int i, int j, int ix, int s) {
+  return ((((size_t)ix * a.nm + j) * (2 * a.nd + 1) + (i - j + a.nd)) * 3 + s);
+}
+
+// Gather for the polarization: the imaginary parts of G^<, G^>, G^r of the band elements
+// into energy-contiguous series, through a 64 x 64 LDS tile (reads along i, writes along
+// the energy: both coalesced).  blockIdx.x: 64-energy tile; blockIdx.y strides over the
+// (column j, slab) pairs.
+__global__ __launch_bounds__(256) void k_pol_gather(GwArgs a) {
+  __shared__ double tile[64][65];
+  const int e0 = blockIdx.x * 64, tid = threadIdx.x;
+  const size_t ES = (size_t)a.nm * a.nm * a.nx;
+  for (long long cj = blockIdx.y; cj < (long long)a.nm * a.nx; cj += gridDim.y) {
+    const int j = (int)(cj % a.nm), ix = (int)(cj / a.nm);
+    const int lo = max(0, j - a.nd), hi = min(a.nm - 1, j + a.nd);
+    for (int r0 = lo; r0 <= hi; r0 += 64) {
+      const int nr = min(64, hi - r0 + 1);
+      for (int s = 0; s < 3; ++s) {
+        const double* G = s == 0 ? a.Gl : (s == 1 ? a.Gg : a.Gr);
+        for (int k = tid; k < 64 * 64; k += 256) {
+          const int r = k % 64, e = k / 64;
+          double v = 0.0;
+          if (r < nr && e0 + e < a.nen)
+            v = G[2 * ((size_t)(r0 + r) + (size_t)a.nm * j + (size_t)a.nm * a.nm * ix + ES * (size_t)(e0 + e)) + 1];
+          tile[r][e] = v;
+        }
+        __syncthreads();
+        for (int k = tid; k < 64 * 64; k += 256) {
+          const int e = k % 64, r = k / 64;
+          if (r < nr && e0 + e < a.nen) a.X[xseries(a, r0 + r, j, ix, s) * a.nen + e0 + e] = tile[r][e];
+        }
+        __syncthreads();
+      }
+    }
+  }
+}
 
 template <int R>
 __device__ __forceinline__ int dpos(int p, int q) { return (p % R) * q + p / R; }  // p >= 0, R a power of 2
@@ -121,6 +162,9 @@ __global__ __launch_bounds__(NTH_P) void k_polarization(GwArgs a) {
   const size_t ES = (size_t)a.nm * a.nm * a.nx;  // energy stride (complex elements)
   const size_t eij = (size_t)i + (size_t)a.nm * j + (size_t)a.nm * a.nm * ix;
   const size_t eji = (size_t)j + (size_t)a.nm * i + (size_t)a.nm * a.nm * ix;
+  // dense bands: the series of ij and ji come gathered (energy-contiguous); else strided
+  const double* xij = a.X ? a.X + xseries(a, i, j, ix, 0) * nen : nullptr;
+  const double* xji = a.X ? a.X + xseries(a, j, i, ix, 0) * nen : nullptr;
   double* xa = lds;
   double* xb = xa + L::XL;
   double* xc = xb + L::XL;
@@ -138,9 +182,15 @@ __global__ __launch_bounds__(NTH_P) void k_polarization(GwArgs a) {
       const int p = tid + u * NTH_P;
       va[u] = vb[u] = vc[u] = 0.0;
       if (p < nen) {
-        va[u] = a.Gl[2 * (eij + ES * p) + 1];
-        vb[u] = a.Gg[2 * (eij + ES * p) + 1];
-        vc[u] = a.Gr[2 * (eij + ES * p) + 1];
+        if (xij) {
+          va[u] = xij[p];
+          vb[u] = xij[nen + p];
+          vc[u] = xij[2 * nen + p];
+        } else {
+          va[u] = a.Gl[2 * (eij + ES * p) + 1];
+          vb[u] = a.Gg[2 * (eij + ES * p) + 1];
+          vc[u] = a.Gr[2 * (eij + ES * p) + 1];
+        }
       }
     }
 #pragma unroll
@@ -161,8 +211,13 @@ __global__ __launch_bounds__(NTH_P) void k_polarization(GwArgs a) {
       const int k = tid + u * NTH_P - PADP;
       vd[u] = ve[u] = 0.0;
       if (k >= 0 && k < nen) {
-        vd[u] = a.Gg[2 * (eji + ES * k) + 1];
-        ve[u] = a.Gl[2 * (eji + ES * k) + 1];
+        if (xji) {
+          vd[u] = xji[nen + k];
+          ve[u] = xji[k];
+        } else {
+          vd[u] = a.Gg[2 * (eji + ES * k) + 1];
+          ve[u] = a.Gl[2 * (eji + ES * k) + 1];
+        }
       }
     }
 #pragma unroll
@@ -450,6 +505,15 @@ __global__ __launch_bounds__(256) void k_sig_offband(GwArgs a) {
   }
 }
 
+// Per-device workspace of the polarization's gathered series (grow-only).
+struct Workspace {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+constexpr int kMaxDev = 64;
+Workspace g_ws[kMaxDev];
+std::mutex g_ws_mu;
+
 int check(const gw_params* p, int min_nx) {
   if (!p) return fail(1, "gw: params are required");
   if (p->nm < 1 || p->nx < min_nx || p->nen < 2)
@@ -522,8 +586,34 @@ int gw_polarization(const gw_params* p, const double* Gl, const double* Gg, cons
   const long long per_slab = a.tiled ? 32LL * ((a.nm + 7) / 8) * ((a.nm + 3) / 4) : (long long)a.nm * (2 * a.nd + 1);
   const long long nel = per_slab * a.nx;
   a.nel = nel;
+  // Dense bands (the tiled mapping) stage from a gathered, energy-contiguous copy of the
+  // band's Im G series: the strided reads of the dense arrays cost a cache line per value,
+  // the gather reads whole lines.  Narrow bands read the arrays directly (a gather would
+  // touch as many lines and add a pass).
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDev) return fail(1, "gw_polarization: device index out of range");
+  const size_t need = a.tiled ? (size_t)a.nm * (2 * a.nd + 1) * a.nx * 3 * (size_t)a.nen * sizeof(double) : 0;
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  Workspace& ws = g_ws[dev];
+  if (need && ws.bytes < need) {
+    if (ws.p) HIPCHK(hipFree(ws.p));  // synchronizes with the work that used it
+    ws.p = nullptr;
+    ws.bytes = 0;
+    if (hipMalloc(&ws.p, need) != hipSuccess) {
+      ws.p = nullptr;
+      return fail(3, "gw_polarization: cannot allocate the " + std::to_string(need >> 20) + " MiB workspace");
+    }
+    ws.bytes = need;
+  }
+  a.X = need ? reinterpret_cast<double*>(ws.p) : nullptr;
   if (a.nd < a.nm - 1) {
     hipLaunchKernelGGL(k_pol_offband, offband_grid(a.nm, (long long)a.nx * a.nen), dim3(256), 0, s, a);
+    HIPCHK(hipGetLastError());
+  }
+  if (a.X) {
+    const unsigned gy = (unsigned)std::min<long long>((long long)a.nm * a.nx, 65535);
+    hipLaunchKernelGGL(k_pol_gather, dim3((unsigned)((a.nen + 63) / 64), gy), dim3(256), 0, s, a);
     HIPCHK(hipGetLastError());
   }
   if (a.nen <= 1024)
