@@ -25,8 +25,8 @@
  * bands (ndiag >= 8) gw_polarization gathers the band's Im G series into a per-device
  * workspace it allocates (nm (2 ndiag + 1) nx 3 nen doubles, kept until the process
  * ends): polarization calls that overlap on one GPU (different streams) race on it.
- * Return codes are those of rgf.h (0 ok, 1 invalid argument, 2 HIP error,
- * 4 unsupported size); gw_last_error() gives the message (per thread).
+ * Return codes are those of rgf.h (0 ok, 1 invalid argument, 2 HIP error, 3 device
+ * memory exhausted, 4 unsupported size); gw_last_error() gives the message (per thread).
  */
 #ifndef LITEQT_GW_H
 #define LITEQT_GW_H
