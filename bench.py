@@ -75,29 +75,65 @@ def algorithmic_flops(nm, nx, iters):
     return float(np.sum(8.0 * nm ** 3 * (17.0 * nx + 7.0 * k)))
 
 
-def cpu_baseline(cfg, params_kw, temp, mu, threads):
-    """Time the CPU oracle (OpenMP over energies, one per thread) on `threads`
-    energies of the workload at nx = 2 and nx = 6 slabs; the cost is linear in
-    nx, so the per-energy time at the full nx is extrapolated from the two."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(cfg, params_kw, temp, mu, threads, rgf=None, p_gpu=None):
+    """Time the CPU oracle (OpenMP over energies, one per thread, every output of
+    green_RGF_RS formed) on `threads` energies spread evenly over the workload's
+    grid at nx = 2 and nx = 6 slabs of the same device; the cost is linear in nx,
+    so the per-energy time at the full nx is extrapolated from the two.  As the
+    checker of the same sample, the GPU's outputs at nx = 6 (all of them, host
+    entry point) are compared with the oracle's: worst err/allowed against the
+    1e-10 relative Frobenius bar (cancellation-formed outputs on the scale of
+    their operands) and the worst literal relative error per output."""
     from liteqt_amd import configs
     from oracle import pyoracle as orc
     p = orc.params(orc.COMPAT, **params_kw)
-    E = configs.energy_grid(cfg["emin"], cfg["emax"], cfg["nE"])[:threads]
-    ts = {}
+    Eall = configs.energy_grid(cfg["emin"], cfg["emax"], cfg["nE"])
+    E = Eall[np.linspace(0, len(Eall) - 1, threads).astype(int)]
+    ts, ref, dev6 = {}, None, None
     for nx_s in (2, 6):
         if cfg["seed"] is None:
             Hii, H1i = configs.cnt_80_ballistic(L=nx_s)
         else:
             Hii, H1i = configs.random_device(cfg["nm"], nx_s, cfg["seed"])
         t0 = time.perf_counter()
-        orc.rgf_energies(p, temp, E, mu, Hii, H1i, outputs=(), nthreads=threads)
+        r = orc.rgf_energies(p, temp, E, mu, Hii, H1i, nthreads=threads)
         ts[nx_s] = time.perf_counter() - t0
+        if nx_s == 6:
+            ref, dev6 = r, (Hii, H1i)
     per_slab = (ts[6] - ts[2]) / 4.0
     t_full = ts[2] + per_slab * (cfg["nx"] - 2)  # wall time for `threads` energies at full nx
-    return {"value": threads / t_full, "unit": "energy-points/s", "cores": threads, "kind": "port",
-            "sample": f"{threads} energies x nx in {{2,6}} slabs at nm={cfg['nm']} "
-                      f"(one energy per thread, {ts[2]:.1f}s + {ts[6]:.1f}s wall), linear in nx "
-                      f"extrapolated to nx={cfg['nx']}"}
+    res = {"value": threads / t_full, "unit": "energy-points/s", "cores": threads, "kind": "port",
+           "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
+           "sample": f"{threads} energies spread over the grid x nx in {{2,6}} slabs at nm={cfg['nm']} "
+                     f"(one energy per thread, all outputs, {ts[2]:.1f}s + {ts[6]:.1f}s wall), linear in nx "
+                     f"extrapolated to nx={cfg['nx']}"}
+    if rgf is not None:
+        from tests import parity
+        Hii, H1i = dev6
+        g = rgf.rgf_rs_energies(p_gpu, temp, E, mu, Hii, H1i)
+        worst, lit = 0.0, {}
+        for e in range(len(E)):
+            ge, re_ = parity.energy_slice(g, e), parity.energy_slice(ref, e)
+            for name, l, err, allowed in parity.block_errors(ge, re_, H1i, "compat"):
+                worst = max(worst, err / allowed if allowed > 0 else float(err > 0))
+                if name in ("tr", "tre"):
+                    rel = err / max(abs(re_[name]), 1e-300)
+                else:
+                    rel = err / max(float(np.linalg.norm(re_[name][..., l])), 1e-300)
+                lit[name] = max(lit.get(name, 0.0), rel)
+        res["parity_sample"] = {"energies": len(E), "nx": 6, "worst_err_over_allowed": worst,
+                                "literal_relerr": lit, "bar": "1e-10 relative Frobenius per block"}
+    return res
 
 
 def adjoint_cost(rgf, torch, dev, stream, nE):
@@ -285,7 +321,7 @@ def devices_cost(rgf, torch, dev, ndev, e_div, compare, seed=3):
         torch.cuda.synchronize()
         t_seq = time.perf_counter() - t0
         print(f"bench: {ndev} devices one launch each {t_seq:.2f}s", file=sys.stderr, flush=True)
-        same = all(torch.equal(a, d["out"]["tr"]) for a, d in zip(tr_one, devs))
+        same = bool(all(torch.equal(a, d["out"]["tr"]) for a, d in zip(tr_one, devs)))
     rgf.lib().rgf_release()
     del devs
     torch.cuda.empty_cache()
@@ -296,7 +332,7 @@ def devices_cost(rgf, torch, dev, ndev, e_div, compare, seed=3):
             "devices": ndev, "energy_points": items, "s_one_launch": t_one, "s_first_call_with_autotune": t_first,
             "energy_points_per_s": items / t_one, "achieved_tflops": achieved, "frac": achieved / PEAK_FP64_MFMA,
             "s_one_launch_per_device": t_seq, "one_launch_over_per_device": (t_seq / t_one) if t_seq else None,
-            "tr_identical_to_per_device_launches": bool(same), "slots": rep["slots"],
+            "tr_identical_to_per_device_launches": same, "slots": rep["slots"],
             "workspace_gib": rep["ws_bytes"] / 2 ** 30,
             "tile_T_by_nm": {str(nm): int(tiles[[s[0] for s in shapes].index(nm)]) for nm in sizes}}
 
@@ -517,7 +553,7 @@ def main():
         if tinfo and tinfo.get("kernel_hash") == kernel_source_hash():
             traffic = tinfo["bytes_per_energy"] * nE  # HBM bytes per launch, from the PMC profile
     res = {
-        "metric": "batched RGF energy-pts/s (complex FP64, err<=1e-10 vs CPU oracle)",
+        "metric": "batched RGF energy-pts/s vs plain per-energy GPU; adjoint/forward cost; err\u22641e-10",
         "value": value,
         "unit": "energy-points/s",
         "n_gpus": 1,
@@ -546,7 +582,7 @@ def main():
     }
     if not args.no_cpu:
         threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count())
-        res["cpu_baseline"] = cpu_baseline(cfg, pkw, temp, mu, threads)
+        res["cpu_baseline"] = cpu_baseline(cfg, pkw, temp, mu, threads, rgf, p)
         res["gpu_over_cpu"] = value / res["cpu_baseline"]["value"]
     print(json.dumps(res))
 

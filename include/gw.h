@@ -23,8 +23,10 @@
  *
  * Calls on one stream run in order; the arrays must not overlap each other.  For dense
  * bands (ndiag >= 8) gw_polarization gathers the band's Im G series into a per-device
- * workspace it allocates (nm (2 ndiag + 1) nx 3 nen doubles, kept until the process
- * ends): polarization calls that overlap on one GPU (different streams) race on it.
+ * workspace it allocates (nm min(2 ndiag + 1, nm) nx 3 nen doubles, kept until
+ * gw_release or the end of the process): polarization calls that overlap on one GPU
+ * (different streams) race on it.  When that allocation fails the call still runs,
+ * staging from the arrays directly.
  * Return codes are those of rgf.h (0 ok, 1 invalid argument, 2 HIP error, 3 device
  * memory exhausted, 4 unsupported size); gw_last_error() gives the message (per thread).
  */
@@ -64,6 +66,9 @@ long long gw_pairs_polarization(int nen, int nopmax);
 long long gw_pairs_selfenergy(int nen, int nopmax);
 
 const char* gw_last_error(void);
+
+/* Free the polarization's gather workspace on the current device. */
+void gw_release(void);
 
 #ifdef __cplusplus
 }

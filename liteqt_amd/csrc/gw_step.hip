@@ -68,9 +68,11 @@ struct GwArgs {
 };
 
 // Index (in series of nen doubles) of series s (0: G^<, 1: G^>, 2: G^r) of band element
-// (i, j, slab ix) in the polarization's gathered copy.
+// (i, j, slab ix) in the polarization's gathered copy: column j holds its band rows
+// max(0, j - nd) .. min(nm - 1, j + nd), at most min(2 nd + 1, nm) of them.
 __device__ __forceinline__ size_t xseries(const GwArgs& a, int i, int j, int ix, int s) {
-  return ((((size_t)ix * a.nm + j) * (2 * a.nd + 1) + (i - j + a.nd)) * 3 + s);
+  const int rows = min(2 * a.nd + 1, a.nm);
+  return ((((size_t)ix * a.nm + j) * rows + (i - max(0, j - a.nd))) * 3 + s);
 }
 
 // Gather for the polarization: the imaginary parts of G^<, G^>, G^r of the band elements
@@ -593,7 +595,8 @@ int gw_polarization(const gw_params* p, const double* Gl, const double* Gg, cons
   int dev = 0;
   HIPCHK(hipGetDevice(&dev));
   if (dev < 0 || dev >= kMaxDev) return fail(1, "gw_polarization: device index out of range");
-  const size_t need = a.tiled ? (size_t)a.nm * (2 * a.nd + 1) * a.nx * 3 * (size_t)a.nen * sizeof(double) : 0;
+  const size_t rows = (size_t)std::min(2 * a.nd + 1, a.nm);
+  size_t need = a.tiled ? (size_t)a.nm * rows * a.nx * 3 * (size_t)a.nen * sizeof(double) : 0;
   std::lock_guard<std::mutex> lk(g_ws_mu);
   Workspace& ws = g_ws[dev];
   if (need && ws.bytes < need) {
@@ -601,10 +604,14 @@ int gw_polarization(const gw_params* p, const double* Gl, const double* Gg, cons
     ws.p = nullptr;
     ws.bytes = 0;
     if (hipMalloc(&ws.p, need) != hipSuccess) {
+      // not fatal: the kernels stage from the arrays directly (slower for dense bands);
+      // the failed allocation's error is cleared so later launch checks do not see it
+      (void)hipGetLastError();
       ws.p = nullptr;
-      return fail(3, "gw_polarization: cannot allocate the " + std::to_string(need >> 20) + " MiB workspace");
+      need = 0;
+    } else {
+      ws.bytes = need;
     }
-    ws.bytes = need;
   }
   a.X = need ? reinterpret_cast<double*>(ws.p) : nullptr;
   if (a.nd < a.nm - 1) {
@@ -672,5 +679,15 @@ long long gw_pairs_selfenergy(int nen, int nopmax) {
 }
 
 const char* gw_last_error(void) { return g_err.c_str(); }
+
+void gw_release(void) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDev) return;
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  Workspace& ws = g_ws[dev];
+  if (ws.p) (void)hipFree(ws.p);
+  ws.p = nullptr;
+  ws.bytes = 0;
+}
 
 }  // extern "C"

@@ -43,6 +43,8 @@ def lib():
     L.gw_pairs_selfenergy.argtypes = [ctypes.c_int, ctypes.c_int]
     L.gw_pairs_selfenergy.restype = ctypes.c_longlong
     L.gw_last_error.restype = ctypes.c_char_p
+    L.gw_release.argtypes = []
+    L.gw_release.restype = None
     _lib = L
     return L
 

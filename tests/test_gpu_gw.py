@@ -46,7 +46,8 @@ def _nopmax(nen, wide):
 @pytest.mark.parametrize("nm,nx,nen,ndiag,wide", [(6, 3, 64, 1, False), (5, 2, 301, 9, True), (4, 2, 1100, 1, False),
                                                   (3, 1, 2048, 2, True), (2, 1, 3072, 1, False),
                                                   (4, 2, 33, 0, 0), (3, 2, 2, 1, 0), (5, 1, 7, 1, True),
-                                                  (12, 2, 200, 11, False), (20, 1, 64, 9, True)])
+                                                  (12, 2, 200, 11, False), (20, 1, 64, 9, True),
+                                                  (72, 1, 130, 71, True), (72, 1, 130, 40, False)])
 def test_polarization_matches_oracle(nm, nx, nen, ndiag, wide):
     import torch
     gw = _gw()

@@ -58,6 +58,51 @@ def test_rgf_rs_matches_oracle(nm, nx, mode):
     compare(g, r, H1i, mode, f"nm={nm} nx={nx} {mode}")
 
 
+def literal_relerrs(gpu, ref):
+    """Worst literal relative Frobenius error per output over its blocks (no scaling)."""
+    out = {}
+    nx = ref["ldos"].shape[2]
+    for k in ("ldos", "ndens", "pdens", "GRi1", "GLi1", "GGi1", "cur"):
+        if k in gpu and ref.get(k) is not None:
+            nb = nx if k in ("ldos", "ndens", "pdens") else nx - 1
+            out[k] = max(relerr(gpu[k][..., l], ref[k][..., l]) for l in range(nb))
+    for k in ("tr", "tre"):
+        if k in gpu:
+            out[k] = abs(gpu[k] - ref[k]) / max(abs(ref[k]), 1e-300)
+    return out
+
+
+@pytest.mark.parametrize("nm,nx,nE", [(256, 8, 16), (192, 5, 8), (128, 6, 8)])
+def test_compat_random_device_per_slab_couplings_matches_oracle(nm, nx, nE):
+    """The benchmark's own mode at reduced size: compat semantics (H1i(:,:,l) couples
+    l, l+1 in the backward sweep), a random device with i.i.d. per-slab couplings and
+    identical lead blocks, energies spread over [-2, 2] eV, eta 1e-5, decimation
+    tolerance 1e-12, Fermi leads mu = +-0.1 eV at kT = 0.0259 eV; every output of
+    green_RGF_RS against the oracle.  The literal relative errors (no scaling of
+    the cancellation-formed outputs) are printed beside the bar."""
+    rgf = _rgf()
+    Hii, H1i = configs.random_device(nm, nx, 2 + nm)
+    E = configs.energy_grid(-2.0, 2.0, 2048)[np.linspace(0, 2047, nE).astype(int)]
+    kw = dict(eta_dev=1e-5, eta_lead=1e-5, dec_tol=1e-12, dec_nmax=100)
+    p, po = rgf.params(rgf.MODE_COMPAT, **kw), orc.params(orc.COMPAT, **kw)
+    T = 0.0259 / configs.BOLTZ_REF
+    temp, mu = (T, T), (0.1, -0.1)
+    g = rgf.rgf_rs_energies(p, temp, E, mu, Hii, H1i)
+    r = orc.rgf_energies(po, temp, E, mu, Hii, H1i)
+    worst_ratio, worst_lit = 0.0, {}
+    for e in range(nE):
+        ge, re_ = parity.energy_slice(g, e), parity.energy_slice(r, e)
+        worst_ratio = max(worst_ratio, compare(ge, re_, H1i, "compat", f"nm={nm} nx={nx} E[{e}]={E[e]:.4f}"))
+        for k, v in literal_relerrs(ge, re_).items():
+            worst_lit[k] = max(worst_lit.get(k, 0.0), v)
+        assert tuple(g["info"][e, :2]) == tuple(r["diag"][e][:2])
+    print(f"\nparity nm={nm} nx={nx} nE={nE}: worst err/allowed {worst_ratio:.3e}; literal relative errors "
+          + ", ".join(f"{k} {v:.2e}" for k, v in worst_lit.items()), flush=True)
+    # outputs not formed by cancellation meet the plain 1e-10 relative bar
+    for k in ("ldos", "ndens", "GRi1", "GLi1"):
+        assert worst_lit[k] <= TOL, (k, worst_lit[k])
+
+
 def test_rgf_rs_default_entry_is_the_reference_configuration():
     rgf = _rgf()
     Hii, H1i, (sL, sG, sR) = case(32, 3, 5, uniform=False)

@@ -252,6 +252,116 @@ def test_compat_nonuniform_forward_and_right_boundary():
     assert relerr(o["ldos"][:, :, 0], d["ldos"][:, :, 0]) > 1e-6
 
 
+def _left_connected_lesser(E, eta, Hii, H1i, SigL, fL, sigL, sigR, l):
+    """(g_l, g^<_l) of the left part {0..l} with H_{k,k-1} = H1i_k (the forward
+    sweep's couplings), the left lead's source i Gamma_L f_L on slab 0 and the
+    diagonal scattering blocks: the last diagonal block of the dense
+    (z - H - Sigma)^-1 and of G Sigma^< G^H."""
+    nm = Hii.shape[0]
+    N = nm * (l + 1)
+    d = dense_rgf(E, eta, Hii[:, :, : l + 1], H1i[:, :, : l + 1], SigL, np.zeros((nm, nm)), fL, 0.0,
+                  None if sigL is None else sigL[:, :, : l + 1], None if sigR is None else sigR[:, :, : l + 1],
+                  [H1i[:, :, k] for k in range(l + 1)])
+    return d["G"][N - nm:, N - nm:], d["Gl"][N - nm:, N - nm:]
+
+
+def _two_block_step(g, gl, C, G1, Gl1):
+    """One backward step as the dense inverse of a two-block system.
+
+    The left block is the left-connected slab (inverse g^-1, Keldysh source
+    s1 = g^-1 g^< g^-H); the right block is chosen so that the coupled system's
+    (2,2) blocks are the given fully connected G1 = G_{l+1,l+1} and
+    Gl1 = G^<_{l+1,l+1}:  M = [[g^-1, -C^H], [-C, G1^-1 + C g C^H]] and
+    s2 = G1^-1 (Gl1 - G21 s1 G21^H) G1^-H.  Returns the full 2x2-block G^R and
+    G^< = M^-1 diag(s1, s2) M^-H; the coupling between the slabs is C (block
+    (2,1)) and C^H (block (1,2))."""
+    n = g.shape[0]
+    gi = np.linalg.inv(g)
+    M = np.zeros((2 * n, 2 * n), np.complex128)
+    M[:n, :n] = gi
+    M[:n, n:] = -dag(C)
+    M[n:, :n] = -C
+    M[n:, n:] = np.linalg.inv(G1) + C @ g @ dag(C)
+    Gf = np.linalg.inv(M)
+    s1 = gi @ gl @ dag(gi)
+    G21, G22 = Gf[n:, :n], Gf[n:, n:]
+    G22i = np.linalg.inv(G22)
+    s2 = G22i @ (Gl1 - G21 @ s1 @ dag(G21)) @ dag(G22i)
+    S = np.zeros((2 * n, 2 * n), np.complex128)
+    S[:n, :n], S[n:, n:] = s1, s2
+    return Gf, Gf @ S @ dag(Gf)
+
+
+@pytest.mark.parametrize("nm,nx,seed,E,scatter", [(3, 6, 41, 0.05, True), (4, 5, 43, -0.4, True),
+                                                   (2, 7, 45, 0.3, False), (5, 4, 47, 0.0, True)])
+def test_compat_nonuniform_backward_two_block(nm, nx, seed, E, scatter):
+    """Compat backward sweep with per-slab couplings (the benchmark's mode, trap ii:
+    H1i(:,:,l) couples l, l+1 in the backward sweep, green_rgf.f90:4069-4160, while
+    the forward sweep used it for l-1, l).  It is no Green's function of one
+    Hamiltonian, so each step is pinned on its own: with g_l, g^<_l from the dense
+    left-connected part (forward couplings) and the oracle's G_{l+1}, G^<_{l+1}
+    (the last slab is pinned to the dense device), the step's outputs equal the
+    dense inverse of the two-block system coupled by C = H1i_l:
+    ldos_l = G_11, GRi1_l = G_12, ndens_l = G^<_11, GLi1_l = G^<_12,
+    cur_l = Re(C^H G^<_21) - Re(C G^<_12), pdens_l = G^<_11 + G_11 - G_11^H,
+    and tre from G_11, G^<_11 and the lead's Gamma_L (T(2) for the Fermi factor)."""
+    Hii, H1i, sigL, sigG, sigR = _small_case(nm, nx, seed, uniform=False)
+    if not scatter:
+        sigL = sigG = sigR = None
+    p = orc.params(orc.COMPAT)
+    temp, mu = (300.0, 300.0), (0.1, -0.1)
+    o = orc.rgf(p, temp, E, mu, Hii, H1i, sigL, sigG, sigR)
+    d = _dense_for(p, E, temp, mu, Hii, H1i, sigL, sigR)
+    # induction start: the last slab is the dense device's (forward couplings throughout)
+    for k in ("ldos", "ndens"):
+        assert relerr(o[k][:, :, nx - 1], d[k][:, :, nx - 1]) < 1e-10, k
+    SigL, _ = lead_self_energies(orc.sancho, E, Hii, H1i, sigR, p.eta_lead, p.dec_tol, p.dec_nmax)
+    fL = fermi(E, mu[0], p.boltz, temp[0])
+    n = nm
+    worst = 0.0
+    for l in range(nx - 2, -1, -1):
+        g, gl = _left_connected_lesser(E, p.eta_dev, Hii, H1i, SigL, fL, sigL, sigR, l)
+        C = H1i[:, :, l]
+        Gf, Glf = _two_block_step(g, gl, C, o["ldos"][:, :, l + 1], o["ndens"][:, :, l + 1])
+        G11, G12 = Gf[:n, :n], Gf[:n, n:]
+        Gl11, Gl12, Gl21 = Glf[:n, :n], Glf[:n, n:], Glf[n:, :n]
+        cur = np.real(dag(C) @ Gl21) - np.real(C @ Gl12)
+        pd = Gl11 + G11 - dag(G11)
+        errs = {"ldos": relerr(o["ldos"][:, :, l], G11), "GRi1": relerr(o["GRi1"][:, :, l], G12),
+                "ndens": relerr(o["ndens"][:, :, l], Gl11), "GLi1": relerr(o["GLi1"][:, :, l], Gl12),
+                "cur": np.linalg.norm(o["cur"][:, :, l] - cur) / (np.linalg.norm(cur) + 2 * np.linalg.norm(C) * np.linalg.norm(Gl12)),
+                "pdens": np.linalg.norm(o["pdens"][:, :, l] - pd) / (np.linalg.norm(Gl11) + 2 * np.linalg.norm(G11))}
+        for k, v in errs.items():
+            assert v < 1e-10, (k, l, v)
+            worst = max(worst, v)
+    # tre from the step-0 blocks (the reference's left-contact formula with T(2))
+    G11, Gl11 = o["ldos"][:, :, 0], o["ndens"][:, :, 0]
+    GamL = -(SigL - dag(SigL))
+    fLt = fermi(E, mu[0], p.boltz, temp[1])
+    tre = np.real(np.trace(fLt * GamL @ (Gl11 + G11 - dag(G11)) - (fLt - 1.0) * GamL @ Gl11))
+    assert abs(o["tre"] - tre) < 1e-10 * max(1.0, abs(tre))
+    # the quirk is exercised: the consistent device differs away from the last slab
+    assert relerr(o["ldos"][:, :, 0], d["ldos"][:, :, 0]) > 1e-6
+    assert worst < 1e-10
+
+
+def test_two_block_step_catches_a_wrong_coupling():
+    """The two-block pin is sensitive to the coupling the step uses: with the forward
+    sweep's coupling H1i_{l+1} in place of H1i_l the step's outputs differ."""
+    nm, nx = 3, 5
+    Hii, H1i, sigL, sigG, sigR = _small_case(nm, nx, 41, uniform=False)
+    p = orc.params(orc.COMPAT)
+    temp, mu, E = (300.0, 300.0), (0.1, -0.1), 0.05
+    o = orc.rgf(p, temp, E, mu, Hii, H1i, sigL, sigG, sigR)
+    SigL, _ = lead_self_energies(orc.sancho, E, Hii, H1i, sigR, p.eta_lead, p.dec_tol, p.dec_nmax)
+    fL = fermi(E, mu[0], p.boltz, temp[0])
+    l = nx - 3
+    g, gl = _left_connected_lesser(E, p.eta_dev, Hii, H1i, SigL, fL, sigL, sigR, l)
+    Gf, Glf = _two_block_step(g, gl, H1i[:, :, l + 1], o["ldos"][:, :, l + 1], o["ndens"][:, :, l + 1])
+    assert relerr(o["ldos"][:, :, l], Gf[:nm, :nm]) > 1e-6
+    assert relerr(o["ndens"][:, :, l], Glf[:nm, :nm]) > 1e-6
+
+
 def test_equilibrium_fluctuation_dissipation():
     """mu_L == mu_R, T_L == T_R, eta_dev = 0, no scattering: G^< = -f (G - G^H) on every block."""
     nm, nx = 4, 6
