@@ -1,0 +1,109 @@
+"""CPU: the sweep kernels' block-level program (solve_energy of
+liteqt_amd/csrc/rgf_sweep.hpp, compiled with g++ against the host stand-in
+tests/hostsweep/rgf_blocks_host.hpp) run on the CPU.  Every block pointer the
+program hands to a block operation is checked against the arrays it may touch
+(inputs, outputs, the workspace slot) and against the operand aliasing the
+device functions allow; the results are compared with the oracle.  Covers the
+plain solve (green_RGF_RS) and the overlap solve (green_RGF_CMS), the latter
+at the shape of its GPU memory fault (nm = 64, nx = 3)."""
+import ctypes
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from liteqt_amd import configs
+from oracle import pyoracle as orc
+from tests import parity
+from tests.test_oracle_cms import _overlap
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HERE = os.path.join(ROOT, "tests", "hostsweep")
+
+
+@pytest.fixture(scope="module")
+def hostlib(tmp_path_factory):
+    d = tmp_path_factory.mktemp("hostsweep")
+    shutil.copy(os.path.join(ROOT, "liteqt_amd", "csrc", "rgf_sweep.hpp"), d / "rgf_sweep.hpp")
+    shutil.copy(os.path.join(HERE, "rgf_blocks_host.hpp"), d / "rgf_blocks.hpp")
+    shutil.copy(os.path.join(HERE, "host_sweep.cpp"), d / "host_sweep.cpp")
+    so = d / "libhostsweep.so"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wno-unused-variable",
+                    "-Wno-unused-but-set-variable", "-o", str(so), str(d / "host_sweep.cpp")], check=True, cwd=d)
+    L = ctypes.CDLL(str(so))
+    vp, ci, cd = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    L.host_sweep.argtypes = ([ci, ci, ci, ci, ci, vp, vp, vp, cd, cd, cd, ci, cd] + [vp] * 5 + [ctypes.c_longlong]
+                             + [vp] * 13 + [ctypes.c_char_p, ci])
+    return L
+
+
+def relerr(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def run(L, cms, mode, p, temp, E, mu, Hii, H1i, Sii=None, sigL=None, sigR=None):
+    nm, _, nx = Hii.shape
+    E = np.ascontiguousarray(np.atleast_1d(E), np.float64)
+    nE = len(E)
+    shp = (nm, nm, nx, nE)
+    out = {k: np.zeros(shp, np.complex128, order="F") for k in ("ldos", "ndens", "pdens", "GRi1", "GLi1", "GGi1")}
+    out["cur"] = np.zeros(shp, np.float64, order="F")
+    for k in ("tr", "tre"):
+        out[k] = np.zeros(nE)
+    out["dens_n"], out["dens_p"] = np.zeros((nE, nx)), np.zeros((nE, nx))
+    out["info"] = np.zeros((nE, 4), np.int32)
+    f = lambda a: None if a is None else np.asfortranarray(a, np.complex128)
+    Hii, H1i, Sii, sigL, sigR = f(Hii), f(H1i), f(Sii), f(sigL), f(sigR)
+    log = ctypes.create_string_buffer(1 << 16)
+    nops = ctypes.c_longlong(0)
+    t, m = np.asarray(temp, np.float64), np.asarray(mu, np.float64)  # kept alive across the call
+    nv = L.host_sweep(int(cms), mode, nm, nx, nE, _p(E), _p(t), _p(m), p.eta_dev, p.eta_lead, p.dec_tol, p.dec_nmax, p.boltz,
+                      _p(Hii), _p(H1i), _p(Sii), _p(sigL), _p(sigR), 0,
+                      *[_p(out[k]) for k in ("ldos", "ndens", "pdens", "GRi1", "GLi1", "GGi1", "cur", "tr", "tre",
+                                             "dens_n", "dens_p", "info")], ctypes.byref(nops), log, len(log))
+    return nv, log.value.decode(), nops.value, out
+
+
+@pytest.mark.parametrize("nm,nx", [(64, 3), (32, 4), (20, 3), (8, 5)])
+def test_overlap_solve_program_in_bounds_and_equal_to_oracle(hostlib, nm, nx):
+    Hii, H1i = configs.random_device(nm, nx, 70 + nm, identical_leads=False)
+    sL, sG, sR = configs.scattering_self_energies(nm, nx, 71 + nm, E=0.1, mu=0.0, scale=0.02)
+    S = _overlap(nm, nx, 72 + nm)
+    temp, mu, E = (300.0, 250.0), (0.1, -0.1), 0.13
+    po = orc.params(orc.COMPAT, eta_dev=1e-4, eta_lead=1e-4, dec_tol=1e-12)
+    nv, log, nops, g = run(hostlib, 1, 0, po, temp, E, mu, Hii, H1i, S, sL, sR)
+    assert nv == 0, log
+    assert nops > 10 * nx
+    r = orc.rgf_cms(po, temp, E, mu, S, Hii, H1i, sL, sG, sR)
+    for k in ("ldos", "ndens", "GRi1", "GLi1"):
+        nb = nx if k in ("ldos", "ndens") else nx - 1
+        for l in range(nb):
+            assert relerr(g[k][..., l, 0], r[k][..., l]) <= 1e-10, (k, l)
+    for l in range(nx):
+        assert np.linalg.norm(g["pdens"][..., l, 0] - r["pdens"][..., l]) <= 1e-9 * np.linalg.norm(r["pdens"][..., l])
+    assert abs(g["tr"][0] - r["tr"]) <= 1e-9 * max(1.0, abs(r["tr"]))
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_plain_solve_program_in_bounds_and_equal_to_oracle(hostlib, mode):
+    nm, nx = 16, 5
+    Hii, H1i = configs.random_device(nm, nx, 5)
+    sL, sG, sR = configs.scattering_self_energies(nm, nx, 6, E=0.0, scale=0.02)
+    E = np.array([-0.4, 0.05, 0.7])
+    temp, mu = (300.0, 250.0), (0.1, -0.1)
+    po = orc.params(mode, eta_dev=1e-4, eta_lead=1e-4, dec_tol=1e-12)
+    nv, log, nops, g = run(hostlib, 0, mode, po, temp, E, mu, Hii, H1i, None, sL, sR)
+    assert nv == 0, log
+    r = orc.rgf_energies(po, temp, E, mu, Hii, H1i, sL, sG, sR)
+    for e in range(len(E)):
+        for k in ("ldos", "ndens", "GRi1", "GLi1"):
+            nb = nx if k in ("ldos", "ndens") else nx - 1
+            for l in range(nb):
+                assert relerr(g[k][..., l, e], r[k][..., l, e]) <= 1e-10, (k, l, e)
+        assert abs(g["tr"][e] - r["tr"][e]) <= 1e-9 * max(1.0, abs(r["tr"][e]))
