@@ -127,8 +127,9 @@ int rgf_rs_energies(const rgf_params *p, const double temp[2], int nm, int nx,
 
 /* Device-resident batched solve (the engine's native entry point).  Every
  * pointer is device memory; the call is asynchronous on `stream` (a
- * hipStream_t, NULL for the default stream).  nm must be a multiple of 32 and
- * at most 256 (the host-array entry points pad other sizes).  Per-energy
+ * hipStream_t, NULL for the default stream).  nm must be a multiple of 32 up
+ * to 256 or a multiple of 64 up to 512 (with off-diagonal Sigma or overlaps:
+ * a multiple of 32 up to 256); the host-array entry points pad other sizes.  Per-energy
  * arrays have a trailing energy index with stride nm*nm*nx (complex for the G
  * blocks, real for cur).  dens_n / dens_p (nx per energy) receive
  * Im Tr G^<_ll and -Im Tr G^>_ll, the integrands of the electron and hole
@@ -277,6 +278,27 @@ long long rgf_adjoint_workspace_bytes(int nm, int nx, int slots);
 int rgf_zgemm_batched(int n, int nbatch, int opA, const double *A, int opB, const double *B,
                       double *C, int c_rowmajor, void *stream);
 int rgf_zinv_batched(int n, int nbatch, const double *A, double *Ainv, int *info, void *stream);
+
+/* Energy and k-point integrals of the batched solve's per-energy reductions,
+ * replacing calc_n_electron (src/green_rgf.f90:3920-3945; called at 752) and
+ * the terminal current of the SCBA drivers (src/green_rgf.f90:3189, 788).
+ * Device arrays, asynchronous on `stream`:
+ *   nelec[l] = dE/(2 pi) * dK * sum_{ik, ie: E[ie] >  midgap[l]} dens_n[(ik*nE + ie)*nx + l]
+ *   pelec[l] = dE/(2 pi) * dK * sum_{ik, ie: E[ie] <= midgap[l]} dens_p[(ik*nE + ie)*nx + l]
+ *   current[0] = scale * dE * sum_{ik, ie} tr[ik*nE + ie]
+ * with dE = E[1] - E[0] and dK = 2 pi / nk (pi = 3.14159265359, the
+ * reference's constant).  dens_n / dens_p are what rgf_solve_batched writes
+ * (Im Tr G^<_ll and -Im Tr G^>_ll per energy and slab), so the integral reads
+ * 2*nx + 1 doubles per energy and k-point instead of the full Green's
+ * functions; it equals the reference's sum of Im G^<_jj / -Im G^>_jj over the
+ * orbitals j of each slab.  The reference's current is scale =
+ * e0^2 / (2 pi hbar) * spindeg / nk (e0 = 1.6022e-19 C, hbar = 1.0546e-34 J s).
+ * dens_n / dens_p / tr may be NULL (their outputs are then not written);
+ * densities need midgap, nelec and pelec, tr needs current.  Sums run in a
+ * fixed order (deterministic).  nE >= 2. */
+int rgf_integrate(int nk, int nE, int nx, const double *E, const double *midgap,
+                  const double *dens_n, const double *dens_p, const double *tr, double scale,
+                  double *nelec, double *pelec, double *current, void *stream);
 
 /* Release cached device buffers of this library on the current device. */
 void rgf_release(void);

@@ -632,13 +632,13 @@ __device__ __forceinline__ double wg_imtrace(int n, const double2* X, ldsd* red)
 // out (a separate n x n buffer) receives inv(i, q) = W[piv[i]][pinv[q]] after the
 // last panel.
 template <int T, int NMAX>
-struct InvLds {
+struct InvSmallLds {
   static constexpr int LDI = NMAX + PAD;
   static constexpr int DOUBLES = 4 * KP * LDI + 2 * NMAX + 16;
 };
 
 template <int T, int NMAX>
-__device__ __noinline__ int wg_inverse(int n, double2* W_, double2* out_, ldsd* lds) {
+__device__ __noinline__ int wg_inverse_small(int n, double2* W_, double2* out_, ldsd* lds) {
   using C = Cfg<T>;
   constexpr int LDI = NMAX + PAD;
   constexpr int MR = C::MR, MC = C::MC, NTH = C::NTH, TA = C::TA, TB = C::TB;
@@ -864,6 +864,272 @@ __device__ __noinline__ int wg_inverse(int n, double2* W_, double2* out_, ldsd* 
     __syncthreads();
   }
   return 0;
+}
+
+// ------------------------------------------------------ inverse, n > NTH
+// The same Gauss-Jordan sweep for blocks larger than the workgroup (NMAX rows,
+// RPT = NMAX / NTH panel rows per thread).  The swept panel M = F - E (n x KP,
+// row-major) and the snapshot of the KP pivot rows (KP x n) do not fit in LDS at
+// these sizes; they are kept in `out`, which is free until the final permutation
+// copy, and staged per update tile through LDS (one K = KP panel of the tile's
+// rows of M and columns of the snapshot), so the update of W is a K = 16 MFMA GEMM
+// per tile accumulating onto the tile's old values as for n <= NTH.
+template <int T, int NMAX>
+struct InvBigLds {
+  static constexpr int LDA = Cfg<T>::MR + PAD, LDB = Cfg<T>::MC + PAD;
+  static constexpr int RB = 8;                                 // rows per step of the final copy
+  static constexpr int PANELS = 2 * KP * LDA + 2 * KP * LDB;  // staged M rows and snapshot columns
+  static constexpr int COPY = 2 * RB * (NMAX + 1);
+  static constexpr int OFF_IDX = PANELS > COPY ? PANELS : COPY;  // piv, used after the larger area
+  static constexpr int XCH = 2 * (4 * 2 * KP) + 2 * 8;          // pivot-row exchange, two parities
+  static constexpr int DOUBLES = OFF_IDX + NMAX + XCH + 16;      // piv + used = 2 NMAX ints
+};
+
+__device__ __forceinline__ int wave_min_int(int v) {
+  for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+template <int T, int NMAX>
+__device__ __noinline__ int wg_inverse_big(int n, double2* W_, double2* out_, ldsd* lds) {
+  using C = Cfg<T>;
+  using L = InvBigLds<T, NMAX>;
+  constexpr int NTH = C::NTH, RPT = (NMAX + NTH - 1) / NTH;
+  constexpr int MR = C::MR, MC = C::MC, TA = C::TA, TB = C::TB, LDA = L::LDA, LDB = L::LDB;
+  static_assert(RPT <= 8, "panel rows per thread");
+  gd2* W = gw(W_);
+  gd2* out = gw(out_);
+  gd2* Mg = out;                       // M = F - E, row-major n x KP
+  gd2* Sg = out + (size_t)n * KP;      // pivot-row snapshot, KP x n (row c at c * n)
+  ldsd* Ar = lds;
+  ldsd* Ai = Ar + KP * LDA;
+  ldsd* Br = Ai + KP * LDA;
+  ldsd* Bi = Br + KP * LDB;
+  ldsi* piv = (ldsi*)(lds + L::OFF_IDX);
+  ldsi* used = piv + NMAX;
+  ldsd* xch = (ldsd*)(used + NMAX);  // [parity][wave][2 KP] rows, then [parity][8] values / rows
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int arow = (wave % C::WR) * 16 * TA, bcol = (wave / C::WR) * 16 * TB;
+  for (int i = tid; i < n; i += NTH) used[i] = 0;
+  __syncthreads();
+  int info = 0;
+  unsigned myused = 0;  // bit q: row tid + q * NTH has been a pivot
+  for (int k0 = 0; k0 < n && !info; k0 += KP) {
+    double2 rv[RPT][KP];
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      const int r = tid + q * NTH;
+#pragma unroll
+      for (int c2 = 0; c2 < KP; ++c2) rv[q][c2] = r < n ? W[(size_t)r * n + k0 + c2] : make_double2(0, 0);
+    }
+    for (int c = 0; c < KP; ++c) {
+      // the thread's candidate: first maximum of |Re| + |Im| over its unused rows
+      double myv = -1.0;
+      int myq = 0;
+#pragma unroll
+      for (int q = 0; q < RPT; ++q) {
+        double2 cand = make_double2(0, 0);
+#pragma unroll
+        for (int c2 = 0; c2 < KP; ++c2)
+          if (c2 == c) cand = rv[q][c2];
+        const bool valid = (tid + q * NTH) < n && !((myused >> q) & 1u);
+        const double v = valid ? fabs(cand.x) + fabs(cand.y) : -1.0;
+        if (v > myv) { myv = v; myq = q; }
+      }
+      const int myrow = tid + myq * NTH;
+      double best = wave_max(myv);
+      int brow = wave_min_int((myv == best && myv >= 0.0) ? myrow : 0x7fffffff);
+      ldsd* xrow = xch + (c & 1) * (4 * 2 * KP);
+      ldsd* xval = xch + 2 * (4 * 2 * KP) + (c & 1) * 8;
+      if (myrow == brow) {
+#pragma unroll
+        for (int q = 0; q < RPT; ++q)
+          if (q == myq) {
+#pragma unroll
+            for (int c2 = 0; c2 < KP; ++c2) {
+              xrow[wave * 2 * KP + 2 * c2] = rv[q][c2].x;
+              xrow[wave * 2 * KP + 2 * c2 + 1] = rv[q][c2].y;
+            }
+          }
+      }
+      if (lane == 0) { xval[wave] = best; xval[4 + wave] = (double)brow; }
+      __syncthreads();
+      best = xval[0];
+      int bw = 0;
+      int bi = (int)xval[4];
+      for (int w = 1; w < NTH / 64; ++w) {
+        const double ov = xval[w];
+        const int oi = (int)xval[4 + w];
+        if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; bw = w; }
+      }
+      if (!(best > 0.0)) { info = k0 + c + 1; break; }
+      const int p = bi;
+      const ldsd* prow = xrow + bw * 2 * KP;
+      const double2 ainv = crecip(make_double2(prow[2 * c], prow[2 * c + 1]));
+      double2 rp[KP];
+#pragma unroll
+      for (int c2 = 0; c2 < KP; ++c2)
+        rp[c2] = (c2 == c) ? ainv : cmul(make_double2(prow[2 * c2], prow[2 * c2 + 1]), ainv);
+#pragma unroll
+      for (int q = 0; q < RPT; ++q) {
+        const int r = tid + q * NTH;
+        if (r == p) {
+          myused |= 1u << q;
+          used[p] = 1;
+          piv[k0 + c] = p;
+#pragma unroll
+          for (int c2 = 0; c2 < KP; ++c2) rv[q][c2] = rp[c2];
+        } else if (r < n) {
+          double2 f = make_double2(0, 0);
+#pragma unroll
+          for (int c2 = 0; c2 < KP; ++c2)
+            if (c2 == c) f = rv[q][c2];
+#pragma unroll
+          for (int c2 = 0; c2 < KP; ++c2) rv[q][c2] = csub((c2 == c) ? make_double2(0, 0) : rv[q][c2], cmul(f, rp[c2]));
+        }
+      }
+    }
+    if (info) break;
+    __syncthreads();  // piv of this panel visible to every thread
+    // M = F - E into out (row-major n x KP): a pivot row p = piv[k0 + c] has 1 subtracted at column c
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      const int r = tid + q * NTH;
+      if (r < n) {
+#pragma unroll
+        for (int c2 = 0; c2 < KP; ++c2) {
+          double2 v = rv[q][c2];
+          v.x -= (piv[k0 + c2] == r) ? 1.0 : 0.0;
+          Mg[(size_t)r * KP + c2] = v;
+        }
+      }
+    }
+    // snapshot of the pivot rows before the update changes them, all loads of a thread in flight
+    for (int j0 = 0; j0 < n; j0 += NTH) {
+      const int j = j0 + tid;
+      if (j < n) {
+        double2 v[KP];
+#pragma unroll
+        for (int c2 = 0; c2 < KP; ++c2) v[c2] = W[(size_t)piv[k0 + c2] * n + j];
+#pragma unroll
+        for (int c2 = 0; c2 < KP; ++c2) Sg[(size_t)c2 * n + j] = v[c2];
+      }
+    }
+    __syncthreads();
+    const int ntr = n / MR, ntc = n / MC;
+    const int m = lane & 15, kq = lane >> 4;
+    const uint32_t t_off = 16u * (uint32_t)((arow + kq) * n + bcol + m);
+    for (int tile = 0; tile < ntr * ntc; ++tile) {
+      const int row0 = (tile % ntr) * MR, col0 = (tile / ntr) * MC;
+      d4 cr[TA][TB], ci[TA][TB];
+#pragma unroll
+      for (int ti = 0; ti < TA; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < TB; ++tj)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const double2 o = ldgb(W_ + (size_t)(row0 + ti * 16 + 4 * r) * n + col0 + tj * 16, t_off);
+            cr[ti][tj][r] = o.x;
+            ci[ti][tj][r] = o.y;
+          }
+      // stage M rows row0.. (k-major, [c][i]) and snapshot columns col0.. ([c][j])
+      for (int idx = tid; idx < MR * KP; idx += NTH) {
+        const int c2 = idx % KP, i = idx / KP;
+        const double2 v = Mg[(size_t)(row0 + i) * KP + c2];
+        Ar[c2 * LDA + i] = v.x;
+        Ai[c2 * LDA + i] = v.y;
+      }
+      for (int idx = tid; idx < MC * KP; idx += NTH) {
+        const int j = idx % MC, c2 = idx / MC;
+        const double2 v = Sg[(size_t)c2 * n + col0 + j];
+        Br[c2 * LDB + j] = v.x;
+        Bi[c2 * LDB + j] = v.y;
+      }
+      __syncthreads();
+      mma_steps<TA, TB, 1, KP / 4>(cr, ci, Ar, Ai, LDA, arow, Br, Bi, LDB, bcol);
+      const bool panel = col0 <= k0 && k0 < col0 + MC;  // the panel's columns take F = M + E
+      if (!panel) {
+#pragma unroll
+        for (int ti = 0; ti < TA; ++ti)
+#pragma unroll
+          for (int tj = 0; tj < TB; ++tj)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+              stgb(W_ + (size_t)(row0 + ti * 16 + 4 * r) * n + col0 + tj * 16, t_off,
+                   make_double2(cr[ti][tj][r], ci[ti][tj][r]));
+      } else {
+#pragma unroll
+        for (int ti = 0; ti < TA; ++ti)
+#pragma unroll
+          for (int tj = 0; tj < TB; ++tj)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int il = arow + ti * 16 + 4 * r + kq;  // row within the tile
+              const int j = col0 + bcol + tj * 16 + m;
+              const bool inp = j >= k0 && j < k0 + KP;
+              const int c2 = inp ? j - k0 : 0;
+              double2 f = make_double2(Ar[c2 * LDA + il], Ai[c2 * LDA + il]);
+              f.x += (piv[k0 + c2] == row0 + il) ? 1.0 : 0.0;
+              const double2 v = inp ? f : make_double2(cr[ti][tj][r], ci[ti][tj][r]);
+              stgb(W_ + (size_t)(row0 + ti * 16 + 4 * r) * n + col0 + tj * 16, t_off, v);
+            }
+      }
+      __syncthreads();  // the staged panels are overwritten by the next tile
+    }
+  }
+  if (info) {
+    __syncthreads();
+    for (int idx = tid; idx < n * n; idx += NTH) out[idx] = make_double2(0, 0);
+    __syncthreads();
+    return info;
+  }
+  for (int q = tid; q < n; q += NTH) used[piv[q]] = q;  // inverse permutation
+  __syncthreads();
+  // out(i, q) = W[piv[i]][used[q]], RB rows at a time staged through LDS
+  constexpr int RB = L::RB;
+  const int ldr = n + 1;
+  ldsd* Lr = lds;
+  ldsd* Li = lds + RB * ldr;
+  for (int i0 = 0; i0 < n; i0 += RB) {
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      const int j = tid + q * NTH;
+      if (j < n) {
+        double2 v[RB];
+#pragma unroll
+        for (int rr = 0; rr < RB; ++rr) v[rr] = W[(size_t)piv[i0 + rr] * n + j];
+#pragma unroll
+        for (int rr = 0; rr < RB; ++rr) {
+          Lr[rr * ldr + j] = v[rr].x;
+          Li[rr * ldr + j] = v[rr].y;
+        }
+      }
+    }
+    __syncthreads();
+    for (int idx = tid; idx < RB * n; idx += NTH) {
+      const int rr = idx % RB, q = idx / RB;
+      const int c = used[q];
+      out[(size_t)(i0 + rr) + (size_t)q * n] = make_double2(Lr[rr * ldr + c], Li[rr * ldr + c]);
+    }
+    __syncthreads();
+  }
+  return 0;
+}
+
+template <int T, int NMAX>
+struct InvLds {
+  static constexpr int DOUBLES =
+      NMAX <= Cfg<T>::NTH ? InvSmallLds<T, NMAX>::DOUBLES : InvBigLds<T, NMAX>::DOUBLES;
+};
+
+// Inverse of an n x n block (n <= NMAX): one panel row per thread up to NTH rows,
+// several rows per thread with the panels staged through `out` above that.
+template <int T, int NMAX>
+__device__ __forceinline__ int wg_inverse(int n, double2* W, double2* out, ldsd* lds) {
+  if constexpr (NMAX <= Cfg<T>::NTH)
+    return wg_inverse_small<T, NMAX>(n, W, out, lds);
+  else
+    return wg_inverse_big<T, NMAX>(n, W, out, lds);
 }
 
 }  // namespace rgf

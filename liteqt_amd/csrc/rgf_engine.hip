@@ -14,6 +14,7 @@
 
 #include "../../include/rgf.h"
 #include "rgf_adjoint.hpp"
+#include "rgf_integrate.hpp"
 #include "rgf_multi.hpp"
 #include "rgf_offdiag.hpp"
 #include "rgf_sweep.hpp"
@@ -162,9 +163,19 @@ struct Variant {
   int T, nmax, nth;
 };
 
+// Largest block size of the plain sweep, and the granule above 256 (blocks
+// above 256 run with several inverse panel rows per thread).
+constexpr int kNmaxBig = 512;
+
 // Tile variant per block size: wave tile (16T)^2, macro tile (32T)^2; the
-// inverse's LDS panel is sized for NMAX rows.
+// inverse's LDS panel is sized for NMAX rows.  Above 256: n a multiple of 64,
+// 128 x 64 macro tiles when n is a multiple of 128, else 64 x 64.
 bool pick_variant(int n, Variant* v) {
+  if (n > 256 && n <= kNmaxBig && n % 64 == 0) {
+    if (n % 128 == 0) *v = {launch_sweep<4, kNmaxBig>, sweep_fn<4, kNmaxBig>(), 4, kNmaxBig, rgf::Cfg<4>::NTH};
+    else *v = {launch_sweep<2, kNmaxBig>, sweep_fn<2, kNmaxBig>(), 2, kNmaxBig, rgf::Cfg<2>::NTH};
+    return true;
+  }
   if (n % 32 != 0 || n < 32 || n > 256) return false;
   switch (n) {
     case 32: *v = {launch_sweep<1, 32>, sweep_fn<1, 32>(), 1, 32, rgf::Cfg<1>::NTH}; return true;
@@ -256,7 +267,8 @@ int prepare(const rgf_params* p, const rgf_batch* b, const rgf_scba* scba, Prepa
   }
   Variant& v = out->v;
   if (!(od ? pick_variant_od(b->nm, &v) : cms ? pick_variant_cms(b->nm, &v) : pick_variant(b->nm, &v)))
-    return fail(RGF_EUNSUP, "rgf_solve_batched: nm must be a multiple of 32 in [32, 256]");
+    return fail(RGF_EUNSUP, od || cms ? "rgf_solve_batched: with off-diagonal Sigma or overlaps nm must be a multiple of 32 in [32, 256]"
+                                      : "rgf_solve_batched: nm must be a multiple of 32 in [32, 256] or of 64 in (256, 512]");
   int dev = 0;
   HIPCHK(hipGetDevice(&dev));
   if (dev < 0 || dev >= kMaxDev) return fail(RGF_EINVAL, "device index out of range");
@@ -347,7 +359,6 @@ int solve_impl(const rgf_params* p, const rgf_batch* b, const rgf_scba* scba, vo
   HIPCHK(hipGetLastError());
   return 0;
 }
-
 
 // ------------------------------------------------------------ many devices
 int mv_T(int v) {
@@ -599,6 +610,7 @@ int rgf_solve_batched(const rgf_params* p, const rgf_batch* b, void* stream) {
   return solve_impl(p, b, nullptr, stream);
 }
 
+
 int rgf_scba_step(const rgf_params* p, const rgf_batch* b, const rgf_scba* scba, void* stream) {
   if (!scba) {
     g_err.clear();
@@ -811,8 +823,10 @@ static int rs_energies_impl(const rgf_params* p, const double temp[2], int nm, i
   if (nm < 1 || nx < 2 || nE < 0) return fail(RGF_EINVAL, "need nm >= 1, nx >= 2, nE >= 0");
   if (!temp || !mu || !Hii || !H1i || (nE > 0 && !E)) return fail(RGF_EINVAL, "missing required array");
   if (nE == 0) return 0;
-  const int np = (nm + 31) / 32 * 32;
-  if (np > 256) return fail(RGF_EUNSUP, "nm > 256 is not supported");
+  // blocks are padded to the engine's granule: 32 up to 256, 64 above (overlaps: up to 256)
+  const int np = nm <= 256 ? (nm + 31) / 32 * 32 : (nm + 63) / 64 * 64;
+  if (np > (Sii ? 256 : kNmaxBig))
+    return fail(RGF_EUNSUP, Sii ? "the overlap solve supports nm <= 256" : "nm > 512 is not supported");
   std::lock_guard<std::mutex> lk(g_mu);
   int dev = 0;
   HIPCHK(hipGetDevice(&dev));
@@ -1076,7 +1090,7 @@ int rgf_zgemm_batched(int n, int nbatch, int opA, const double* A, int opB, cons
   if (nbatch == 0) return 0;
   if (!A || !B || !C) return fail(RGF_EINVAL, "NULL matrix");
   Variant v;
-  if (!pick_variant(n, &v)) return fail(RGF_EUNSUP, "n must be a multiple of 32 in [32, 256]");
+  if (!pick_variant(n, &v)) return fail(RGF_EUNSUP, "n must be a multiple of 32 in [32, 256] or of 64 in (256, 512]");
   const int grid = std::min(nbatch, 4096);
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   const double2* a = reinterpret_cast<const double2*>(A);
@@ -1098,7 +1112,7 @@ int rgf_zinv_batched(int n, int nbatch, const double* A, double* Ainv, int* info
   if (nbatch == 0) return 0;
   if (!A || !Ainv) return fail(RGF_EINVAL, "NULL matrix");
   Variant v;
-  if (!pick_variant(n, &v)) return fail(RGF_EUNSUP, "n must be a multiple of 32 in [32, 256]");
+  if (!pick_variant(n, &v)) return fail(RGF_EUNSUP, "n must be a multiple of 32 in [32, 256] or of 64 in (256, 512]");
   const int grid = std::min(nbatch, 1024);
   int dev = 0;
   HIPCHK(hipGetDevice(&dev));
@@ -1116,8 +1130,43 @@ int rgf_zinv_batched(int n, int nbatch, const double* A, double* Ainv, int* info
     case 128: hipLaunchKernelGGL((rgf::k_inv_batched<4, 128>), dim3(grid), dim3(rgf::Cfg<4>::NTH), 0, s, n, nbatch, a, o, w, info); break;
     case 192: hipLaunchKernelGGL((rgf::k_inv_batched<3, 192>), dim3(grid), dim3(rgf::Cfg<3>::NTH), 0, s, n, nbatch, a, o, w, info); break;
     case 256: hipLaunchKernelGGL((rgf::k_inv_batched<4, 256>), dim3(grid), dim3(rgf::Cfg<4>::NTH), 0, s, n, nbatch, a, o, w, info); break;
-    default: hipLaunchKernelGGL((rgf::k_inv_batched<1, 256>), dim3(grid), dim3(rgf::Cfg<1>::NTH), 0, s, n, nbatch, a, o, w, info); break;
+    default:
+      if (n <= 256)
+        hipLaunchKernelGGL((rgf::k_inv_batched<1, 256>), dim3(grid), dim3(rgf::Cfg<1>::NTH), 0, s, n, nbatch, a, o, w, info);
+      else if (v.T == 4)
+        hipLaunchKernelGGL((rgf::k_inv_batched<4, kNmaxBig>), dim3(grid), dim3(rgf::Cfg<4>::NTH), 0, s, n, nbatch, a, o, w, info);
+      else
+        hipLaunchKernelGGL((rgf::k_inv_batched<2, kNmaxBig>), dim3(grid), dim3(rgf::Cfg<2>::NTH), 0, s, n, nbatch, a, o, w, info);
+      break;
   }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int rgf_integrate(int nk, int nE, int nx, const double* E, const double* midgap, const double* dens_n,
+                  const double* dens_p, const double* tr, double scale, double* nelec, double* pelec,
+                  double* current, void* stream) {
+  g_err.clear();
+  if (nk < 1 || nx < 1) return fail(RGF_EINVAL, "rgf_integrate: need nk >= 1, nx >= 1");
+  if (nE < 2) return fail(RGF_EINVAL, "rgf_integrate: need nE >= 2 (dE = E[1] - E[0])");
+  if (!E) return fail(RGF_EINVAL, "rgf_integrate: E is required");
+  const bool dens = dens_n || dens_p;
+  if (dens && (!midgap || !nelec || !pelec))
+    return fail(RGF_EINVAL, "rgf_integrate: densities need midgap, nelec and pelec");
+  if (tr && !current) return fail(RGF_EINVAL, "rgf_integrate: tr needs current");
+  if (!dens && !tr) return 0;
+  // pi as the reference's module constant (src/green_rgf.f90:23): dE = (E2 - E1) / (2 pi), dK = 2 pi / nk
+  const double pi_ref = 3.14159265359;
+  rgf::IntegrateArgs a;
+  a.nE = nE; a.nk = nk; a.nx = nx;
+  a.E = E; a.midgap = midgap; a.dens_n = dens_n; a.dens_p = dens_p; a.tr = tr;
+  a.two_pi = 2.0 * pi_ref;
+  a.dK = 2.0 * pi_ref / nk;
+  a.scale = scale;
+  a.nelec = dens ? nelec : nullptr;
+  a.pelec = dens ? pelec : nullptr;
+  a.current = tr ? current : nullptr;
+  hipLaunchKernelGGL(rgf::k_integrate, dim3(nx + 1), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), a);
   HIPCHK(hipGetLastError());
   return 0;
 }

@@ -66,7 +66,7 @@ EXPORTS = ("rgf_params_reference", "rgf_last_error", "rgf_version", "rgf_rs", "r
            "rgf_rs_energies", "rgf_solve_batched", "rgf_workspace_bytes", "rgf_release",
            "rgf_zgemm_batched", "rgf_zinv_batched", "rgf_adjoint_batched", "rgf_adjoint_workspace_bytes",
            "rgf_scba_step", "rgf_graph_create", "rgf_graph_launch", "rgf_graph_destroy", "rgf_solve_devices",
-           "rgf_graph_create_devices", "rgf_cms", "rgf_cms_ex", "rgf_cms_energies")
+           "rgf_graph_create_devices", "rgf_cms", "rgf_cms_ex", "rgf_cms_energies", "rgf_integrate")
 
 _lib = None
 
@@ -88,6 +88,8 @@ def lib():
     L.rgf_rs_energies.argtypes = ([ctypes.POINTER(Params), vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp,
                                    vp, vp, vp, vp, vp, ctypes.c_int] + [vp] * 10)
     L.rgf_solve_batched.argtypes = [ctypes.POINTER(Params), ctypes.POINTER(Batch), vp]
+    L.rgf_integrate.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp, vp, ctypes.c_double,
+                                vp, vp, vp, vp]
     L.rgf_cms_ex.argtypes = [ctypes.POINTER(Params), vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, vp] + [vp] * 15
     L.rgf_cms_energies.argtypes = ([ctypes.POINTER(Params), vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp,
                                     vp, vp, vp, vp, vp, vp, ctypes.c_int] + [vp] * 10)
@@ -430,6 +432,55 @@ def adjoint_batched(p, E, temp, mu, Hii, H1i, sigL=None, sigR=None, w_tr=None, w
     _check(lib().rgf_adjoint_batched(ctypes.byref(p), ctypes.byref(b), pw, pn, pl, _tp(J), _tp(dU),
                                      _tp(dH), ctypes.c_void_p(stream.cuda_stream)))
     return J, dU, dH
+
+
+# constants of the reference's current sum (src/green_rgf.f90:18, 22, 24)
+E0_REF = 1.6022e-19
+HBAR_REF = 1.0546e-34
+TPI_REF = 3.14159265359 * 2.0
+
+
+def current_scale(spindeg=2.0, nk=1):
+    """scale of rgf_integrate's current for the reference's formula
+    sum(tr) * dE * e0 / tpi / hbar * e0 * spindeg / nk (src/green_rgf.f90:3189, 788): amperes."""
+    return E0_REF / TPI_REF / HBAR_REF * E0_REF * float(spindeg) / float(nk)
+
+
+def integrate(E, midgap=None, dens_n=None, dens_p=None, tr=None, scale=None, stream=None):
+    """Energy/k integrals of per-energy outputs on the GPU (rgf_integrate):
+    calc_n_electron's nelec/pelec (src/green_rgf.f90:3920-3945) and the current.
+    E float64 (nE,); dens_n / dens_p float64 (nk, nE, nx) or (nE, nx); tr (nk, nE)
+    or (nE,); midgap float64 (nx,).  scale defaults to current_scale(2, nk).
+    Returns (nelec, pelec, current) tensors (None where not computed)."""
+    import torch
+    dev = E.device
+    nE = int(E.shape[0])
+    ref = dens_n if dens_n is not None else dens_p
+    if ref is not None:
+        nx = int(ref.shape[-1])
+        nk = int(ref.numel() // (nE * nx))
+    elif tr is not None:
+        nx = 1
+        nk = int(tr.numel() // nE)
+    else:
+        raise RgfError("integrate: nothing to integrate")
+    if scale is None:
+        scale = current_scale(2.0, nk)
+    nelec = pelec = cur = None
+    if ref is not None:
+        nelec = torch.zeros(nx, dtype=torch.float64, device=dev)
+        pelec = torch.zeros(nx, dtype=torch.float64, device=dev)
+    if tr is not None:
+        cur = torch.zeros(1, dtype=torch.float64, device=dev)
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    _check(lib().rgf_integrate(nk, nE, nx, _req(E, "E", torch.float64, nE, dev),
+                               _req(midgap, "midgap", torch.float64, nx, dev) if ref is not None else None,
+                               _req(dens_n, "dens_n", torch.float64, nk * nE * nx, dev),
+                               _req(dens_p, "dens_p", torch.float64, nk * nE * nx, dev),
+                               _req(tr, "tr", torch.float64, nk * nE, dev), float(scale), _tp(nelec), _tp(pelec),
+                               _tp(cur), ctypes.c_void_p(stream.cuda_stream)))
+    return nelec, pelec, cur
 
 
 def _device_batches(torch, devices):

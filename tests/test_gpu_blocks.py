@@ -6,6 +6,8 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 SIZES = [32, 64, 96, 128, 160, 192, 256]
+# blocks above 256: several inverse panel rows per thread, 128 x 64 (n % 128 == 0) or 64 x 64 tiles
+BIG = [320, 384, 448, 512]
 
 
 def _env():
@@ -25,7 +27,7 @@ def _back(t):
     return np.swapaxes(t.cpu().numpy(), -1, -2)
 
 
-@pytest.mark.parametrize("n", SIZES)
+@pytest.mark.parametrize("n", SIZES + [320, 512])
 @pytest.mark.parametrize("opA,opB", [(0, 0), (0, 1), (1, 0), (1, 1)])
 def test_zgemm_batched(n, opA, opB):
     torch, rgf = _env()
@@ -43,7 +45,7 @@ def test_zgemm_batched(n, opA, opB):
         assert err < 1e-14, (n, opA, opB, rowmajor, err)
 
 
-@pytest.mark.parametrize("n", SIZES)
+@pytest.mark.parametrize("n", SIZES + BIG)
 def test_zinv_batched(n):
     torch, rgf = _env()
     rng = np.random.default_rng(n)
@@ -72,7 +74,7 @@ def test_zinv_singular_zero_and_info():
     assert np.all(Ainv.cpu().numpy() == 0)
 
 
-@pytest.mark.parametrize("n", [96, 128, 192, 256])
+@pytest.mark.parametrize("n", [96, 128, 192, 256, 384, 512])
 def test_zinv_singular_after_strip_updates(n):
     """A zero row is never a usable pivot, so the elimination runs through every
     strip update and reports the last column; the output is zeroed."""
@@ -87,3 +89,24 @@ def test_zinv_singular_after_strip_updates(n):
     assert np.all(Ainv[0].cpu().numpy() == 0)
     ref = np.linalg.inv(A[1])
     assert np.linalg.norm(_back(Ainv)[1] - ref) / np.linalg.norm(ref) < 1e-11
+
+
+@pytest.mark.parametrize("n", [320, 512])
+def test_zinv_pivots_from_rows_above_the_workgroup(n):
+    """Blocks above 256 rows hold rows tid and tid + 256 per thread: a matrix whose
+    column maxima sit in the upper rows (a row-reversed diagonally dominant
+    matrix) takes every pivot from the second row set, and a zero leading block
+    forces exchanges across it."""
+    torch, rgf = _env()
+    rng = np.random.default_rng(n + 1)
+    A = rng.standard_normal((2, n, n)) + 1j * rng.standard_normal((2, n, n))
+    A[0] += 8.0 * np.sqrt(n) * np.eye(n)
+    A[0] = A[0][::-1].copy()  # the dominant entry of column j is in row n - 1 - j
+    A[1, :16, :16] = 0.0
+    Ainv, info = rgf.zinv_batched(_colmajor(torch, A))
+    torch.cuda.synchronize()
+    assert np.all(info.cpu().numpy() == 0)
+    got = _back(Ainv)
+    for b in range(2):
+        ref = np.linalg.inv(A[b])
+        assert np.linalg.norm(got[b] - ref) / np.linalg.norm(ref) < 1e-11, (n, b)
