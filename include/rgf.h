@@ -166,6 +166,21 @@ typedef struct rgf_batch {
 
 int rgf_solve_batched(const rgf_params *p, const rgf_batch *b, void *stream);
 
+/* rgf_solve_batched without any allocation: no counterpart in the reference.
+ * It is for callers that cannot allocate device memory (code under HIP-graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, semantics and results as rgf_solve_batched, but the per-energy
+ * workspace (the 2*nx stored left-connected blocks plus temporaries of each
+ * energy slot) comes from a fixed device array of 1 GiB per GPU that the
+ * library holds until the process ends; the number of energies in flight is
+ * bounded by how many slots fit in it.  A batch whose single slot exceeds it
+ * ((2*nx + 30) * nm*nm*16 bytes > 1 GiB, or (2*nx + 36) blocks with
+ * off-diagonal Sigma) is rejected with RGF_EUNSUP before anything runs.  The
+ * workspace needs no initialisation.  Every call on one GPU shares that GPU's
+ * array: calls that overlap in time on one GPU (different streams or host
+ * threads without ordering) race and must be serialised by the caller. */
+int rgf_solve_batched_noalloc(const rgf_params *p, const rgf_batch *b, void *stream);
+
 /* One elastic self-consistent Born (SCBA) iteration: the batched solve of b
  * (RGF_MODE_PHYSICAL, per-energy Sigma with sig_stride = nm*nm*nx) that also
  * refreshes, in place and per energy, every Sigma block it was given:

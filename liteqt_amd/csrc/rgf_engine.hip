@@ -360,6 +360,51 @@ int solve_impl(const rgf_params* p, const rgf_batch* b, const rgf_scba* scba, vo
   return 0;
 }
 
+// ------------------------------------------------- allocation-free workspace
+// Workspace of rgf_solve_batched_noalloc: a fixed device array, so the entry
+// point allocates nothing.  Every GPU the library runs on holds its own copy
+// (kScratchBytes) until the process ends.  In host code `scratch` names a
+// host-side stand-in; its device address is resolved once per device with
+// hipGetSymbolAddress and only that address is handed to kernels.
+constexpr size_t kScratchBytes = size_t(1) << 30;
+static __device__ __align__(256) double2 scratch[kScratchBytes / sizeof(double2)];
+void* g_scratch[kMaxDev];
+std::mutex g_scratch_mu;
+
+int scratch_address(int dev, double2** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(RGF_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch[dev] = ptr;
+  }
+  *out = reinterpret_cast<double2*>(g_scratch[dev]);
+  return 0;
+}
+
+int solve_noalloc_impl(const rgf_params* p, const rgf_batch* b, void* stream) {
+  g_err.clear();
+  Prepared pr;
+  if (int rc = prepare(p, b, nullptr, &pr)) return rc;
+  if (pr.nothing) return 0;
+  const long long slot_bytes = pr.slot_elems * 16LL;
+  const long long fit = (long long)(kScratchBytes / (size_t)slot_bytes);
+  if (fit < 1)
+    return fail(RGF_EUNSUP, "rgf_solve_batched_noalloc: one energy slot needs " + std::to_string(slot_bytes) +
+                                " bytes ((2*nx + " + std::to_string(pr.slot_elems / ((long long)b->nm * b->nm) - 2 * b->nx) +
+                                ") blocks of nm*nm*16 bytes), more than the fixed workspace of " +
+                                std::to_string((long long)kScratchBytes) + " bytes");
+  double2* ptr = nullptr;
+  if (int rc = scratch_address(pr.dev, &ptr)) return rc;
+  pr.a.ws = ptr;
+  pr.v.launch(pr.a, (int)std::min(pr.slots, fit), reinterpret_cast<hipStream_t>(stream));
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+
 // ------------------------------------------------------------ many devices
 int mv_T(int v) {
   static const int t[rgf::MV_COUNT] = {1, 2, 1, 3, 1, 4, 2, 3, 1, 4, 2, 1};
@@ -610,6 +655,9 @@ int rgf_solve_batched(const rgf_params* p, const rgf_batch* b, void* stream) {
   return solve_impl(p, b, nullptr, stream);
 }
 
+int rgf_solve_batched_noalloc(const rgf_params* p, const rgf_batch* b, void* stream) {
+  return solve_noalloc_impl(p, b, stream);
+}
 
 int rgf_scba_step(const rgf_params* p, const rgf_batch* b, const rgf_scba* scba, void* stream) {
   if (!scba) {

@@ -66,7 +66,8 @@ EXPORTS = ("rgf_params_reference", "rgf_last_error", "rgf_version", "rgf_rs", "r
            "rgf_rs_energies", "rgf_solve_batched", "rgf_workspace_bytes", "rgf_release",
            "rgf_zgemm_batched", "rgf_zinv_batched", "rgf_adjoint_batched", "rgf_adjoint_workspace_bytes",
            "rgf_scba_step", "rgf_graph_create", "rgf_graph_launch", "rgf_graph_destroy", "rgf_solve_devices",
-           "rgf_graph_create_devices", "rgf_cms", "rgf_cms_ex", "rgf_cms_energies", "rgf_integrate")
+           "rgf_graph_create_devices", "rgf_cms", "rgf_cms_ex", "rgf_cms_energies", "rgf_solve_batched_noalloc",
+           "rgf_integrate")
 
 _lib = None
 
@@ -88,6 +89,7 @@ def lib():
     L.rgf_rs_energies.argtypes = ([ctypes.POINTER(Params), vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp,
                                    vp, vp, vp, vp, vp, ctypes.c_int] + [vp] * 10)
     L.rgf_solve_batched.argtypes = [ctypes.POINTER(Params), ctypes.POINTER(Batch), vp]
+    L.rgf_solve_batched_noalloc.argtypes = [ctypes.POINTER(Params), ctypes.POINTER(Batch), vp]
     L.rgf_integrate.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp, vp, ctypes.c_double,
                                 vp, vp, vp, vp]
     L.rgf_cms_ex.argtypes = [ctypes.POINTER(Params), vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, vp] + [vp] * 15
