@@ -249,6 +249,56 @@ def scba_cost(rgf, torch, dev, nE, iters):
             "current_e_over_h_per_iteration": cur, "singular": int(info[:, 3].sum())}
 
 
+def small_block_cost(rgf, torch, dev, steps=3):
+    """The small-block workload: the (8,0) carbon nanotube, pz tight binding (t = -2.7 eV,
+    32 orbitals per 4.26 A cell), nx = 100 cells with a linear potential drop of 0.2 eV,
+    1000 energies in [-3, 3] eV, eta 1e-4, ballistic (no scattering), T(E) with
+    f_L = 1, f_R = 0, per-slab densities; compat mode.  One persistent launch per step,
+    timed with HIP events on its stream."""
+    from liteqt_amd import configs
+    rgf.lib().rgf_release()
+    cfg = CONFIGS["cnt32"]
+    nm, nx, nE = cfg["nm"], cfg["nx"], cfg["nE"]
+    Hii, H1i = configs.cnt_80_ballistic(L=nx)
+    E = configs.energy_grid(cfg["emin"], cfg["emax"], nE)
+    p = rgf.params(rgf.MODE_COMPAT, eta_dev=cfg["eta"], eta_lead=cfg["eta"], dec_tol=1e-12, dec_nmax=100)
+    Hd, Cd = rgf.to_device_blocks(Hii, dev), rgf.to_device_blocks(H1i, dev)
+    Ed = torch.tensor(E, dtype=torch.float64, device=dev)
+    out = {"tr": torch.zeros(nE, dtype=torch.float64, device=dev),
+           "dens_n": torch.zeros(nE, nx, dtype=torch.float64, device=dev),
+           "info": torch.zeros(nE, 4, dtype=torch.int32, device=dev)}
+    stream = torch.cuda.current_stream()
+    temp, mu = (300.0, 300.0), (100.0, -100.0)  # f_L = 1, f_R = 0: tr = T(E)
+    rgf.solve_batched(p, Ed, temp, mu, Hd, Cd, out=out, stream=stream)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(steps):
+        rgf.solve_batched(p, Ed, temp, mu, Hd, Cd, out=out, stream=stream)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / steps
+    ms = ev0.elapsed_time(ev1) / steps
+    info = out["info"].cpu().numpy()
+    flops = algorithmic_flops(nm, nx, info)
+    tr = out["tr"].cpu().numpy()
+    achieved = flops / (ms * 1e-3) / 1e12
+    print(f"bench: cnt32 (nm={nm} nx={nx} nE={nE}) {ms:.1f} ms per launch", file=sys.stderr, flush=True)
+    res = {"workload": cfg["workload"] + "; T(E) with f_L = 1, f_R = 0, per-slab densities",
+           "energy_points_per_s": nE / wall, "ms_per_launch": ms, "achieved_tflops": achieved,
+           "frac": achieved / PEAK_FP64_MFMA, "flops_per_launch": flops,
+           "decimation_iters_mean": float(info[:, :2].mean()), "singular": int(info[:, 3].sum()),
+           "T_max": float(tr.max()), "T_at_E0": float(tr[np.argmin(np.abs(E))])}
+    tfile = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tfile):
+        tinfo = json.load(open(tfile)).get("cnt32")
+        if tinfo and tinfo.get("kernel_hash") == kernel_source_hash():
+            res["hbm_bytes_per_launch"] = tinfo["bytes_per_energy"] * nE
+            res["hbm_gb_per_s"] = tinfo["bytes_per_energy"] * nE / (ms * 1e-3) / 1e9
+    return res
+
+
 def random_device_torch(torch, nm, nx, gen, dev, u_range=0.3, coupling_sigma=0.5):
     """Random device generated on the GPU with the distribution of
     liteqt_amd.configs.random_device: H_ll = (A + A^H)/(2 sqrt(nm)) + U_l I with
@@ -430,6 +480,7 @@ def main():
     ap.add_argument("--no-scba", action="store_true", help="skip the SCBA-step (off-diagonal Sigma) measurement")
     ap.add_argument("--scba-energies", type=int, default=4096)
     ap.add_argument("--scba-iterations", type=int, default=10)
+    ap.add_argument("--no-small", action="store_true", help="skip the small-block (CNT nm=32) measurement")
     ap.add_argument("--no-devices", action="store_true", help="skip the 64-device one-launch measurement")
     ap.add_argument("--devices", type=int, default=64)
     ap.add_argument("--devices-compare", action="store_true",
@@ -543,6 +594,9 @@ def main():
     scba = None
     if not args.no_scba:
         scba = scba_cost(rgf, torch, dev, args.scba_energies, args.scba_iterations)
+    small = None
+    if not args.no_small and args.config != "cnt32":
+        small = small_block_cost(rgf, torch, dev)
     devices = None
     if not args.no_devices:
         devices = devices_cost(rgf, torch, dev, args.devices, args.devices_energy_div, args.devices_compare)
@@ -576,6 +630,7 @@ def main():
         "adjoint": adjoint,
         "adjoint_over_forward": adjoint["ratio"] if adjoint else None,
         "scba_step": scba,
+        "small_blocks": small,
         "devices_one_launch": devices,
         "decimation_iters_mean": float(info[:, :2].mean()),
         "sane": ok,
