@@ -162,6 +162,15 @@ typedef struct rgf_batch {
    * green_RGF_CMS solve (src/green_rgf.f90:4167-4383); NULL for green_RGF_RS.
    * Needs RGF_MODE_COMPAT and no off-diagonal Sigma. */
   const double *Sii;
+  /* Storage of the left-connected blocks g_l, g^<_l between the forward and the
+   * backward sweep (diagonal-Sigma solve): -1 keeps every slab's blocks (2*nx
+   * blocks per energy in flight); K > 0 keeps them every K slabs and recomputes
+   * each K-slab segment from its checkpoint before the backward sweep enters it
+   * (2*((nx-2)/K + 1) + 2*K blocks; one more forward sweep, results bit-identical);
+   * 0 (zero-initialised struct) keeps every slab unless the slots that fit in
+   * free device memory leave CUs idle and checkpoints every ceil(sqrt(nx)) slabs
+   * run more energies per unit of time. */
+  int ckpt;
 } rgf_batch;
 
 int rgf_solve_batched(const rgf_params *p, const rgf_batch *b, void *stream);

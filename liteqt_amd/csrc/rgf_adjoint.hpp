@@ -340,12 +340,11 @@ __device__ void adjoint_energy(const AdjArgs& a, int e, double2* ws, double2* dH
 }
 
 template <int T, int NMAX>
-__global__ __launch_bounds__(Cfg<T>::NTH, 1) void rgf_adjoint(AdjArgs a) {
+__global__ __launch_bounds__(Cfg<T>::NTH, Cfg<T>::OCC) void rgf_adjoint(AdjArgs a) {
   __shared__ double lds[SweepLds<T, NMAX>::DOUBLES];
-  __shared__ double red[16];
   double2* ws = a.ws + (size_t)blockIdx.x * (size_t)a.ws_slot;
   double2* dH = a.dH + (size_t)blockIdx.x * a.nx * (size_t)a.n * a.n;
-  for (int e = blockIdx.x; e < a.nE; e += gridDim.x) adjoint_energy<T, NMAX>(a, e, ws, dH, (ldsd*)lds, (ldsd*)red);
+  for (int e = blockIdx.x; e < a.nE; e += gridDim.x) adjoint_energy<T, NMAX>(a, e, ws, dH, (ldsd*)lds, (ldsd*)lds);
 }
 
 // Sum the per-slot coupling adjoints: out[l] = sum_s dH[s][l] for l = 1..nx-2, zero at the edges.

@@ -218,14 +218,19 @@ __device__ __forceinline__ void block_argmax(double& v, int& idx, ldsd* redv, ld
 }
 
 // ------------------------------------------------------------ tile configs
-// Per block-size variant T: a pass computes an MR x MC macro tile of C with NW
-// waves in a WR x WC grid, each wave owning TA x TB MFMA tiles of 16x16
-// complex (2*TA*TB accumulators of 4 doubles).  T = 4 (n = 128, 256) uses a
-// 128 x 64 macro tile: 64 x 32 per wave keeps a wave at 128 accumulator
+// Per block-size variant V = T + 8 (OCC - 1): a pass computes an MR x MC macro
+// tile of C with NW waves in a WR x WC grid, each wave owning TA x TB MFMA tiles
+// of 16x16 complex (2*TA*TB accumulators of 4 doubles).  T = 4 (n = 128, 256)
+// uses a 128 x 64 macro tile: 64 x 32 per wave keeps a wave at 128 accumulator
 // registers, so nothing spills with one wave per SIMD.  Smaller T use square
-// (32T)^2 tiles.
-template <int T>
+// (32T)^2 tiles.  OCC is the number of workgroups per CU the variant's kernels
+// are compiled for (their launch bounds): the same code at OCC = 2 is a
+// separate instantiation of every block function, so the register budget of
+// two workgroups per CU applies to it alone.
+template <int V>
 struct Cfg {
+  static constexpr int T = V & 7;
+  static constexpr int OCC = 1 + (V >> 3);
   static constexpr int MR = 32 * T;
   static constexpr int MC = (T == 4) ? 64 : 32 * T;
   static constexpr int NW = 4;
@@ -335,7 +340,7 @@ __device__ __forceinline__ void mma_steps(d4 (&cr)[TA][TB], d4 (&ci)[TA][TB],
 // value in every thread).
 template <int T>
 struct GemmLds {
-  static constexpr int DOUBLES = 2 * Cfg<T>::STAGE + 8;  // two stages + reduction scratch
+  static constexpr int DOUBLES = 2 * Cfg<T>::STAGE;  // two stages (the abs-sum reduction reuses stage 0)
 };
 
 // Byte offset of each of a thread's panel elements from the panel origin.
@@ -439,7 +444,8 @@ __device__ __noinline__ double wg_gemm_k(int n_, const double2* __restrict__ A_,
   ep.D3 = uni(ep_.D3); ep.s3 = uni(ep_.s3);
   ep.real_out = uni(ep_.real_out); ep.abs_sum = uni(ep_.abs_sum);
   const int emode = (ep.D2 || ep.D3 || ep.real_out || ep.abs_sum) ? 2 : (ep.D1 ? 1 : 0);
-  ldsd* red = lds + 2 * STAGE;
+  // the abs-sum reduction runs after the last panel's barrier, when stage 0 is free
+  ldsd* red = lds;
   const ldsd* s0Ar = lds;
   const ldsd* s0Ai = s0Ar + KP * LDA;
   const ldsd* s0Br = s0Ai + KP * LDA;
@@ -1116,20 +1122,148 @@ __device__ __noinline__ int wg_inverse_big(int n, double2* W_, double2* out_, ld
   return 0;
 }
 
-template <int T, int NMAX>
-struct InvLds {
-  static constexpr int DOUBLES =
-      NMAX <= Cfg<T>::NTH ? InvSmallLds<T, NMAX>::DOUBLES : InvBigLds<T, NMAX>::DOUBLES;
+// ------------------------------------------------ inverse, n = 32 or 64
+// The whole matrix held in registers: thread t owns row i = t % N and the C = N*N / NTH
+// columns cb*C .. cb*C + C - 1, cb = t / N (16 columns at N = 64, 4 at N = 32).  The
+// same Gauss-Jordan sweep with the same pivot rule and logical row exchanges as the
+// blocked inverse, unblocked: per column k the threads holding column k (one wave at
+// N = 64, half a wave at N = 32) find the pivot row p and publish column k, the
+// threads of row p publish row p, and every thread updates its elements in registers.
+// Two barriers per column and no global memory between the load of W and the store
+// of the inverse, where the blocked inverse at these sizes spends most of its time
+// waiting on the loads and stores of each panel update.  Column and row exchange
+// buffers alternate by column parity, so a column's reads never meet the next
+// column's writes.
+template <int T, int N>
+struct InvRegLds {
+  static constexpr int LDO = N + 1;           // odd row stride of the output staging tile
+  static constexpr int XCH = 2 * 4 * N;       // [parity][col re, col im, row re, row im][N]
+  static constexpr int OUT = 2 * N * LDO;     // re / im planes of the matrix for the final copy
+  static constexpr int AREA = XCH > OUT ? XCH : OUT;
+  static constexpr int DOUBLES = AREA + N + 8;  // + piv, pinv (2 N ints) + pivot slots
 };
 
-// Inverse of an n x n block (n <= NMAX): one panel row per thread up to NTH rows,
-// several rows per thread with the panels staged through `out` above that.
+template <int T, int N>
+__device__ __noinline__ int wg_inverse_reg(double2* W_, double2* out_, ldsd* lds) {
+  using L = InvRegLds<T, N>;
+  constexpr int NTH = Cfg<T>::NTH, C = N * N / NTH, LDO = L::LDO;
+  static_assert(NTH == 256 && (N == 32 || N == 64), "register inverse: 256 threads, N = 32 or 64");
+  gd2* W = gw(W_);
+  gd2* out = gw(out_);
+  ldsi* piv = (ldsi*)(lds + L::AREA);
+  ldsi* pinv = piv + N;
+  ldsi* pvs = pinv + N;  // [parity] pivot row of the current column (-1: none)
+  const int tid = threadIdx.x;
+  const int i = tid % N, cb = tid / N;
+  double2 a[C];
+#pragma unroll
+  for (int c = 0; c < C; ++c) a[c] = W[(size_t)i * N + cb * C + c];
+  bool myused = false;
+  int info = 0;
+  const int wave = tid >> 6;
+  for (int k = 0; k < N; ++k) {
+    ldsd* X = lds + (k & 1) * (4 * N);
+    const int kb = k / C, kc = k % C;
+    const bool mine = (cb == kb);
+    // (a) the wave holding column k finds the pivot row and publishes the column
+    if (wave == (kb * N) / 64) {
+      double2 v = a[0];
+#pragma unroll
+      for (int c = 1; c < C; ++c)
+        if (c == kc) v = a[c];
+      const double m = (mine && !myused) ? fabs(v.x) + fabs(v.y) : -1.0;
+      const double best = wave_max(m);
+      const unsigned long long hit = __ballot(mine && !myused && m == best);
+      if (mine) {
+        X[i] = v.x;
+        X[N + i] = v.y;
+        if (i == 0) {
+          const int p = hit ? (int)(__builtin_ctzll(hit) % N) : -1;  // lowest row among equal maxima
+          pvs[k & 1] = (best > 0.0 && hit) ? p : -1;
+          piv[k] = p;
+        }
+      }
+    }
+    __syncthreads();
+    const int p = pvs[k & 1];
+    if (p < 0) {
+      info = k + 1;
+      break;
+    }
+    // (b) the threads of row p normalise their part of the pivot row and publish it
+    if (i == p) {
+      const double2 ainv = crecip(make_double2(X[p], X[N + p]));
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        const double2 r = (cb * C + c == k) ? ainv : cmul(a[c], ainv);
+        a[c] = r;
+        X[2 * N + cb * C + c] = r.x;
+        X[3 * N + cb * C + c] = r.y;
+      }
+      myused = true;
+    }
+    __syncthreads();
+    // (c) every other row: a -= a_ik * (pivot row), column k starting from zero
+    if (i != p) {
+      const double2 f = make_double2(X[i], X[N + i]);
+      double2 rp[C];
+#pragma unroll
+      for (int c = 0; c < C; ++c) rp[c] = make_double2(X[2 * N + cb * C + c], X[3 * N + cb * C + c]);
+      if (mine) {
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+          if (c == kc) a[c] = make_double2(0, 0);
+      }
+#pragma unroll
+      for (int c = 0; c < C; ++c) a[c] = csub(a[c], cmul(f, rp[c]));
+    }
+  }
+  if (info) {
+    for (int idx = tid; idx < N * N; idx += NTH) out[idx] = make_double2(0, 0);
+    __syncthreads();
+    return info;
+  }
+  __syncthreads();  // the last column's exchange reads are done before the staging tile is written
+  ldsd* Lr = lds;
+  ldsd* Li = lds + N * LDO;
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    Lr[i * LDO + cb * C + c] = a[c].x;
+    Li[i * LDO + cb * C + c] = a[c].y;
+  }
+  for (int q = tid; q < N; q += NTH) pinv[piv[q]] = q;
+  __syncthreads();
+  // out(r, q) = W[piv[r]][pinv[q]], r fastest across the lanes: coalesced column-major stores
+  for (int idx = tid; idx < N * N; idx += NTH) {
+    const int r = idx % N, q = idx / N;
+    const int src = piv[r] * LDO + pinv[q];
+    out[idx] = make_double2(Lr[src], Li[src]);
+  }
+  __syncthreads();
+  return 0;
+}
+
+template <int T, int NMAX>
+struct InvLds {
+  static constexpr int A =
+      NMAX <= Cfg<T>::NTH ? InvSmallLds<T, NMAX>::DOUBLES : InvBigLds<T, NMAX>::DOUBLES;
+  static constexpr int B = (NMAX == 32 || NMAX == 64) ? InvRegLds<T, NMAX>::DOUBLES : 0;
+  static constexpr int DOUBLES = A > B ? A : B;
+};
+
+// Inverse of an n x n block (n <= NMAX): the whole matrix in registers when n = NMAX is
+// 32 or 64, one panel row per thread up to NTH rows, several rows per thread with the
+// panels staged through `out` above that.
 template <int T, int NMAX>
 __device__ __forceinline__ int wg_inverse(int n, double2* W, double2* out, ldsd* lds) {
-  if constexpr (NMAX <= Cfg<T>::NTH)
+  if constexpr (NMAX == 32 || NMAX == 64) {
+    if (n == NMAX) return wg_inverse_reg<T, NMAX>(W, out, lds);
     return wg_inverse_small<T, NMAX>(n, W, out, lds);
-  else
+  } else if constexpr (NMAX <= Cfg<T>::NTH) {
+    return wg_inverse_small<T, NMAX>(n, W, out, lds);
+  } else {
     return wg_inverse_big<T, NMAX>(n, W, out, lds);
+  }
 }
 
 }  // namespace rgf

@@ -4,6 +4,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -166,6 +169,11 @@ struct Variant {
 // Largest block size of the plain sweep, and the granule above 256 (blocks
 // above 256 run with several inverse panel rows per thread).
 constexpr int kNmaxBig = 512;
+// Tile-variant offset of the kernels compiled for two workgroups per CU (the
+// 32- and 64-orbital blocks, whose single 256-thread workgroup leaves the MFMA
+// pipes idle during the latency-bound column steps of the inverse and the short
+// K loops of the GEMM).
+constexpr int kOcc2 = 8;
 
 // Tile variant per block size: wave tile (16T)^2, macro tile (32T)^2; the
 // inverse's LDS panel is sized for NMAX rows.  Above 256: n a multiple of 64,
@@ -178,8 +186,8 @@ bool pick_variant(int n, Variant* v) {
   }
   if (n % 32 != 0 || n < 32 || n > 256) return false;
   switch (n) {
-    case 32: *v = {launch_sweep<1, 32>, sweep_fn<1, 32>(), 1, 32, rgf::Cfg<1>::NTH}; return true;
-    case 64: *v = {launch_sweep<2, 64>, sweep_fn<2, 64>(), 2, 64, rgf::Cfg<2>::NTH}; return true;
+    case 32: *v = {launch_sweep<kOcc2 + 1, 32>, sweep_fn<kOcc2 + 1, 32>(), 1, 32, rgf::Cfg<1>::NTH}; return true;
+    case 64: *v = {launch_sweep<kOcc2 + 2, 64>, sweep_fn<kOcc2 + 2, 64>(), 2, 64, rgf::Cfg<2>::NTH}; return true;
     case 96: *v = {launch_sweep<3, 96>, sweep_fn<3, 96>(), 3, 96, rgf::Cfg<3>::NTH}; return true;
     case 128: *v = {launch_sweep<4, 128>, sweep_fn<4, 128>(), 4, 128, rgf::Cfg<4>::NTH}; return true;
     case 192: *v = {launch_sweep<3, 192>, sweep_fn<3, 192>(), 3, 192, rgf::Cfg<3>::NTH}; return true;
@@ -192,8 +200,8 @@ bool pick_variant(int n, Variant* v) {
 bool pick_variant_od(int n, Variant* v) {
   if (n % 32 != 0 || n < 32 || n > 256) return false;
   switch (n) {
-    case 32: *v = {launch_sweep_od<1, 32>, sweep_od_fn<1, 32>(), 1, 32, rgf::Cfg<1>::NTH}; return true;
-    case 64: *v = {launch_sweep_od<2, 64>, sweep_od_fn<2, 64>(), 2, 64, rgf::Cfg<2>::NTH}; return true;
+    case 32: *v = {launch_sweep_od<kOcc2 + 1, 32>, sweep_od_fn<kOcc2 + 1, 32>(), 1, 32, rgf::Cfg<1>::NTH}; return true;
+    case 64: *v = {launch_sweep_od<kOcc2 + 2, 64>, sweep_od_fn<kOcc2 + 2, 64>(), 2, 64, rgf::Cfg<2>::NTH}; return true;
     case 96: *v = {launch_sweep_od<3, 96>, sweep_od_fn<3, 96>(), 3, 96, rgf::Cfg<3>::NTH}; return true;
     case 128: *v = {launch_sweep_od<4, 128>, sweep_od_fn<4, 128>(), 4, 128, rgf::Cfg<4>::NTH}; return true;
     case 192: *v = {launch_sweep_od<3, 192>, sweep_od_fn<3, 192>(), 3, 192, rgf::Cfg<3>::NTH}; return true;
@@ -216,7 +224,9 @@ bool pick_variant_cms(int n, Variant* v) {
   }
 }
 
-long long slot_elems(int n, int nx) { return (long long)(2 * nx + rgf::kTmpBlocks) * n * n; }
+long long slot_elems(int n, int nx, int ckpt = 0) {
+  return rgf::sweep_slot_blocks(nx, ckpt, rgf::kTmpBlocks) * n * n;
+}
 long long slot_elems_od(int n, int nx) { return (long long)(2 * nx + rgf::kTmpBlocksOd) * n * n; }
 
 bool has_offdiag(const rgf_batch* b) { return b->sigR_lo || b->sigR_up || b->sigL_up; }
@@ -260,6 +270,9 @@ int prepare(const rgf_params* p, const rgf_batch* b, const rgf_scba* scba, Prepa
   // The overlap kernel (rgf_sweep_cms) is switched off: its first run at nm = 64
   // ended in a GPU memory fault whose cause has not been found.
   if (cms) return fail(RGF_EUNSUP, "the overlap (green_RGF_CMS) GPU solve is switched off (unresolved memory fault)");
+  if (b->ckpt < -1) return fail(RGF_EINVAL, "ckpt must be -1 (keep every slab), 0 (automatic) or a segment length");
+  if (b->ckpt > 0 && (od || cms))
+    return fail(RGF_EUNSUP, "segment recompute (ckpt > 0) is for the diagonal-Sigma solve only");
   if (scba) {
     if (!b->sigR || !b->sigL) return fail(RGF_EINVAL, "SCBA refresh: sigR and sigL are required");
     if (b->sig_stride != (long long)b->nm * b->nm * b->nx)
@@ -277,7 +290,7 @@ int prepare(const rgf_params* p, const rgf_batch* b, const rgf_scba* scba, Prepa
   int per_cu = 0;
   HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, v.fn, v.nth, 0));
   if (per_cu < 1) return fail(RGF_EHIP, "sweep kernel does not fit on a CU");
-  const long long se = od ? slot_elems_od(b->nm, b->nx) : slot_elems(b->nm, b->nx);
+  const long long se = od ? slot_elems_od(b->nm, b->nx) : slot_elems(b->nm, b->nx, std::max(b->ckpt, 0));
   long long slots = std::min<long long>(b->nE, (long long)prop.multiProcessorCount * per_cu);
   if (b->max_slots > 0) slots = std::min<long long>(slots, b->max_slots);
   out->slots = slots;
@@ -332,6 +345,7 @@ void fill_args(const rgf_params* p, const rgf_batch* b, long long se, rgf::Sweep
   a.sigR_up = reinterpret_cast<const double2*>(b->sigR_up);
   a.sigL_up = reinterpret_cast<const double2*>(b->sigL_up);
   a.Sii = reinterpret_cast<const double2*>(b->Sii);
+  a.ckpt = std::max(b->ckpt, 0);
 }
 
 int solve_impl(const rgf_params* p, const rgf_batch* b, const rgf_scba* scba, void* stream) {
@@ -339,7 +353,7 @@ int solve_impl(const rgf_params* p, const rgf_batch* b, const rgf_scba* scba, vo
   Prepared pr;
   if (int rc = prepare(p, b, scba, &pr)) return rc;
   if (pr.nothing) return 0;
-  const long long slot_bytes = pr.slot_elems * 16LL;
+  long long slot_bytes = pr.slot_elems * 16LL;
   long long slots = pr.slots;
   std::lock_guard<std::mutex> lk(g_ws_mu);
   Buf& ws = g_ws[pr.dev];
@@ -348,11 +362,29 @@ int solve_impl(const rgf_params* p, const rgf_batch* b, const rgf_scba* scba, vo
     size_t fr = 0, tot = 0;
     HIPCHK(hipMemGetInfo(&fr, &tot));
     const size_t avail = (size_t)((double)(fr + ws.bytes) * 0.9);
-    const long long fit = (long long)(avail / (size_t)slot_bytes);
+    long long fit = (long long)(avail / (size_t)slot_bytes);
+    // automatic segment recompute: when the full slots leave CUs idle, keep the
+    // blocks only at checkpoints every ceil(sqrt(nx)) slabs and one segment, at the
+    // cost of one more forward sweep (7 of the 17 GEMM-equivalents of a slab), if
+    // that runs more energies per unit of time
+    const bool od = has_offdiag(b) || scba;
+    if (b->ckpt == 0 && !od && !b->Sii && fit < slots) {
+      const int K = std::max(2, (int)std::ceil(std::sqrt((double)b->nx)));
+      const long long se_k = slot_elems(b->nm, b->nx, K);
+      const long long fit_k = std::min<long long>(slots, (long long)(avail / (size_t)(se_k * 16LL)));
+      if (fit_k * 17 > fit * 24) {
+        pr.a.ckpt = K;
+        pr.a.ws_slot = se_k;
+        slot_bytes = se_k * 16LL;
+        fit = fit_k;
+      }
+    }
     if (fit < 1) return fail(RGF_ENOMEM, "not enough device memory for one energy slot");
     slots = std::min(slots, fit);
-    ws.release();
-    if (int rc = ws.ensure((size_t)(slots * slot_bytes))) return rc;
+    if ((size_t)(slots * slot_bytes) > ws.bytes) {
+      ws.release();
+      if (int rc = ws.ensure((size_t)(slots * slot_bytes))) return rc;
+    }
   }
   pr.a.ws = reinterpret_cast<double2*>(ws.p);
   pr.v.launch(pr.a, (int)slots, reinterpret_cast<hipStream_t>(stream));
@@ -860,6 +892,37 @@ int rgf_cms(const double temp[2], int nm, int nx, double E, const double mu[2], 
                     GRi1, GLi1, GGi1);
 }
 
+// A per-energy host array of a host-array solve: energy e at base + e * stride (the
+// drivers' (nm, nm, nx, nen) arrays), or at list[e] (separate per-energy calls
+// gathered into one batch).
+struct EArr {
+  const char* base = nullptr;
+  size_t stride = 0;
+  std::vector<const char*> list;
+  bool present() const { return base || !list.empty(); }
+  bool strided() const { return list.empty(); }
+  char* at(long long e) const { return const_cast<char*>(strided() ? base + stride * (size_t)e : list[(size_t)e]); }
+};
+
+inline EArr arr1(const void* p) {
+  EArr a;
+  a.base = (const char*)p;
+  return a;
+}
+
+// The host arrays of one solve.  sigL/sigR either per energy or one shared slab set.
+struct HostIO {
+  int nE = 0;
+  const double* E = nullptr;
+  EArr sigL, sigR;
+  bool sig_per_energy = false;
+  EArr out[6];  // ldos, ndens, pdens, GRi1, GLi1, GGi1
+  EArr cur, tr, tre, info;
+};
+
+static int rs_core(const rgf_params* p, const double temp[2], int nm, int nx, const double mu[2], const double* Sii,
+                   const double* Hii, const double* H1i, const HostIO& io);
+
 static int rs_energies_impl(const rgf_params* p, const double temp[2], int nm, int nx, int nE, const double* E,
                             const double mu[2], const double* Sii, const double* Hii, const double* H1i,
                             const double* sigL, const double* sigG, const double* sigR, int sig_per_energy,
@@ -871,6 +934,33 @@ static int rs_energies_impl(const rgf_params* p, const double temp[2], int nm, i
   if (nm < 1 || nx < 2 || nE < 0) return fail(RGF_EINVAL, "need nm >= 1, nx >= 2, nE >= 0");
   if (!temp || !mu || !Hii || !H1i || (nE > 0 && !E)) return fail(RGF_EINVAL, "missing required array");
   if (nE == 0) return 0;
+  const size_t cslab_u = (size_t)nm * nm * nx * 16, rslab_u = (size_t)nm * nm * nx * 8;
+  HostIO io;
+  io.nE = nE;
+  io.E = E;
+  io.sig_per_energy = sig_per_energy != 0;
+  auto arr = [](const void* b, size_t stride) {
+    EArr a;
+    a.base = (const char*)b;
+    a.stride = stride;
+    return a;
+  };
+  io.sigL = arr(sigL, sig_per_energy ? cslab_u : 0);
+  io.sigR = arr(sigR, sig_per_energy ? cslab_u : 0);
+  double* outs[6] = {ldos, ndens, pdens, GRi1, GLi1, GGi1};
+  for (int k = 0; k < 6; ++k) io.out[k] = arr(outs[k], cslab_u);
+  io.cur = arr(cur, rslab_u);
+  io.tr = arr(tr, 8);
+  io.tre = arr(tre, 8);
+  io.info = arr(info, 16);
+  return rs_core(p, temp, nm, nx, mu, Sii, Hii, H1i, io);
+}
+
+// Stage the inputs in HBM (padded to the engine's block granule), run the batched
+// engine chunk by chunk over the energies, copy (and unpad) the outputs back.
+static int rs_core(const rgf_params* p, const double temp[2], int nm, int nx, const double mu[2], const double* Sii,
+                   const double* Hii, const double* H1i, const HostIO& io) {
+  const int nE = io.nE;
   // blocks are padded to the engine's granule: 32 up to 256, 64 above (overlaps: up to 256)
   const int np = nm <= 256 ? (nm + 31) / 32 * 32 : (nm + 63) / 64 * 64;
   if (np > (Sii ? 256 : kNmaxBig))
@@ -883,16 +973,17 @@ static int rs_energies_impl(const rgf_params* p, const double temp[2], int nm, i
   const size_t cslab = bp * nx * 16, rslab = bp * nx * 8;       // padded, per energy
   const size_t cslab_u = bn * nx * 16, rslab_u = bn * nx * 8;   // caller layout, per energy
   const bool padded = (np != nm);
-  double* outs[6] = {ldos, ndens, pdens, GRi1, GLi1, GGi1};
+  const bool sig_pe = io.sig_per_energy;
   int nout = 0;
-  for (double* o : outs) nout += (o != nullptr);
-  const int nsig = (sigL ? 1 : 0) + (sigR ? 1 : 0);
+  for (int k = 0; k < 6; ++k) nout += io.out[k].present();
+  const bool has_cur = io.cur.present();
+  const int nsig = (io.sigL.present() ? 1 : 0) + (io.sigR.present() ? 1 : 0);
   // bytes per energy in a chunk, then chunk size from free memory
-  const size_t per_e = (size_t)nout * cslab + (cur ? rslab : 0) + (sig_per_energy ? nsig * cslab : 0) +
+  const size_t per_e = (size_t)nout * cslab + (has_cur ? rslab : 0) + (sig_pe ? nsig * cslab : 0) +
                        (padded ? cslab_u : 0) + 64;
   size_t fr = 0, tot = 0;
   HIPCHK(hipMemGetInfo(&fr, &tot));
-  const size_t fixed = (Sii ? 3 : 2) * cslab + (sig_per_energy ? 0 : nsig * cslab) + (padded ? cslab_u : 0);
+  const size_t fixed = (Sii ? 3 : 2) * cslab + (sig_pe ? 0 : nsig * cslab) + (padded ? cslab_u : 0);
   const size_t ws_reserve = (size_t)rgf_workspace_bytes(np, nx, 256);
   const size_t budget = (size_t)((double)(fr + g_stage[dev].bytes) * 0.45);
   long long C = nE;
@@ -903,11 +994,11 @@ static int rs_energies_impl(const rgf_params* p, const double temp[2], int nm, i
   size_t off = 0;
   auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) / 256 * 256; return o; };
   const size_t oH = take(cslab), oC = take(cslab), oS = Sii ? take(cslab) : 0;
-  const size_t sig_n = sig_per_energy ? (size_t)C : 1;
-  const size_t oSL = sigL ? take(cslab * sig_n) : 0, oSR = sigR ? take(cslab * sig_n) : 0;
+  const size_t sig_n = sig_pe ? (size_t)C : 1;
+  const size_t oSL = io.sigL.present() ? take(cslab * sig_n) : 0, oSR = io.sigR.present() ? take(cslab * sig_n) : 0;
   size_t oOut[6];
-  for (int k = 0; k < 6; ++k) oOut[k] = outs[k] ? take(cslab * C) : 0;
-  const size_t oCur = cur ? take(rslab * C) : 0;
+  for (int k = 0; k < 6; ++k) oOut[k] = io.out[k].present() ? take(cslab * C) : 0;
+  const size_t oCur = has_cur ? take(rslab * C) : 0;
   const size_t oE = take(8 * C), oTr = take(8 * C), oTre = take(8 * C), oInfo = take(16 * C);
   const size_t oU = padded ? take(cslab_u * std::max<size_t>(C, 1)) : 0;
   if (int rc = g_stage[dev].ensure(off)) return rc;
@@ -915,60 +1006,42 @@ static int rs_energies_impl(const rgf_params* p, const double temp[2], int nm, i
   auto D = [&](size_t o) { return (void*)(base + o); };
   hipStream_t s = nullptr;
   const double2 fill = make_double2(1.0e3, 0.0);
-  // H2D of (count) energies' (nm,nm,nx) complex slabs into a padded device slab
-  auto h2d_c = [&](const double* src, size_t dst_off, size_t count) -> int {
-    if (!padded) {
-      HIPCHK(hipMemcpyAsync(D(dst_off), src, cslab_u * count, hipMemcpyHostToDevice, s));
+  // (count) energies' (nm, nm, nx) complex slabs, energy e0 + i of `a`, into a padded
+  // device area; `diag` fills the padded diagonal
+  auto h2d = [&](const EArr& a, long long e0, long long count, size_t dst_off, double2 diag) -> int {
+    char* stage = padded ? (char*)D(oU) : (char*)D(dst_off);
+    if (a.strided()) {
+      HIPCHK(hipMemcpyAsync(stage, a.at(e0), cslab_u * count, hipMemcpyHostToDevice, s));
     } else {
-      HIPCHK(hipMemcpyAsync(D(oU), src, cslab_u * count, hipMemcpyHostToDevice, s));
+      for (long long i = 0; i < count; ++i)
+        HIPCHK(hipMemcpyAsync(stage + cslab_u * i, a.at(e0 + i), cslab_u, hipMemcpyHostToDevice, s));
+    }
+    if (padded) {
       hipLaunchKernelGGL(pad_blocks, dim3(1024), dim3(256), 0, s, (const double2*)D(oU), nm, (double2*)D(dst_off),
-                         np, (long long)(nx * count), fill);
+                         np, (long long)(nx * count), diag);
       HIPCHK(hipGetLastError());
     }
     return 0;
   };
-  if (int rc = h2d_c(Hii, oH, 1)) return rc;
-  if (int rc = h2d_c(H1i, oC, 1)) return rc;
-  if (Sii) {
-    // overlaps pad with the identity (padded orbitals stay decoupled: z - 1e3 on their diagonal)
-    if (!padded) {
-      HIPCHK(hipMemcpyAsync(D(oS), Sii, cslab_u, hipMemcpyHostToDevice, s));
-    } else {
-      HIPCHK(hipMemcpyAsync(D(oU), Sii, cslab_u, hipMemcpyHostToDevice, s));
-      hipLaunchKernelGGL(pad_blocks, dim3(1024), dim3(256), 0, s, (const double2*)D(oU), nm, (double2*)D(oS), np,
-                         (long long)nx, make_double2(1, 0));
-      HIPCHK(hipGetLastError());
-    }
+  if (int rc = h2d(arr1(Hii), 0, 1, oH, fill)) return rc;
+  // couplings pad with zeros (no diagonal fill)
+  if (int rc = h2d(arr1(H1i), 0, 1, oC, make_double2(0, 0))) return rc;
+  // overlaps pad with the identity (padded orbitals stay decoupled: z - 1e3 on their diagonal)
+  if (Sii)
+    if (int rc = h2d(arr1(Sii), 0, 1, oS, make_double2(1, 0))) return rc;
+  if (!sig_pe) {
+    if (io.sigL.present()) if (int rc = h2d(io.sigL, 0, 1, oSL, make_double2(0, 0))) return rc;
+    if (io.sigR.present()) if (int rc = h2d(io.sigR, 0, 1, oSR, make_double2(0, 0))) return rc;
   }
-  if (padded) {
-    // couplings pad with zeros (no diagonal fill)
-    HIPCHK(hipMemcpyAsync(D(oU), H1i, cslab_u, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(pad_blocks, dim3(1024), dim3(256), 0, s, (const double2*)D(oU), nm, (double2*)D(oC), np,
-                       (long long)nx, make_double2(0, 0));
-    HIPCHK(hipGetLastError());
-  }
-  auto h2d_sig = [&](const double* src, size_t dst_off, size_t count) -> int {
-    if (!padded) {
-      HIPCHK(hipMemcpyAsync(D(dst_off), src, cslab_u * count, hipMemcpyHostToDevice, s));
-    } else {
-      HIPCHK(hipMemcpyAsync(D(oU), src, cslab_u * count, hipMemcpyHostToDevice, s));
-      hipLaunchKernelGGL(pad_blocks, dim3(1024), dim3(256), 0, s, (const double2*)D(oU), nm, (double2*)D(dst_off),
-                         np, (long long)(nx * count), make_double2(0, 0));
-      HIPCHK(hipGetLastError());
-    }
-    return 0;
-  };
-  if (!sig_per_energy) {
-    if (sigL) if (int rc = h2d_sig(sigL, oSL, 1)) return rc;
-    if (sigR) if (int rc = h2d_sig(sigR, oSR, 1)) return rc;
-  }
+  std::vector<double> trh((size_t)C), treh((size_t)C);
+  std::vector<int> infoh((size_t)(4 * C));
   for (long long e0 = 0; e0 < nE; e0 += C) {
     const long long ce = std::min<long long>(C, nE - e0);
-    if (sig_per_energy) {
-      if (sigL) if (int rc = h2d_sig(sigL + 2 * bn * nx * e0, oSL, ce)) return rc;
-      if (sigR) if (int rc = h2d_sig(sigR + 2 * bn * nx * e0, oSR, ce)) return rc;
+    if (sig_pe) {
+      if (io.sigL.present()) if (int rc = h2d(io.sigL, e0, ce, oSL, make_double2(0, 0))) return rc;
+      if (io.sigR.present()) if (int rc = h2d(io.sigR, e0, ce, oSR, make_double2(0, 0))) return rc;
     }
-    HIPCHK(hipMemcpyAsync(D(oE), E + e0, 8 * ce, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(D(oE), io.E + e0, 8 * ce, hipMemcpyHostToDevice, s));
     rgf_batch b;
     std::memset(&b, 0, sizeof(b));
     b.nm = np; b.nx = nx; b.nE = (int)ce;
@@ -976,25 +1049,28 @@ static int rs_energies_impl(const rgf_params* p, const double temp[2], int nm, i
     b.temp[0] = temp[0]; b.temp[1] = temp[1]; b.mu[0] = mu[0]; b.mu[1] = mu[1];
     b.Hii = (const double*)D(oH); b.H1i = (const double*)D(oC);
     b.Sii = Sii ? (const double*)D(oS) : nullptr;
-    b.sigL = sigL ? (const double*)D(oSL) : nullptr;
-    b.sigR = sigR ? (const double*)D(oSR) : nullptr;
-    b.sig_stride = sig_per_energy ? (long long)(bp * nx) : 0;
-    b.ldos = ldos ? (double*)D(oOut[0]) : nullptr;
-    b.ndens = ndens ? (double*)D(oOut[1]) : nullptr;
-    b.pdens = pdens ? (double*)D(oOut[2]) : nullptr;
-    b.GRi1 = GRi1 ? (double*)D(oOut[3]) : nullptr;
-    b.GLi1 = GLi1 ? (double*)D(oOut[4]) : nullptr;
-    b.GGi1 = GGi1 ? (double*)D(oOut[5]) : nullptr;
-    b.cur = cur ? (double*)D(oCur) : nullptr;
+    b.sigL = io.sigL.present() ? (const double*)D(oSL) : nullptr;
+    b.sigR = io.sigR.present() ? (const double*)D(oSR) : nullptr;
+    b.sig_stride = sig_pe ? (long long)(bp * nx) : 0;
+    double** bo[6] = {&b.ldos, &b.ndens, &b.pdens, &b.GRi1, &b.GLi1, &b.GGi1};
+    for (int k = 0; k < 6; ++k) *bo[k] = io.out[k].present() ? (double*)D(oOut[k]) : nullptr;
+    b.cur = has_cur ? (double*)D(oCur) : nullptr;
     b.tr = (double*)D(oTr); b.tre = (double*)D(oTre);
     b.info = (int*)D(oInfo);
     if (int rc = rgf_solve_batched(p, &b, s)) return rc;
     // outputs: diagonal blocks for l = 1..nx, off-diagonal and cur for l = 1..nx-1
+    auto d2h = [&](const EArr& a, const void* src, size_t slab, size_t bytes) -> int {
+      if (a.strided()) {
+        HIPCHK(hipMemcpy2DAsync(a.at(e0), slab, src, slab, bytes, ce, hipMemcpyDeviceToHost, s));
+      } else {
+        for (long long i = 0; i < ce; ++i)
+          HIPCHK(hipMemcpyAsync(a.at(e0 + i), (const char*)src + slab * i, bytes, hipMemcpyDeviceToHost, s));
+      }
+      return 0;
+    };
     for (int k = 0; k < 6; ++k) {
-      if (!outs[k]) continue;
-      const bool offd = (k >= 3);
-      const size_t nb_copy = offd ? (size_t)(nx - 1) : (size_t)nx;
-      double* host = outs[k] + 2 * bn * nx * e0;
+      if (!io.out[k].present()) continue;
+      const size_t nb_copy = (k >= 3) ? (size_t)(nx - 1) : (size_t)nx;
       const void* src = D(oOut[k]);
       if (padded) {
         hipLaunchKernelGGL(unpad_blocks<double2>, dim3(1024), dim3(256), 0, s, (const double2*)src, np,
@@ -1002,10 +1078,9 @@ static int rs_energies_impl(const rgf_params* p, const double temp[2], int nm, i
         HIPCHK(hipGetLastError());
         src = D(oU);
       }
-      HIPCHK(hipMemcpy2DAsync(host, cslab_u, src, cslab_u, nb_copy * bn * 16, ce, hipMemcpyDeviceToHost, s));
+      if (int rc = d2h(io.out[k], src, cslab_u, nb_copy * bn * 16)) return rc;
     }
-    if (cur) {
-      double* host = cur + bn * nx * e0;
+    if (has_cur) {
       const void* src = D(oCur);
       if (padded) {
         hipLaunchKernelGGL(unpad_blocks<double>, dim3(1024), dim3(256), 0, s, (const double*)src, np,
@@ -1013,24 +1088,159 @@ static int rs_energies_impl(const rgf_params* p, const double temp[2], int nm, i
         HIPCHK(hipGetLastError());
         src = D(oU);
       }
-      HIPCHK(hipMemcpy2DAsync(host, rslab_u, src, rslab_u, (size_t)(nx - 1) * bn * 8, ce, hipMemcpyDeviceToHost, s));
+      if (int rc = d2h(io.cur, src, rslab_u, (size_t)(nx - 1) * bn * 8)) return rc;
     }
-    if (tr) HIPCHK(hipMemcpyAsync(tr + e0, D(oTr), 8 * ce, hipMemcpyDeviceToHost, s));
-    if (tre) HIPCHK(hipMemcpyAsync(tre + e0, D(oTre), 8 * ce, hipMemcpyDeviceToHost, s));
-    if (info) HIPCHK(hipMemcpyAsync(info + 4 * e0, D(oInfo), 16 * ce, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(trh.data(), D(oTr), 8 * ce, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(treh.data(), D(oTre), 8 * ce, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(infoh.data(), D(oInfo), 16 * ce, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
-    if (cur)  // the reference zeroes cur on entry and never writes block nx
-      for (long long e = 0; e < ce; ++e) std::memset(cur + bn * nx * (e0 + e) + bn * (nx - 1), 0, bn * 8);
+    for (long long i = 0; i < ce; ++i) {
+      if (io.tr.present()) *(double*)io.tr.at(e0 + i) = trh[(size_t)i];
+      if (io.tre.present()) *(double*)io.tre.at(e0 + i) = treh[(size_t)i];
+      if (io.info.present()) std::memcpy(io.info.at(e0 + i), &infoh[(size_t)(4 * i)], 16);
+      // the reference zeroes cur on entry and never writes block nx
+      if (has_cur) std::memset(io.cur.at(e0 + i) + bn * 8 * (nx - 1), 0, bn * 8);
+    }
   }
   return 0;
+}
+
+// ---------------------------------------------- concurrent per-energy callers
+// The reference's drivers call green_RGF_RS from every OpenMP thread of an energy
+// loop (src/green_rgf.f90:1305-1316).  Concurrent rgf_rs / rgf_rs_ex calls are
+// gathered into one batched launch: a caller that finds no batch being formed
+// becomes the leader, waits while further callers keep arriving (a gap of 200 us
+// ends the wait; at most 5 ms, and no wait once as many callers are queued as the
+// last batch had), takes every queued call that shares its device, nm, nx,
+// parameters, temperatures, chemical potentials, Hii / H1i arrays and the set of
+// given arrays, and solves them as one energy batch; the others sleep until their
+// call is done and return its status.  Results are those of a separate call.
+struct RsCall {
+  rgf_params p;
+  double temp[2], mu[2], E;
+  int nm, nx, dev;
+  const double *Hii, *H1i, *sigL, *sigR;
+  double* out[6];  // ldos, ndens, pdens, GRi1, GLi1, GGi1
+  double *cur, *tr, *tre;
+  int rc = 0;
+  bool done = false;
+  std::string err;
+};
+
+std::mutex g_rs_mu;
+std::condition_variable g_rs_cv;
+std::vector<RsCall*> g_rs_queue;
+bool g_rs_forming = false;
+size_t g_rs_last_batch = 1;
+
+bool rs_same_batch(const RsCall& a, const RsCall& b) {
+  if (a.dev != b.dev || a.nm != b.nm || a.nx != b.nx || a.Hii != b.Hii || a.H1i != b.H1i) return false;
+  if (std::memcmp(&a.p, &b.p, sizeof(rgf_params)) != 0) return false;
+  if (a.temp[0] != b.temp[0] || a.temp[1] != b.temp[1] || a.mu[0] != b.mu[0] || a.mu[1] != b.mu[1]) return false;
+  if (!a.sigL != !b.sigL || !a.sigR != !b.sigR || !a.cur != !b.cur || !a.tr != !b.tr || !a.tre != !b.tre) return false;
+  for (int k = 0; k < 6; ++k)
+    if (!a.out[k] != !b.out[k]) return false;
+  return true;
+}
+
+int rs_solve_calls(const std::vector<RsCall*>& v) {
+  const RsCall& h = *v[0];
+  const int n = (int)v.size();
+  std::vector<double> E(n);
+  HostIO io;
+  io.nE = n;
+  auto list = [&](auto get) {
+    EArr a;
+    if (get(h)) {
+      a.list.resize(n);
+      for (int i = 0; i < n; ++i) a.list[i] = (const char*)get(*v[i]);
+    }
+    return a;
+  };
+  for (int i = 0; i < n; ++i) E[i] = v[i]->E;
+  io.E = E.data();
+  // one Sigma array passed by every call (a ballistic or energy-independent
+  // scattering self-energy) is staged once
+  bool shared = true;
+  for (int i = 1; i < n; ++i) shared = shared && v[i]->sigL == h.sigL && v[i]->sigR == h.sigR;
+  io.sig_per_energy = !shared;
+  if (shared) {
+    io.sigL = arr1(h.sigL);
+    io.sigR = arr1(h.sigR);
+  } else {
+    io.sigL = list([](const RsCall& c) { return (const void*)c.sigL; });
+    io.sigR = list([](const RsCall& c) { return (const void*)c.sigR; });
+  }
+  for (int k = 0; k < 6; ++k) io.out[k] = list([k](const RsCall& c) { return (const void*)c.out[k]; });
+  io.cur = list([](const RsCall& c) { return (const void*)c.cur; });
+  io.tr = list([](const RsCall& c) { return (const void*)c.tr; });
+  io.tre = list([](const RsCall& c) { return (const void*)c.tre; });
+  return rs_core(&h.p, h.temp, h.nm, h.nx, h.mu, nullptr, h.Hii, h.H1i, io);
 }
 
 int rgf_rs_ex(const rgf_params* p, const double temp[2], int nm, int nx, double E, const double mu[2],
               const double* Hii, const double* H1i, const double* sigL, const double* sigG, const double* sigR,
               double* ndens, double* pdens, double* ldos, double* tr, double* tre, double* cur, double* GRi1,
               double* GLi1, double* GGi1) {
-  return rgf_rs_energies(p, temp, nm, nx, 1, &E, mu, Hii, H1i, sigL, sigG, sigR, 0, ndens, pdens, ldos, tr, tre,
-                         cur, GRi1, GLi1, GGi1, nullptr);
+  (void)sigG;  // accepted and unused, as green_RGF_RS (src/green_rgf.f90:3953)
+  g_err.clear();
+  if (int rc = check_params(p)) return rc;
+  if (nm < 1 || nx < 2) return fail(RGF_EINVAL, "need nm >= 1, nx >= 2");
+  if (!temp || !mu || !Hii || !H1i) return fail(RGF_EINVAL, "missing required array");
+  RsCall c;
+  c.p = *p;
+  c.temp[0] = temp[0]; c.temp[1] = temp[1]; c.mu[0] = mu[0]; c.mu[1] = mu[1];
+  c.E = E; c.nm = nm; c.nx = nx;
+  HIPCHK(hipGetDevice(&c.dev));
+  c.Hii = Hii; c.H1i = H1i; c.sigL = sigL; c.sigR = sigR;
+  double* outs[6] = {ldos, ndens, pdens, GRi1, GLi1, GGi1};
+  for (int k = 0; k < 6; ++k) c.out[k] = outs[k];
+  c.cur = cur; c.tr = tr; c.tre = tre;
+  std::unique_lock<std::mutex> lk(g_rs_mu);
+  g_rs_queue.push_back(&c);
+  g_rs_cv.notify_all();
+  while (!c.done) {
+    if (g_rs_forming) {
+      g_rs_cv.wait(lk);
+      continue;
+    }
+    g_rs_forming = true;
+    // gather the callers that are arriving: wait while they keep coming (a 200 us
+    // gap ends it), and up to the size of the previous batch (5 ms at most)
+    const auto t0 = std::chrono::steady_clock::now();
+    size_t seen = g_rs_queue.size();
+    while (std::chrono::steady_clock::now() - t0 < std::chrono::milliseconds(5)) {
+      g_rs_cv.wait_for(lk, std::chrono::microseconds(200));
+      const bool grew = g_rs_queue.size() > seen;
+      seen = g_rs_queue.size();
+      if (!grew && seen >= g_rs_last_batch) break;
+    }
+    std::vector<RsCall*> batch;
+    RsCall* head = g_rs_queue.front();
+    for (size_t i = 0; i < g_rs_queue.size();) {
+      if (rs_same_batch(*head, *g_rs_queue[i])) {
+        batch.push_back(g_rs_queue[i]);
+        g_rs_queue.erase(g_rs_queue.begin() + (long)i);
+      } else {
+        ++i;
+      }
+    }
+    lk.unlock();
+    const int rc = rs_solve_calls(batch);
+    const std::string msg = rc ? g_err : std::string();
+    lk.lock();
+    for (RsCall* b : batch) {
+      b->rc = rc;
+      b->err = msg;
+      b->done = true;
+    }
+    g_rs_last_batch = std::max<size_t>(1, batch.size());
+    g_rs_forming = false;
+    g_rs_cv.notify_all();
+  }
+  lk.unlock();
+  g_err = c.err;
+  return c.rc;
 }
 
 int rgf_rs(const double temp[2], int nm, int nx, double E, const double mu[2], const double* Hii,

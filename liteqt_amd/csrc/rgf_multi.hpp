@@ -45,13 +45,12 @@ struct MultiLds {
 
 __global__ __launch_bounds__(256, 1) void rgf_sweep_multi(MultiArgs m) {
   __shared__ double lds[MultiLds::DOUBLES];
-  __shared__ double red[16];
   __shared__ int next;
   static_assert(Cfg<1>::NTH == 256 && Cfg<2>::NTH == 256 && Cfg<3>::NTH == 256 && Cfg<4>::NTH == 256,
                 "all variants run 256 threads");
   double2* ws = m.ws + (size_t)blockIdx.x * (size_t)m.ws_slot;
   ldsd* L = (ldsd*)lds;
-  ldsd* R = (ldsd*)red;
+  ldsd* R = L;  // block reductions run between block operations, when the LDS is free
   for (;;) {
     if (threadIdx.x == 0) next = atomicAdd(m.counter, 1);
     __syncthreads();

@@ -210,11 +210,10 @@ __device__ void solve_energy_od(const SweepArgs& a, int e, double2* ws, ldsd* ld
 }
 
 template <int T, int NMAX>
-__global__ __launch_bounds__(Cfg<T>::NTH, 1) void rgf_sweep_od(SweepArgs a) {
+__global__ __launch_bounds__(Cfg<T>::NTH, Cfg<T>::OCC) void rgf_sweep_od(SweepArgs a) {
   __shared__ double lds[SweepLds<T, NMAX>::DOUBLES];
-  __shared__ double red[16];
   double2* ws = a.ws + (size_t)blockIdx.x * (size_t)a.ws_slot;
-  for (int e = blockIdx.x; e < a.nE; e += gridDim.x) solve_energy_od<T, NMAX>(a, e, ws, (ldsd*)lds, (ldsd*)red);
+  for (int e = blockIdx.x; e < a.nE; e += gridDim.x) solve_energy_od<T, NMAX>(a, e, ws, (ldsd*)lds, (ldsd*)lds);
 }
 
 }  // namespace rgf

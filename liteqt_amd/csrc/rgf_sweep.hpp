@@ -54,7 +54,19 @@ struct SweepArgs {
   double scba_D, scba_mix;
   // overlap blocks (green_RGF_CMS semantics, solve_energy<..., true> only)
   const double2* Sii;
+  // segment recompute: 0 keeps g_l, g^<_l of every slab; K > 0 keeps them at the
+  // checkpoint slabs 0, K, 2K, ... and one K-slab segment, recomputing a segment's
+  // left-connected blocks from its checkpoint before the backward sweep enters it
+  int ckpt;
 };
+
+// Workspace blocks of one energy slot: the stored left-connected blocks (all slabs,
+// or the checkpoints and one segment) and the temporaries.
+__host__ __device__ inline long long sweep_slot_blocks(int nx, int ckpt, int tmp) {
+  if (ckpt <= 0) return 2LL * nx + tmp;
+  const long long ncp = (nx - 2) / ckpt + 1;  // checkpoint slabs 0, K, .. <= nx - 2
+  return 2 * ncp + 2LL * ckpt + tmp;
+}
 
 __device__ __forceinline__ double ferm_dev(double a) { return 1.0 / (1.0 + exp(a)); }
 
@@ -141,9 +153,14 @@ __device__ void solve_energy(const SweepArgs& a, int e, double2* ws, ldsd* lds, 
   const double E = a.E[e];
   const double2 z = make_double2(E, a.eta_dev);
   const double2 one = make_double2(1, 0), mone = make_double2(-1, 0), zero = make_double2(0, 0);
+  // stored left-connected blocks: slab l at position l (keep-all), or, with
+  // checkpoints every K slabs, at its checkpoint slot (l % K == 0) or in the segment
+  // buffer (slot l % K), which holds the segment the sweeps are in
+  const int K = CMS ? 0 : a.ckpt;
+  const long long nstore = K > 0 ? (long long)((nx - 2) / K + 1) + K : nx;
   double2* Gl = ws;
-  double2* Gln = ws + (size_t)nx * nn;
-  double2* tb = ws + 2 * (size_t)nx * nn;
+  double2* Gln = ws + (size_t)nstore * nn;
+  double2* tb = ws + 2 * (size_t)nstore * nn;
   double2* G = tb + 0 * nn;
   double2* Gn = tb + 1 * nn;
   double2* Gp = tb + 2 * nn;
@@ -169,8 +186,13 @@ __device__ void solve_energy(const SweepArgs& a, int e, double2* ws, ldsd* lds, 
   auto sR = [&](int l) -> const double2* { return a.sigR ? a.sigR + (size_t)e * a.sig_stride + (size_t)l * nn : nullptr; };
   auto sL = [&](int l) -> const double2* { return a.sigL ? a.sigL + (size_t)e * a.sig_stride + (size_t)l * nn : nullptr; };
   auto ob = [&](double2* arr, int l) -> double2* { return arr ? arr + ((size_t)e * nx + l) * nn : nullptr; };
-  auto gl = [&](int l) { return Gl + (size_t)l * nn; };
-  auto gln = [&](int l) { return Gln + (size_t)l * nn; };
+  auto gpos = [&](int l) -> size_t {
+    if (K <= 0) return (size_t)l;
+    const int ncp = (nx - 2) / K + 1;
+    return (l % K == 0) ? (size_t)(l / K) : (size_t)(ncp + l % K);
+  };
+  auto gl = [&](int l) { return Gl + gpos(l) * nn; };
+  auto gln = [&](int l) { return Gln + gpos(l) * nn; };
   const Epi st = epi_store();
   int fails = 0, hitL = 0, hitR = 0;
 
@@ -213,7 +235,9 @@ __device__ void solve_energy(const SweepArgs& a, int e, double2* ws, ldsd* lds, 
   wg_gemm<T>(n, OP_N, T0, OP_C, gl(0), gln(0), 0, st, lds);
 
   // ---- forward sweep
-  for (int l = 1; l < nx - 1; ++l) {
+  // one step: g_l, g^<_l from g_{l-1}, g^<_{l-1} (also the segment recompute); a
+  // singular slab is counted on the first pass only
+  auto forward_step = [&](int l, bool count) {
     Epi ew = epi_store();
     if (CMS) {
       onsite(l, T1);
@@ -223,14 +247,15 @@ __device__ void solve_energy(const SweepArgs& a, int e, double2* ws, ldsd* lds, 
     }
     wg_gemm<T>(n, OP_N, Cb(l), OP_N, gl(l - 1), T0, 0, st, lds);
     wg_gemm<T>(n, OP_N, T0, OP_C, Cb(l), W, 1, ew, lds);
-    if (wg_inverse<T, NMAX>(n, W, gl(l), lds)) ++fails;
+    if (wg_inverse<T, NMAX>(n, W, gl(l), lds) && count) ++fails;
     Epi es = epi_store();
     es.D1 = lesser(l);
     wg_gemm<T>(n, OP_N, Cb(l), OP_N, gln(l - 1), T0, 0, st, lds);
     wg_gemm<T>(n, OP_N, T0, OP_C, Cb(l), T1, 0, es, lds);
     wg_gemm<T>(n, OP_N, gl(l), OP_N, T1, T0, 0, st, lds);
     wg_gemm<T>(n, OP_N, T0, OP_C, gl(l), gln(l), 0, st, lds);
-  }
+  };
+  for (int l = 1; l < nx - 1; ++l) forward_step(l, true);
 
   // ---- right contact
   const int L1 = nx - 1;
@@ -277,6 +302,10 @@ __device__ void solve_energy(const SweepArgs& a, int e, double2* ws, ldsd* lds, 
   // ---- backward sweep
   const bool want_cur = a.cur != nullptr;
   for (int l = nx - 2; l >= 0; --l) {
+    // entering a segment from above: recompute its left-connected blocks from its
+    // checkpoint (the last segment is still in the buffer from the forward sweep)
+    if (K > 0 && l % K == K - 1 && l < nx - 2)
+      for (int m = l - K + 2; m <= l; ++m) forward_step(m, false);
     const double2* H = Cb(phys ? l + 1 : l);
     if (want_cur) {
       wg_gemm<T>(n, OP_N, H, OP_C, gl(l), T0, 0, st, lds);    // H g^H
@@ -371,27 +400,25 @@ struct SweepLds {
 };
 
 template <int T, int NMAX>
-__global__ __launch_bounds__(Cfg<T>::NTH, 1) void rgf_sweep(SweepArgs a) {
+__global__ __launch_bounds__(Cfg<T>::NTH, Cfg<T>::OCC) void rgf_sweep(SweepArgs a) {
   __shared__ double lds[SweepLds<T, NMAX>::DOUBLES];
-  __shared__ double red[16];
   double2* ws = a.ws + (size_t)blockIdx.x * (size_t)a.ws_slot;
-  for (int e = blockIdx.x; e < a.nE; e += gridDim.x) solve_energy<T, NMAX>(a, e, ws, (ldsd*)lds, (ldsd*)red);
+  for (int e = blockIdx.x; e < a.nE; e += gridDim.x) solve_energy<T, NMAX>(a, e, ws, (ldsd*)lds, (ldsd*)lds);
 }
 
 // green_RGF_CMS: the same persistent loop with the overlap variant of the solve.
 template <int T, int NMAX>
-__global__ __launch_bounds__(Cfg<T>::NTH, 1) void rgf_sweep_cms(SweepArgs a) {
+__global__ __launch_bounds__(Cfg<T>::NTH, Cfg<T>::OCC) void rgf_sweep_cms(SweepArgs a) {
   __shared__ double lds[SweepLds<T, NMAX>::DOUBLES];
-  __shared__ double red[16];
   double2* ws = a.ws + (size_t)blockIdx.x * (size_t)a.ws_slot;
   for (int e = blockIdx.x; e < a.nE; e += gridDim.x)
-    solve_energy<T, NMAX, true>(a, e, ws, (ldsd*)lds, (ldsd*)red);
+    solve_energy<T, NMAX, true>(a, e, ws, (ldsd*)lds, (ldsd*)lds);
 }
 
 // Batched building blocks: one workgroup per matrix (used by the block-level
 // entry points rgf_zgemm_batched / rgf_zinv_batched).
 template <int T>
-__global__ __launch_bounds__(Cfg<T>::NTH, 1) void k_gemm_batched(int n, int nb, int opA, const double2* A, int opB,
+__global__ __launch_bounds__(Cfg<T>::NTH, Cfg<T>::OCC) void k_gemm_batched(int n, int nb, int opA, const double2* A, int opB,
                                                         const double2* B, double2* C, int c_rowmajor) {
   __shared__ double lds[GemmLds<T>::DOUBLES];
   const size_t nn = (size_t)n * n;
@@ -402,7 +429,7 @@ __global__ __launch_bounds__(Cfg<T>::NTH, 1) void k_gemm_batched(int n, int nb, 
 // Inverse of column-major matrices: transposed copy into the row-major work
 // buffer W, then the Gauss-Jordan sweep.
 template <int T, int NMAX>
-__global__ __launch_bounds__(Cfg<T>::NTH, 1) void k_inv_batched(int n, int nb, const double2* A, double2* Ainv, double2* W,
+__global__ __launch_bounds__(Cfg<T>::NTH, Cfg<T>::OCC) void k_inv_batched(int n, int nb, const double2* A, double2* Ainv, double2* W,
                                                        int* info) {
   __shared__ double lds[SweepLds<T, NMAX>::DOUBLES];
   const size_t nn = (size_t)n * n;

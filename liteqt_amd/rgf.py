@@ -49,7 +49,7 @@ class Batch(ctypes.Structure):
                 ("cur", ctypes.c_void_p), ("tr", ctypes.c_void_p), ("tre", ctypes.c_void_p),
                 ("dens_n", ctypes.c_void_p), ("dens_p", ctypes.c_void_p), ("info", ctypes.c_void_p),
                 ("sigR_lo", ctypes.c_void_p), ("sigR_up", ctypes.c_void_p), ("sigL_up", ctypes.c_void_p),
-                ("Sii", ctypes.c_void_p)]
+                ("Sii", ctypes.c_void_p), ("ckpt", ctypes.c_int)]
 
 
 class MultiReport(ctypes.Structure):
@@ -269,7 +269,7 @@ def plain_lib():
 
 
 def solve_batched(p, E, temp, mu, Hii, H1i, sigL=None, sigR=None, out=None, max_slots=0, stream=None,
-                  plain=False, sigR_lo=None, sigR_up=None, sigL_up=None, scba=None):
+                  plain=False, sigR_lo=None, sigR_up=None, sigL_up=None, scba=None, ckpt=0):
     """Device-resident batched solve (plain=True: the per-energy rocBLAS/rocSOLVER
     comparison path with the same arguments).  E, Hii, H1i, sig*, out[...] are torch
     tensors on the HIP device: complex blocks as complex128 tensors of shape
@@ -278,11 +278,14 @@ def solve_batched(p, E, temp, mu, Hii, H1i, sigL=None, sigR=None, out=None, max_
     {ldos, ndens, pdens, GRi1, GLi1, GGi1, cur, tr, tre, dens_n, dens_p, info}
     to preallocated tensors.  sigR_lo / sigR_up / sigL_up: first off-diagonal
     Sigma blocks (physical mode).  scba=(D, mix): one SCBA iteration that also
-    refreshes the given Sigma tensors in place (rgf_scba_step).  Asynchronous on
-    `stream` (default: torch's current stream)."""
+    refreshes the given Sigma tensors in place (rgf_scba_step).  ckpt: storage of
+    the left-connected blocks (rgf_batch.ckpt: -1 every slab, K > 0 checkpoints every
+    K slabs with segment recompute, 0 automatic).  Asynchronous on `stream`
+    (default: torch's current stream)."""
     import torch
     out = out or {}
     b = _batch(torch, E, temp, mu, Hii, H1i, sigL, sigR, max_slots, sigR_lo, sigR_up, sigL_up)
+    b.ckpt = int(ckpt)
     _out_batch(torch, b, out, E.device)
     if stream is None:
         stream = torch.cuda.current_stream()

@@ -14,11 +14,12 @@ extern "C" int host_sweep(int cms, int mode, int nm, int nx, int nE, const doubl
                           const double* Hii, const double* H1i, const double* Sii, const double* sigL,
                           const double* sigR, long long sig_stride, double* ldos, double* ndens, double* pdens,
                           double* GRi1, double* GLi1, double* GGi1, double* cur, double* tr, double* tre,
-                          double* dens_n, double* dens_p, int* info, long long* nops, char* log, int loglen) {
+                          double* dens_n, double* dens_p, int* info, long long* nops, char* log, int loglen,
+                          int ckpt) {
   using namespace rgf;
   hostcheck::reset();
   const size_t nn = (size_t)nm * nm, blk = nn * 16 * nx;
-  const long long slot = (long long)(2 * nx + kTmpBlocks) * (long long)nn;
+  const long long slot = sweep_slot_blocks(nx, ckpt, kTmpBlocks) * (long long)nn;
   std::vector<double2> ws((size_t)slot);
   hostcheck::add(Hii, blk, "Hii");
   hostcheck::add(H1i, blk, "H1i");
@@ -42,6 +43,7 @@ extern "C" int host_sweep(int cms, int mode, int nm, int nx, int nE, const doubl
   a.cur = cur; a.tr = tr; a.tre = tre; a.dens_n = dens_n; a.dens_p = dens_p; a.info = info;
   a.ws = ws.data();
   a.ws_slot = slot;
+  a.ckpt = ckpt;
   if (cms)
     rgf_sweep_cms<1, 4096>(a);
   else

@@ -35,7 +35,7 @@ def hostlib(tmp_path_factory):
     L = ctypes.CDLL(str(so))
     vp, ci, cd = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
     L.host_sweep.argtypes = ([ci, ci, ci, ci, ci, vp, vp, vp, cd, cd, cd, ci, cd] + [vp] * 5 + [ctypes.c_longlong]
-                             + [vp] * 13 + [ctypes.c_char_p, ci])
+                             + [vp] * 13 + [ctypes.c_char_p, ci, ci])
     return L
 
 
@@ -47,7 +47,7 @@ def _p(a):
     return None if a is None else a.ctypes.data
 
 
-def run(L, cms, mode, p, temp, E, mu, Hii, H1i, Sii=None, sigL=None, sigR=None):
+def run(L, cms, mode, p, temp, E, mu, Hii, H1i, Sii=None, sigL=None, sigR=None, ckpt=0):
     nm, _, nx = Hii.shape
     E = np.ascontiguousarray(np.atleast_1d(E), np.float64)
     nE = len(E)
@@ -66,7 +66,7 @@ def run(L, cms, mode, p, temp, E, mu, Hii, H1i, Sii=None, sigL=None, sigR=None):
     nv = L.host_sweep(int(cms), mode, nm, nx, nE, _p(E), _p(t), _p(m), p.eta_dev, p.eta_lead, p.dec_tol, p.dec_nmax, p.boltz,
                       _p(Hii), _p(H1i), _p(Sii), _p(sigL), _p(sigR), 0,
                       *[_p(out[k]) for k in ("ldos", "ndens", "pdens", "GRi1", "GLi1", "GGi1", "cur", "tr", "tre",
-                                             "dens_n", "dens_p", "info")], ctypes.byref(nops), log, len(log))
+                                             "dens_n", "dens_p", "info")], ctypes.byref(nops), log, len(log), ckpt)
     return nv, log.value.decode(), nops.value, out
 
 
@@ -107,3 +107,25 @@ def test_plain_solve_program_in_bounds_and_equal_to_oracle(hostlib, mode):
             for l in range(nb):
                 assert relerr(g[k][..., l, e], r[k][..., l, e]) <= 1e-10, (k, l, e)
         assert abs(g["tr"][e] - r["tr"][e]) <= 1e-9 * max(1.0, abs(r["tr"][e]))
+
+
+@pytest.mark.parametrize("nx,K", [(11, 3), (12, 5), (9, 4), (7, 1), (6, 6), (10, 8)])
+def test_checkpointed_program_in_bounds_and_equal_to_keep_all(hostlib, nx, K):
+    """Segment recompute: with the left-connected blocks kept only every K slabs and
+    in one K-slab segment (a workspace slot of 2 ((nx-2)/K + 1) + 2K + 30 blocks),
+    every block pointer stays inside that smaller slot and every output equals the
+    keep-all program's bit for bit."""
+    nm = 8
+    Hii, H1i = configs.random_device(nm, nx, 90 + nx)
+    sL, sG, sR = configs.scattering_self_energies(nm, nx, 91 + nx, E=0.0, scale=0.02)
+    E = np.array([-0.3, 0.2])
+    temp, mu = (300.0, 250.0), (0.1, -0.1)
+    for mode in (0, 1):
+        po = orc.params(mode, eta_dev=1e-4, eta_lead=1e-4, dec_tol=1e-12)
+        nv0, log0, n0, g0 = run(hostlib, 0, mode, po, temp, E, mu, Hii, H1i, None, sL, sR, ckpt=0)
+        nv, log, n1, g = run(hostlib, 0, mode, po, temp, E, mu, Hii, H1i, None, sL, sR, ckpt=K)
+        assert nv0 == 0 and nv == 0, log
+        if K > 1 and nx - 2 > K:
+            assert n1 > n0  # segments were recomputed
+        for k in ("ldos", "ndens", "pdens", "GRi1", "GLi1", "GGi1", "cur", "tr", "tre", "info"):
+            assert np.array_equal(g[k], g0[k]), (k, nx, K, mode)
