@@ -21,6 +21,18 @@
  * condition (E > max(w, E_1) and E - w < E_nen: the last energy of G never enters and
  * Sigma_new(E_1) = 0).
  *
+ * Two forms compute the same sums.  The direct form (an element's energy series staged in
+ * LDS, O(nen^2) FMAs per element) serves gw_polarization up to nen = 3072 and
+ * gw_selfenergy up to nen = 2040; above that, and always in the _fft entry points, the
+ * FFT form of the reference's corr1d2 / conv1d2 with method 'fft' (src/fft_mod.f90:56-212)
+ * runs: every band element's series zero-padded to N = 2^m >= nen + nopmax points
+ * (N >= 64, N <= 2^24), transformed, multiplied and transformed back (O(N log N) per
+ * element).  Its spectra live in a per-device workspace of K N * 16 bytes per band slot
+ * (K = 3 for P, 5 for Sigma; slots nm min(2 ndiag + 1, nm) per slab), allocated for a
+ * chunk of slabs of up to 4 GiB (one slab at least) and kept like the gather workspace.
+ * The two forms agree to rounding: the FFT's error is relative to the norms of the
+ * series, not to each output value.
+ *
  * Calls on one stream run in order; the arrays must not overlap each other.  For dense
  * bands (ndiag >= 8) gw_polarization gathers the band's Im G series into a per-device
  * workspace it allocates (nm min(2 ndiag + 1, nm) nx 3 nen doubles, kept until
@@ -52,14 +64,25 @@ typedef struct {
 void gw_params_reference(gw_params* p, int nm, int nx, int nen, int ndiag, double dE, double spindeg,
                          double alpha_mix);
 
-/* P^<, P^>, P^r from G^<, G^>, G^r (src/green_rgf.f90:3190-3221).  nen <= 3072. */
+/* P^<, P^>, P^r from G^<, G^>, G^r (src/green_rgf.f90:3190-3221): the direct form for
+ * nen <= 3072, the FFT form above. */
 int gw_polarization(const gw_params* p, const double* Gl, const double* Gg, const double* Gr, double* Pl,
                     double* Pg, double* Pr, void* stream);
 
 /* Sigma^<, Sigma^>, Sigma^r (the running self-energy, updated in place) from G and
- * W^<, W^>, W^r (src/green_rgf.f90:3233-3290).  nen <= 2040, nx >= 5. */
+ * W^<, W^>, W^r (src/green_rgf.f90:3233-3290): the direct form for nen <= 2040, the FFT
+ * form above.  nx >= 5. */
 int gw_selfenergy(const gw_params* p, const double* Gl, const double* Gg, const double* Gr, const double* Wl,
                   const double* Wg, const double* Wr, double* Sl, double* Sg, double* Sr, void* stream);
+
+/* The same steps always in the FFT form: the polarization's corr1d2(..., method='fft')
+ * calls of green_rgf_solve_gw_ephoton_3d_ijs (src/green_rgf.f90:1397-1468,
+ * src/fft_mod.f90:56-90) and the self-energy's conv1d2 (src/green_rgf.f90:1600-1672,
+ * src/fft_mod.f90:182-212), applied to the one-dimensional driver's index ranges. */
+int gw_polarization_fft(const gw_params* p, const double* Gl, const double* Gg, const double* Gr, double* Pl,
+                        double* Pg, double* Pr, void* stream);
+int gw_selfenergy_fft(const gw_params* p, const double* Gl, const double* Gg, const double* Gr, const double* Wl,
+                      const double* Wg, const double* Wr, double* Sl, double* Sg, double* Sr, void* stream);
 
 /* Valid (ie, nop) pairs summed per band element (algorithmic work): for P and for Sigma. */
 long long gw_pairs_polarization(int nen, int nopmax);

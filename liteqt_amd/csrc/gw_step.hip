@@ -25,6 +25,7 @@
 #include <string>
 
 #include "../../include/gw.h"
+#include "gw_fft.hpp"
 
 namespace {
 
@@ -47,6 +48,8 @@ constexpr int PADP = SP;   // a wave's window reaches SP - 1 below its first out
 // self-energy: 8 waves x 64 lanes x 4 outputs = 2048 outputs per pass (two waves per SIMD)
 constexpr int NTH_S = 512, RS = 4, SS_ = 64 * RS;
 constexpr int PADY = 260;  // zero padding of a self-energy Y series on each side (reach 258)
+// largest nen of the direct forms (an element's series live in LDS)
+constexpr int kPolDirectMax = 3072, kSigDirectMax = 2040;
 
 struct GwArgs {
   int nm, nx, nen, nd, nopmax, half;  // nd = min(ndiag, nm - 1), half = nen / 2
@@ -507,6 +510,292 @@ __global__ __launch_bounds__(256) void k_sig_offband(GwArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------------
+// FFT form (src/fft_mod.f90:56-212): every band element's series are zero-padded to
+// N = 2^logN >= nen + nopmax points (no wrap-around reaches the frequency window or the
+// energy range), transformed, multiplied and transformed back.  A chunk of slabs holds
+// K series of N points per slot, slot = (slab - slab0) * nm + j) * R + r for row
+// i = max(0, j - nd) + r (R = min(2 nd + 1, nm)), series (slot * K + kind) * N.
+//   polarization, K = 3: 0 = Im G^< + i Im G^> of ij, 1 = Im G^r of ij (then
+//     F_l + i F_g), 2 = F_r.  With a, b, c the ij series and d, e = g^>_ji, g^<_ji:
+//     F_l = A conj(D), F_g = B conj(E), F_r = A conj(C) - C conj(E) (the correlations
+//     sum_E x(E) y(E - w) of the direct form are IDFT(X conj(Y)) at w mod N).
+//   self-energy, K = 5: 0 = Im G^< + i Im G^>, 1 = Im G^r (energies 0 .. nen-2; the
+//     last energy of G never enters), 2, 3, 4 = W^<, W^>, W^r with W(w) at w mod N,
+//     then Y^< = G^< W^<, Y^> = G^> W^>, Y^r = G^< W^r + G^r (W^< + W^r) (the sums
+//     sum_w g(E - w) W(w) are IDFT(G W) at E).
+// Real series are transformed two to a complex series and separated with
+// X_k = (Z_k + conj Z_{N-k}) / 2, Y_k = (Z_k - conj Z_{N-k}) / (2 i).
+struct FftGw {
+  int nm, nx, nen, nd, nopmax, half, R, K;
+  int lrb;  // log2 of the rows of a gather / scatter tile
+  int slab0, nsl;
+  int logN, logN1, logN2;
+  double dr, di, alpha, inv_n;
+  const double* Gl;
+  const double* Gg;
+  const double* Gr;
+  const double* Wl;
+  const double* Wg;
+  const double* Wr;
+  double* Ol;
+  double* Og;
+  double* Or;
+  double2* Z;
+};
+
+__device__ __forceinline__ long long fslot(const FftGw& a, int i, int j, int ixl) {
+  return ((long long)ixl * a.nm + j) * a.R + (i - max(0, j - a.nd));
+}
+
+// Column run (ixl, j) and row block rb (rows r0 .. r0 + nr - 1 of the band rows of
+// column j, blocks of RB = 2^lrb rows) of blockIdx-strided index cy; false when the block
+// holds no row (uniform over the workgroup).
+__device__ __forceinline__ bool fft_run(const FftGw& a, long long cy, int& j, int& ixl, int& r0, int& nr, int& lo) {
+  const int RB = 1 << a.lrb;
+  const int nrb = (a.R + RB - 1) / RB;
+  const int rb = (int)(cy % nrb);
+  const long long cj = cy / nrb;
+  j = (int)(cj % a.nm);
+  ixl = (int)(cj / a.nm);
+  lo = max(0, j - a.nd);
+  const int hi = min(a.nm - 1, j + a.nd);
+  r0 = rb * RB;
+  nr = min(RB, hi - lo + 1 - r0);
+  return nr > 0;
+}
+
+// Series gather through an LDS tile of RB rows x EB = 4096 / RB positions (reads along
+// i, writes along the energy; RB = the band's rows per column rounded up to a power of
+// two, at most 64, so narrow bands fill the tile with energies instead of empty rows).
+// POL: kinds 0, 1 of the polarization; else kinds 0 .. 4 of the self-energy.
+template <bool POL>
+__global__ __launch_bounds__(256) void k_fft_gather(FftGw a) {
+  __shared__ double2 tile[4096 + 64];
+  const int lrb = a.lrb, RB = 1 << lrb, EB = min(4096 >> lrb, 1 << a.logN), TS = EB + 1;
+  const int p0 = blockIdx.x * EB, tid = threadIdx.x, N = 1 << a.logN;
+  const size_t ES = (size_t)a.nm * a.nm * a.nx;
+  const long long runs = (long long)a.nsl * a.nm * ((a.R + RB - 1) / RB);
+  const int glim = POL ? a.nen : a.nen - 1;  // G energies that enter
+  for (long long cy = blockIdx.y; cy < runs; cy += gridDim.y) {
+    int j, ixl, r0, nr, lo;
+    if (!fft_run(a, cy, j, ixl, r0, nr, lo)) continue;
+    const int ix = a.slab0 + ixl;
+    const size_t col = (size_t)a.nm * j + (size_t)a.nm * a.nm * ix + lo + r0;
+    for (int s = 0; s < (POL ? 2 : 5); ++s) {
+      for (int k = tid; k < RB * EB; k += 256) {
+        const int r = k & (RB - 1), p = p0 + (k >> lrb);
+        double2 v = make_double2(0.0, 0.0);
+        if (r < nr) {
+          if (s < 2) {
+            if (p < glim) {
+              const size_t o = 2 * (col + r + ES * (size_t)p) + 1;
+              v = s == 0 ? make_double2(a.Gl[o], a.Gg[o]) : make_double2(a.Gr[o], 0.0);
+            }
+          } else {
+            int nop = -1 - a.nopmax;  // none
+            if (p <= a.nopmax) nop = p;
+            else if (p >= N - a.nopmax) nop = p - N;
+            if (nop >= -a.nopmax) {
+              const double* W = s == 2 ? a.Wl : (s == 3 ? a.Wg : a.Wr);
+              v = reinterpret_cast<const double2*>(W)[col + r + ES * (size_t)(a.half - 1 + nop)];
+            }
+          }
+        }
+        tile[r * TS + (k >> lrb)] = v;
+      }
+      __syncthreads();
+      for (int k = tid; k < RB * EB; k += 256) {
+        const int e = k % EB, r = k / EB;
+        if (r < nr)
+          a.Z[((size_t)fslot(a, lo + r0 + r, j, ixl) * a.K + s) * N + p0 + e] = tile[r * TS + e];
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// One slot per blockIdx.x / pblocks; each frequency pair {k, N - k} is handled by the
+// thread at the storage position of its smaller member.
+__device__ __forceinline__ bool fft_pair(const FftGw& a, long long& slot, int& i, int& j, int& ixl, long long& p,
+                                         long long& pp) {
+  // 32-bit index math: slots and positions of one launch stay below 2^31
+  const int N = 1 << a.logN;
+  const int lpb = a.logN > 8 ? a.logN - 8 : 0;  // log2 of the 256-position blocks per slot
+  const unsigned sl = blockIdx.x >> lpb;
+  slot = sl;
+  p = (long long)((blockIdx.x & ((1u << lpb) - 1)) * 256 + threadIdx.x);
+  if (p >= N) return false;
+  const unsigned col = sl / (unsigned)a.R;
+  const int r = (int)(sl - col * (unsigned)a.R);
+  ixl = (int)(col / (unsigned)a.nm);
+  j = (int)(col - (unsigned)ixl * (unsigned)a.nm);
+  i = max(0, j - a.nd) + r;
+  if (i > min(a.nm - 1, j + a.nd)) return false;
+  const long long k = gwfft::freq(p, a.logN, a.logN1, a.logN2);
+  const long long kk = (N - k) & (N - 1);
+  if (k > kk) return false;
+  pp = gwfft::pos(kk, a.logN, a.logN1, a.logN2);
+  return true;
+}
+
+__device__ __forceinline__ double2 sep_x(double2 zk, double2 zm) {  // spectrum of Re z at k
+  return make_double2(0.5 * (zk.x + zm.x), 0.5 * (zk.y - zm.y));
+}
+__device__ __forceinline__ double2 sep_y(double2 zk, double2 zm) {  // spectrum of Im z at k
+  return make_double2(0.5 * (zk.y + zm.y), -0.5 * (zk.x - zm.x));
+}
+
+__global__ __launch_bounds__(256) void k_pol_fft_product(FftGw a) {
+  long long slot, p, pp;
+  int i, j, ixl;
+  if (!fft_pair(a, slot, i, j, ixl, p, pp)) return;
+  const size_t N = (size_t)1 << a.logN;
+  const double2* Z1 = a.Z + (size_t)slot * 3 * N;
+  double2* Z2 = a.Z + ((size_t)slot * 3 + 1) * N;
+  double2* O2 = a.Z + ((size_t)slot * 3 + 2) * N;
+  const double2* T1 = a.Z + (size_t)fslot(a, j, i, ixl) * 3 * N;
+  const double2 zk = Z1[p], zm = Z1[pp], tk = T1[p], tm = T1[pp], C = Z2[p];
+  const double2 A = sep_x(zk, zm), B = sep_y(zk, zm);
+  const double2 E = sep_x(tk, tm), D = sep_y(tk, tm);
+  using gwfft::cmul;
+  using gwfft::conj2;
+  const double2 Fl = cmul(A, conj2(D)), Fg = cmul(B, conj2(E));
+  const double2 Fr = gwfft::csub(cmul(A, conj2(C)), cmul(C, conj2(E)));
+  Z2[p] = make_double2(Fl.x - Fg.y, Fl.y + Fg.x);  // F_l + i F_g
+  O2[p] = Fr;
+  if (pp != p) {
+    Z2[pp] = make_double2(Fl.x + Fg.y, -Fl.y + Fg.x);  // conj(F_l) + i conj(F_g)
+    O2[pp] = conj2(Fr);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_sig_fft_product(FftGw a) {
+  long long slot, p, pp;
+  int i, j, ixl;
+  if (!fft_pair(a, slot, i, j, ixl, p, pp)) return;
+  const size_t N = (size_t)1 << a.logN;
+  const double2* G1 = a.Z + (size_t)slot * 5 * N;
+  const double2* G2 = G1 + N;
+  double2* W1 = a.Z + ((size_t)slot * 5 + 2) * N;
+  double2* W2 = W1 + N;
+  double2* W3 = W2 + N;
+  using gwfft::cadd;
+  using gwfft::cmul;
+  using gwfft::conj2;
+  const double2 zk = G1[p], zm = G1[pp];
+  const double2 Gl = sep_x(zk, zm), Gg = sep_y(zk, zm), Gr = G2[p];
+  const double2 w1 = W1[p], w2 = W2[p], w3 = W3[p];
+  const double2 w1m = W1[pp], w2m = W2[pp], w3m = W3[pp];
+  W1[p] = cmul(Gl, w1);
+  W2[p] = cmul(Gg, w2);
+  W3[p] = cadd(cmul(Gl, w3), cmul(Gr, cadd(w1, w3)));
+  if (pp != p) {
+    W1[pp] = cmul(conj2(Gl), w1m);
+    W2[pp] = cmul(conj2(Gg), w2m);
+    W3[pp] = cadd(cmul(conj2(Gl), w3m), cmul(conj2(Gr), cadd(w1m, w3m)));
+  }
+}
+
+// P from the transformed-back series: S_l + i S_g (kind 1) and S_r (kind 2) at
+// w mod N, times 1/N; P^< = -dE_P S_l, P^> = -dE_P S_g, P^r = dE_P S_r in the window,
+// zero at the other energies.  Tiles of RB rows x 2048 / RB energies through LDS.
+__global__ __launch_bounds__(256) void k_pol_fft_scatter(FftGw a) {
+  __shared__ double tl[2048 + 64], tg[2048 + 64], tr[2048 + 64];
+  const int lrb = a.lrb, RB = 1 << lrb, EB = 2048 >> lrb, TS = EB + 1;
+  const int e0 = blockIdx.x * EB, tid = threadIdx.x, N = 1 << a.logN;
+  const size_t ES = (size_t)a.nm * a.nm * a.nx;
+  const long long runs = (long long)a.nsl * a.nm * ((a.R + RB - 1) / RB);
+  for (long long cy = blockIdx.y; cy < runs; cy += gridDim.y) {
+    int j, ixl, r0, nr, lo;
+    if (!fft_run(a, cy, j, ixl, r0, nr, lo)) continue;
+    for (int k = tid; k < RB * EB; k += 256) {
+      const int e = k % EB, r = k / EB, eo = e0 + e;
+      const int nop = eo - (a.half - 1);
+      double sl = 0.0, sg = 0.0, sr = 0.0;
+      if (r < nr && eo < a.nen && nop >= -a.nopmax && nop <= a.nopmax) {
+        const size_t b = (size_t)fslot(a, lo + r0 + r, j, ixl) * 3 * N + (size_t)(nop & (N - 1));
+        const double2 v1 = a.Z[b + N], v2 = a.Z[b + 2 * N];
+        sl = v1.x * a.inv_n;
+        sg = v1.y * a.inv_n;
+        sr = v2.x * a.inv_n;
+      }
+      tl[r * TS + e] = sl;
+      tg[r * TS + e] = sg;
+      tr[r * TS + e] = sr;
+    }
+    __syncthreads();
+    const size_t col = (size_t)a.nm * j + (size_t)a.nm * a.nm * (a.slab0 + ixl) + lo + r0;
+    for (int k = tid; k < RB * EB; k += 256) {
+      const int r = k & (RB - 1), e = k >> lrb, eo = e0 + e;
+      if (r < nr && eo < a.nen) {
+        const size_t w = 2 * (col + r + ES * (size_t)eo);
+        const double vl = tl[r * TS + e], vg = tg[r * TS + e], vr = tr[r * TS + e];
+        a.Ol[w] = -vl * a.dr; a.Ol[w + 1] = -vl * a.di;
+        a.Og[w] = -vg * a.dr; a.Og[w + 1] = -vg * a.di;
+        a.Or[w] = vr * a.dr;  a.Or[w + 1] = vr * a.di;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Sigma from the transformed-back series (kinds 2, 3, 4 at E, times 1/N): with
+// G = i g, Sigma_new^x = dE_S i y^x (Sigma_new(E_1) = 0), Im Sigma_new^r =
+// Im(Sigma_new^> - Sigma_new^<) / 2, mixing and the edge copy as in the direct form.
+// Tiles of RB rows x 1024 / RB energies.
+__global__ __launch_bounds__(256) void k_sig_fft_scatter(FftGw a) {
+  __shared__ double2 tl[1024 + 64], tg[1024 + 64], tr[1024 + 64];
+  const int lrb = a.lrb, RB = 1 << lrb, EB = 1024 >> lrb, TS = EB + 1;
+  const int e0 = blockIdx.x * EB, tid = threadIdx.x, N = 1 << a.logN;
+  const size_t ES = (size_t)a.nm * a.nm * a.nx;
+  const size_t SS = (size_t)a.nm * a.nm;
+  const int last = a.nx - 3;
+  const long long runs = (long long)a.nsl * a.nm * ((a.R + RB - 1) / RB);
+  for (long long cy = blockIdx.y; cy < runs; cy += gridDim.y) {
+    int j, ixl, r0, nr, lo;
+    if (!fft_run(a, cy, j, ixl, r0, nr, lo)) continue;
+    for (int k = tid; k < RB * EB; k += 256) {
+      const int e = k % EB, r = k / EB, eo = e0 + e;
+      double2 nl = make_double2(0, 0), ng = nl, nrr = nl;
+      if (r < nr && eo < a.nen && eo > 0) {
+        const size_t b = ((size_t)fslot(a, lo + r0 + r, j, ixl) * 5 + 2) * N + eo;
+        const double2 y1 = a.Z[b], y2 = a.Z[b + N], y3 = a.Z[b + 2 * N];
+        const double2 al = make_double2(-y1.y * a.inv_n, y1.x * a.inv_n);  // i y / N
+        const double2 ag = make_double2(-y2.y * a.inv_n, y2.x * a.inv_n);
+        const double2 ar = make_double2(-y3.y * a.inv_n, y3.x * a.inv_n);
+        nl = make_double2(al.x * a.dr - al.y * a.di, al.x * a.di + al.y * a.dr);
+        ng = make_double2(ag.x * a.dr - ag.y * a.di, ag.x * a.di + ag.y * a.dr);
+        nrr = make_double2(ar.x * a.dr - ar.y * a.di, (ng.y - nl.y) / 2.0);
+      }
+      tl[r * TS + e] = nl;
+      tg[r * TS + e] = ng;
+      tr[r * TS + e] = nrr;
+    }
+    __syncthreads();
+    const int ix = a.slab0 + ixl;
+    const size_t col = (size_t)a.nm * j + SS * ix + lo + r0;
+    double2* S[3] = {reinterpret_cast<double2*>(a.Ol), reinterpret_cast<double2*>(a.Og),
+                     reinterpret_cast<double2*>(a.Or)};
+    for (int k = tid; k < RB * EB; k += 256) {
+      const int r = k & (RB - 1), e = k >> lrb, eo = e0 + e;
+      if (r < nr && eo < a.nen) {
+        const size_t o = col + r + ES * (size_t)eo;
+        const double2 nv[3] = {tl[r * TS + e], tg[r * TS + e], tr[r * TS + e]};
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          const double2 old = S[q][o];
+          const double2 m = make_double2(old.x + a.alpha * (nv[q].x - old.x), old.y + a.alpha * (nv[q].y - old.y));
+          S[q][o] = m;
+          if (ix == 2) { S[q][o - 2 * SS] = m; S[q][o - SS] = m; }
+          if (ix == last) { S[q][o + SS] = m; S[q][o + 2 * SS] = m; }
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // Per-device workspace of the polarization's gathered series (grow-only).
 struct Workspace {
   void* p = nullptr;
@@ -552,6 +841,137 @@ dim3 offband_grid(int nm, long long rows) {
   return dim3(gx, (unsigned)std::min<long long>(gy, 65535));
 }
 
+
+// FFT form: workspace bytes per slab of series (K kinds per slot) and the launches.
+constexpr size_t kFftChunkBytes = size_t(4) << 30;  // spectra of one chunk of slabs
+
+int fft_log2(const gw_params* p, int& logN) {
+  const long long need = (long long)p->nen + p->nopmax;
+  logN = 6;
+  while ((1LL << logN) < need) ++logN;
+  if (logN > 24) return fail(4, "gw fft: nen + nopmax > 2^24");
+  return 0;
+}
+
+void fft_pass(hipStream_t s, const FftGw& g, int k0, int kc, bool inverse) {
+  gwfft::FftArgs f{};
+  f.base = g.Z;
+  f.nslots = (long long)g.nsl * g.nm * g.R;
+  f.K = g.K;
+  f.k0 = k0;
+  f.kc = kc;
+  f.logN = g.logN;
+  f.logN1 = g.logN1;
+  f.logN2 = g.logN2;
+  f.nm = g.nm;
+  f.nd = g.nd;
+  f.R = g.R;
+  const long long nser = f.nslots * kc;
+  if (g.logN <= gwfft::kLog) {
+    const long long B = gwfft::kLds >> g.logN;
+    const unsigned grid = (unsigned)((nser + B - 1) / B);
+    if (inverse) hipLaunchKernelGGL((gwfft::k_fft_single<true, 256, 4096>), dim3(grid), dim3(256), 0, s, f);
+    else hipLaunchKernelGGL((gwfft::k_fft_single<false, 256, 4096>), dim3(grid), dim3(256), 0, s, f);
+    return;
+  }
+  if (g.logN == gwfft::kLog1) {  // 8192 points: one series per 512-thread workgroup
+    if (inverse) hipLaunchKernelGGL((gwfft::k_fft_single<true, 512, 8192>), dim3((unsigned)nser), dim3(512), 0, s, f);
+    else hipLaunchKernelGGL((gwfft::k_fft_single<false, 512, 8192>), dim3((unsigned)nser), dim3(512), 0, s, f);
+    return;
+  }
+  const long long N1 = 1LL << g.logN1, N2 = 1LL << g.logN2;
+  const unsigned gc = (unsigned)(nser * (N1 / (gwfft::kLds / N2)));
+  const unsigned gr = (unsigned)(nser * (N2 / (gwfft::kLds / N1)));
+  if (!inverse) {
+    hipLaunchKernelGGL((gwfft::k_fft_cols<false, true>), dim3(gc), dim3(gwfft::kNth), 0, s, f);
+    hipLaunchKernelGGL((gwfft::k_fft_rows<false, false>), dim3(gr), dim3(gwfft::kNth), 0, s, f);
+  } else {
+    hipLaunchKernelGGL((gwfft::k_fft_rows<true, true>), dim3(gr), dim3(gwfft::kNth), 0, s, f);
+    hipLaunchKernelGGL((gwfft::k_fft_cols<true, false>), dim3(gc), dim3(gwfft::kNth), 0, s, f);
+  }
+}
+
+// The polarization (pol) or self-energy step in the FFT form, chunk by chunk of slabs.
+int fft_step(const gw_params* p, bool pol, const double* Gl, const double* Gg, const double* Gr, const double* Wl,
+             const double* Wg, const double* Wr, double* Ol, double* Og, double* Or, hipStream_t s) {
+  FftGw g{};
+  g.nm = p->nm;
+  g.nx = p->nx;
+  g.nen = p->nen;
+  g.nd = std::min(p->ndiag, p->nm - 1);
+  g.nopmax = p->nopmax;
+  g.half = p->nen / 2;
+  g.R = std::min(2 * g.nd + 1, g.nm);
+  g.K = pol ? 3 : 5;
+  if (int rc = fft_log2(p, g.logN)) return rc;
+  gwfft::split(g.logN, g.logN1, g.logN2);
+  g.lrb = 0;
+  while ((1 << g.lrb) < std::min(g.R, 64)) ++g.lrb;
+  g.dr = pol ? p->dE_P[0] : p->dE_S[0];
+  g.di = pol ? p->dE_P[1] : p->dE_S[1];
+  g.alpha = p->alpha_mix;
+  g.inv_n = 1.0 / (double)(1LL << g.logN);
+  g.Gl = Gl; g.Gg = Gg; g.Gr = Gr;
+  g.Wl = Wl; g.Wg = Wg; g.Wr = Wr;
+  g.Ol = Ol; g.Og = Og; g.Or = Or;
+  const int slab_lo = pol ? 0 : 2, nslab = pol ? p->nx : p->nx - 4;
+  const size_t per_slab = (size_t)g.K * g.nm * g.R * ((size_t)1 << g.logN) * sizeof(double2);
+  const int chunk = (int)std::max<size_t>(1, std::min<size_t>(nslab, kFftChunkBytes / per_slab));
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDev) return fail(1, "gw fft: device index out of range");
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  Workspace& ws = g_ws[dev];
+  const size_t need = per_slab * chunk;
+  if (ws.bytes < need) {
+    if (ws.p) HIPCHK(hipFree(ws.p));
+    ws.p = nullptr;
+    ws.bytes = 0;
+    if (hipMalloc(&ws.p, need) != hipSuccess) {
+      (void)hipGetLastError();
+      ws.p = nullptr;
+      return fail(3, "gw fft: cannot allocate " + std::to_string(need) + " bytes of spectra");
+    }
+    ws.bytes = need;
+  }
+  g.Z = reinterpret_cast<double2*>(ws.p);
+  hipLaunchKernelGGL(gwfft::k_fft_twiddles, dim3(gwfft::kTw / 256), dim3(256), 0, s);
+  HIPCHK(hipGetLastError());
+  GwArgs ob = args(p);  // off-band elements: as in the direct form
+  ob.Ol = Ol; ob.Og = Og; ob.Or = Or;
+  if (ob.nd < ob.nm - 1) {
+    if (pol) hipLaunchKernelGGL(k_pol_offband, offband_grid(ob.nm, (long long)ob.nx * ob.nen), dim3(256), 0, s, ob);
+    else hipLaunchKernelGGL(k_sig_offband, offband_grid(ob.nm, (long long)(ob.nx - 4) * ob.nen), dim3(256), 0, s, ob);
+    HIPCHK(hipGetLastError());
+  }
+  const int N = 1 << g.logN;
+  for (int s0 = 0; s0 < nslab; s0 += chunk) {
+    g.slab0 = slab_lo + s0;
+    g.nsl = std::min(chunk, nslab - s0);
+    const int RB = 1 << g.lrb;
+    const long long runs = (long long)g.nsl * g.nm * ((g.R + RB - 1) / RB);
+    const unsigned gy = (unsigned)std::min<long long>(runs, 65535);
+    const long long slots = (long long)g.nsl * g.nm * g.R;
+    const unsigned gp = (unsigned)(slots * std::max(1, N / 256));
+    const int EBg = std::min(4096 / RB, N), EBp = 2048 / RB, EBs = 1024 / RB;
+    if (pol) {
+      hipLaunchKernelGGL(k_fft_gather<true>, dim3(N / EBg, gy), dim3(256), 0, s, g);
+      fft_pass(s, g, 0, 2, false);
+      hipLaunchKernelGGL(k_pol_fft_product, dim3(gp), dim3(256), 0, s, g);
+      fft_pass(s, g, 1, 2, true);
+      hipLaunchKernelGGL(k_pol_fft_scatter, dim3((g.nen + EBp - 1) / EBp, gy), dim3(256), 0, s, g);
+    } else {
+      hipLaunchKernelGGL(k_fft_gather<false>, dim3(N / EBg, gy), dim3(256), 0, s, g);
+      fft_pass(s, g, 0, 5, false);
+      hipLaunchKernelGGL(k_sig_fft_product, dim3(gp), dim3(256), 0, s, g);
+      fft_pass(s, g, 2, 3, true);
+      hipLaunchKernelGGL(k_sig_fft_scatter, dim3((g.nen + EBs - 1) / EBs, gy), dim3(256), 0, s, g);
+    }
+    HIPCHK(hipGetLastError());
+  }
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -575,8 +995,13 @@ int gw_polarization(const gw_params* p, const double* Gl, const double* Gg, cons
                     double* Pg, double* Pr, void* stream) {
   g_err.clear();
   if (int rc = check(p, 1)) return rc;
-  if (p->nen > 3072) return fail(4, "gw_polarization: nen > 3072 (the energy series of an element live in LDS)");
+  int logN = 0;
+  if (p->nen > kPolDirectMax)
+    if (int rc = fft_log2(p, logN)) return rc;
   if (!Gl || !Gg || !Gr || !Pl || !Pg || !Pr) return fail(1, "gw_polarization: null array");
+  // the direct form keeps an element's series in LDS: longer grids take the FFT form
+  if (p->nen > kPolDirectMax)
+    return fft_step(p, true, Gl, Gg, Gr, nullptr, nullptr, nullptr, Pl, Pg, Pr, reinterpret_cast<hipStream_t>(stream));
   GwArgs a = args(p);
   a.slab0 = 0;
   a.dr = p->dE_P[0];
@@ -637,8 +1062,12 @@ int gw_selfenergy(const gw_params* p, const double* Gl, const double* Gg, const 
                   const double* Wg, const double* Wr, double* Sl, double* Sg, double* Sr, void* stream) {
   g_err.clear();
   if (int rc = check(p, 5)) return rc;
-  if (p->nen > 2040) return fail(4, "gw_selfenergy: nen > 2040 (the energy series of an element live in LDS)");
+  int logN = 0;
+  if (p->nen > kSigDirectMax)
+    if (int rc = fft_log2(p, logN)) return rc;
   if (!Gl || !Gg || !Gr || !Wl || !Wg || !Wr || !Sl || !Sg || !Sr) return fail(1, "gw_selfenergy: null array");
+  if (p->nen > kSigDirectMax)
+    return fft_step(p, false, Gl, Gg, Gr, Wl, Wg, Wr, Sl, Sg, Sr, reinterpret_cast<hipStream_t>(stream));
   GwArgs a = args(p);
   a.slab0 = 2;
   a.dr = p->dE_S[0];
@@ -661,6 +1090,26 @@ int gw_selfenergy(const gw_params* p, const double* Gl, const double* Gg, const 
     hipLaunchKernelGGL(k_selfenergy<2040>, dim3(grid8(nel)), dim3(NTH_S), 0, s, a);
   HIPCHK(hipGetLastError());
   return 0;
+}
+
+int gw_polarization_fft(const gw_params* p, const double* Gl, const double* Gg, const double* Gr, double* Pl,
+                        double* Pg, double* Pr, void* stream) {
+  g_err.clear();
+  if (int rc = check(p, 1)) return rc;
+  int logN = 0;
+  if (int rc = fft_log2(p, logN)) return rc;
+  if (!Gl || !Gg || !Gr || !Pl || !Pg || !Pr) return fail(1, "gw_polarization_fft: null array");
+  return fft_step(p, true, Gl, Gg, Gr, nullptr, nullptr, nullptr, Pl, Pg, Pr, reinterpret_cast<hipStream_t>(stream));
+}
+
+int gw_selfenergy_fft(const gw_params* p, const double* Gl, const double* Gg, const double* Gr, const double* Wl,
+                      const double* Wg, const double* Wr, double* Sl, double* Sg, double* Sr, void* stream) {
+  g_err.clear();
+  if (int rc = check(p, 5)) return rc;
+  int logN = 0;
+  if (int rc = fft_log2(p, logN)) return rc;
+  if (!Gl || !Gg || !Gr || !Wl || !Wg || !Wr || !Sl || !Sg || !Sr) return fail(1, "gw_selfenergy_fft: null array");
+  return fft_step(p, false, Gl, Gg, Gr, Wl, Wg, Wr, Sl, Sg, Sr, reinterpret_cast<hipStream_t>(stream));
 }
 
 long long gw_pairs_polarization(int nen, int nopmax) {

@@ -38,6 +38,8 @@ def lib():
                                       ctypes.c_double, ctypes.c_double]
     L.gw_polarization.argtypes = [P] + [vp] * 7
     L.gw_selfenergy.argtypes = [P] + [vp] * 10
+    L.gw_polarization_fft.argtypes = [P] + [vp] * 7
+    L.gw_selfenergy_fft.argtypes = [P] + [vp] * 10
     L.gw_pairs_polarization.argtypes = [ctypes.c_int, ctypes.c_int]
     L.gw_pairs_polarization.restype = ctypes.c_longlong
     L.gw_pairs_selfenergy.argtypes = [ctypes.c_int, ctypes.c_int]
@@ -84,17 +86,23 @@ def _stream():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
-def polarization(p, Gl, Gg, Gr, Pl, Pg, Pr):
-    """P^<, P^>, P^r into Pl, Pg, Pr (src/green_rgf.f90:3186-3221)."""
+def polarization(p, Gl, Gg, Gr, Pl, Pg, Pr, method="auto"):
+    """P^<, P^>, P^r into Pl, Pg, Pr (src/green_rgf.f90:3186-3221).  method "auto" (the
+    direct form up to nen = 3072, else the FFT form) or "fft" (the reference's
+    corr1d2(..., method='fft'), src/fft_mod.f90:56-90)."""
     a = [_arr(t, p, n) for t, n in ((Gl, "Gl"), (Gg, "Gg"), (Gr, "Gr"), (Pl, "Pl"), (Pg, "Pg"), (Pr, "Pr"))]
-    _check(lib().gw_polarization(ctypes.byref(p), *a, _stream()))
+    fn = {"auto": lib().gw_polarization, "fft": lib().gw_polarization_fft}[method]
+    _check(fn(ctypes.byref(p), *a, _stream()))
 
 
-def selfenergy(p, Gl, Gg, Gr, Wl, Wg, Wr, Sl, Sg, Sr):
-    """Sigma^<, Sigma^>, Sigma^r updated in place (src/green_rgf.f90:3233-3290)."""
+def selfenergy(p, Gl, Gg, Gr, Wl, Wg, Wr, Sl, Sg, Sr, method="auto"):
+    """Sigma^<, Sigma^>, Sigma^r updated in place (src/green_rgf.f90:3233-3290).  method
+    "auto" (the direct form up to nen = 2040, else the FFT form) or "fft" (conv1d2 with
+    method='fft', src/fft_mod.f90:182-212)."""
     a = [_arr(t, p, n) for t, n in ((Gl, "Gl"), (Gg, "Gg"), (Gr, "Gr"), (Wl, "Wl"), (Wg, "Wg"), (Wr, "Wr"),
                                     (Sl, "Sl"), (Sg, "Sg"), (Sr, "Sr"))]
-    _check(lib().gw_selfenergy(ctypes.byref(p), *a, _stream()))
+    fn = {"auto": lib().gw_selfenergy, "fft": lib().gw_selfenergy_fft}[method]
+    _check(fn(ctypes.byref(p), *a, _stream()))
 
 
 def to_device(a, device="cuda:0"):

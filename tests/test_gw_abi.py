@@ -23,7 +23,7 @@ def test_library_exports_every_declared_symbol():
         pytest.fail("libgw_hip.so not built (run __graft_entry__.build())")
     names = header_functions()
     assert names == ["gw_last_error", "gw_pairs_polarization", "gw_pairs_selfenergy", "gw_params_reference",
-                     "gw_polarization", "gw_release", "gw_selfenergy"]
+                     "gw_polarization", "gw_polarization_fft", "gw_release", "gw_selfenergy", "gw_selfenergy_fft"]
     L = ctypes.CDLL(gw.LIB_PATH)
     for name in names:
         assert hasattr(L, name), name
@@ -50,17 +50,25 @@ def test_pair_counts():
 
 
 @pytest.mark.parametrize("fn,kw,msg", [
-    ("gw_polarization", dict(nen=3074), "nen > 3072"),
     ("gw_polarization", dict(nopmax=1000), "nopmax"),
     ("gw_polarization", dict(ndiag=-1), "ndiag"),
-    ("gw_selfenergy", dict(nen=2042), "nen > 2040"),
     ("gw_selfenergy", dict(nx=4), "nx >= 5"),
     ("gw_polarization", dict(), "null array"),
+    # above the direct forms' limits the FFT form takes over (no size error), up to
+    # nen + nopmax = 2^24 points per series
+    ("gw_polarization", dict(nen=3074), "null array"),
+    ("gw_selfenergy", dict(nen=2042), "null array"),
+    ("gw_polarization", dict(nen=1 << 24, nopmax=2), "2^24"),
+    ("gw_selfenergy", dict(nen=(1 << 24) - 2, nopmax=3), "2^24"),
+    ("gw_polarization_fft", dict(nen=(1 << 24) - 8, nopmax=8), "null array"),
+    ("gw_polarization_fft", dict(nen=(1 << 24) - 8, nopmax=9), "2^24"),
+    ("gw_selfenergy_fft", dict(nx=4), "nx >= 5"),
+    ("gw_selfenergy_fft", dict(), "null array"),
 ])
 def test_invalid_arguments_fail_before_any_launch(fn, kw, msg):
     args = dict(nm=4, nx=6, nen=2000, ndiag=1, nopmax=990)
     args.update(kw)
     p = gw.params(args["nm"], args["nx"], args["nen"], args["ndiag"], args["nopmax"], -1j, 1j, 0.5)
-    n = 7 if fn == "gw_polarization" else 10
+    n = 7 if fn.startswith("gw_polarization") else 10
     rc = getattr(gw.lib(), fn)(ctypes.byref(p), *([None] * n))
     assert rc != 0 and msg in gw.lib().gw_last_error().decode()
