@@ -324,6 +324,41 @@ int rgf_integrate(int nk, int nE, int nx, const double *E, const double *midgap,
                   const double *dens_n, const double *dens_p, const double *tr, double scale,
                   double *nelec, double *pelec, double *current, void *stream);
 
+/* Screened interaction W of the GW loop: green_rgf_calc_w
+ * (src/green_rgf.f90:3399-3831) for a batch of energies (frequencies), one
+ * persistent workgroup per energy on the solve's GEMM and inverse blocks.  For
+ * each energy, M = I - V P^r is block-tridiagonal (P^r block-diagonal) and
+ *   W^r_ii  = diagonal blocks of M^-1 V by the reference's right-to-left /
+ *             left-to-right block recursion, whose off-diagonal step uses plain
+ *             transposes (src/green_rgf.f90:3771-3791): it gives the blocks of
+ *             (I - V P^r)^-1 V when V and P^r are symmetric, as in the reference;
+ *   W^<_ii  = W^r_ii P^<_ii W^r_ii^H - X_a P^<_b X_a^T - X_m^T P^<_m X_m with X the
+ *             imaginary part (times i) of W^r_{i,i+1}, a = i, b = i + 1 (a = nx - 1,
+ *             b = nx at the last slab), m = max(i - 1, 1) (1-based; 3798-3826);
+ *   W^>_ii  likewise with P^>.
+ * Every driver calls it with NBC = 0 and without off-diagonal P blocks
+ * (src/green_rgf.f90:846, 1551, 3008); those are the supported inputs.
+ * Device arrays of complex double in the reference layout: Vii, V1i (nm, nm, nx)
+ * with V1i block ix = V_{ix+1,ix}; PL, PG, PR, WL, WG, WR (nm, nm, nx, nE).
+ * Asynchronous on `stream`; any nm up to 512 (zero-padded internally to a
+ * multiple of 32, or of 64 above 256), nx >= 2.  Outputs must not alias inputs. */
+typedef struct {
+  int nm, nx, nE;
+  int nbc;             /* open-boundary blocks: 0 (others: RGF_EUNSUP) */
+  const double *Vii;   /* (nm, nm, nx) */
+  const double *V1i;   /* (nm, nm, nx) */
+  const double *PL;    /* (nm, nm, nx, nE) diagonal blocks of P^<, P^>, P^r */
+  const double *PG;
+  const double *PR;
+  double *WL;          /* (nm, nm, nx, nE) diagonal blocks of W^<, W^>, W^r */
+  double *WG;
+  double *WR;
+  double *WRi1;        /* optional: W^r_{ix,ix+1} (the reference's WRi1; slab nx zero) */
+  int *info;           /* optional device array [nE]: singular blocks met per energy */
+} rgf_w_batch;
+
+int rgf_w_solve_batched(const rgf_w_batch *b, void *stream);
+
 /* Release cached device buffers of this library on the current device. */
 void rgf_release(void);
 

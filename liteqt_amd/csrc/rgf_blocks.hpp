@@ -24,7 +24,7 @@ constexpr int KP = 16;    // K panel depth (complex)
 // the fragment reads of 16 rows x 2 k-steps per half-wave hit distinct 8-byte bank pairs.
 constexpr int LDK = KP + 2;
 
-enum { OP_N = 0, OP_C = 1 };
+enum { OP_N = 0, OP_C = 1, OP_T = 2 };  // op(X) = X, X^H, X^T
 
 // Explicit address spaces: global (1) and LDS (3).  Pointers that reach a
 // non-inlined function as generic pointers would be accessed with FLAT
@@ -432,7 +432,7 @@ __device__ __noinline__ double wg_gemm_k(int n_, const double2* __restrict__ A_,
                                          void* Cout_, Epi ep_, ldsd* lds) {
   using C = Cfg<T>;
   constexpr int MR = C::MR, MC = C::MC, LDA = C::LDA, LDB = C::LDB, STAGE = C::STAGE, TA = C::TA, TB = C::TB;
-  constexpr bool AKC = (OPA == OP_C), BKC = (OPB == OP_N);
+  constexpr bool AKC = (OPA != OP_N), BKC = (OPB == OP_N);
   const int n = uni(n_);
   const double2* A = uni(A_);
   const double2* B = uni(B_);
@@ -601,6 +601,17 @@ __device__ __noinline__ void wg_ew(int n, double2* out, int out_rowmajor, double
     if (D2) v = cadd(v, cmul(s2, gc(D2)[i + (size_t)j * n]));
     if (D3) v = cadd(v, cmul(s3, cconj(gc(D3)[j + (size_t)i * n])));
     gw(out)[out_rowmajor ? (size_t)i * n + j : (size_t)i + (size_t)j * n] = v;
+  }
+  __syncthreads();
+}
+
+// Y = i Im X (real parts zeroed); Z = X as well when Z is given.  Y, Z must not alias X.
+__device__ __noinline__ void wg_imag_part(int n, const double2* X, double2* Y, double2* Z) {
+  const int nn = n * n;
+  for (int idx = threadIdx.x; idx < nn; idx += blockDim.x) {
+    const double2 v = gc(X)[idx];
+    gw(Y)[idx] = make_double2(0.0, v.y);
+    if (Z) gw(Z)[idx] = v;
   }
   __syncthreads();
 }

@@ -21,6 +21,7 @@
 #include "rgf_multi.hpp"
 #include "rgf_offdiag.hpp"
 #include "rgf_sweep.hpp"
+#include "rgf_wsweep.hpp"
 
 namespace {
 
@@ -221,6 +222,49 @@ bool pick_variant_cms(int n, Variant* v) {
     case 192: *v = {launch_sweep_cms<3, 192>, sweep_cms_fn<3, 192>(), 3, 192, rgf::Cfg<3>::NTH}; return true;
     case 256: *v = {launch_sweep_cms<4, 256>, sweep_cms_fn<4, 256>(), 4, 256, rgf::Cfg<4>::NTH}; return true;
     default: *v = {launch_sweep_cms<1, 256>, sweep_cms_fn<1, 256>(), 1, 256, rgf::Cfg<1>::NTH}; return true;
+  }
+}
+
+// The screened-interaction sweep (green_rgf_calc_w) on the same tile variants.
+typedef void (*WLaunch)(const rgf::WArgs&, int grid, hipStream_t);
+template <int T, int NMAX>
+void launch_w(const rgf::WArgs& a, int grid, hipStream_t s) {
+  hipLaunchKernelGGL((rgf::rgf_wsweep<T, NMAX>), dim3(grid), dim3(rgf::Cfg<T>::NTH), 0, s, a);
+}
+struct WVariant {
+  WLaunch launch;
+  const void* fn;
+  int nth;
+};
+// Supported block size at or above n (the blocks are zero-padded up to it).
+int w_block(int n) {
+  if (n <= 256) return (n + 31) / 32 * 32;
+  return (n + 63) / 64 * 64;
+}
+bool pick_w(int n, WVariant* v) {
+#define WV(T, NM) {launch_w<T, NM>, reinterpret_cast<const void*>(&rgf::rgf_wsweep<T, NM>), rgf::Cfg<T>::NTH}
+  if (n > 256 && n <= kNmaxBig && n % 64 == 0) {
+    if (n % 128 == 0) *v = WV(4, kNmaxBig);
+    else *v = WV(2, kNmaxBig);
+    return true;
+  }
+  if (n % 32 != 0 || n < 32 || n > 256) return false;
+  switch (n) {
+    case 32: *v = WV(kOcc2 + 1, 32); return true;
+    case 64: *v = WV(kOcc2 + 2, 64); return true;
+    case 96: *v = WV(3, 96); return true;
+    case 128: *v = WV(4, 128); return true;
+    case 192: *v = WV(3, 192); return true;
+    case 256: *v = WV(4, 256); return true;
+    default: *v = WV(1, 256); return true;
+  }
+#undef WV
+}
+
+__global__ void conj_blocks(const double2* __restrict__ src, double2* __restrict__ dst, size_t count) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x) {
+    const double2 v = src[i];
+    dst[i] = make_double2(v.x, -v.y);
   }
 }
 
@@ -892,6 +936,10 @@ int rgf_cms(const double temp[2], int nm, int nx, double E, const double mu[2], 
                     GRi1, GLi1, GGi1);
 }
 
+}  // extern "C"
+
+namespace {
+
 // A per-energy host array of a host-array solve: energy e at base + e * stride (the
 // drivers' (nm, nm, nx, nen) arrays), or at list[e] (separate per-energy calls
 // gathered into one batch).
@@ -919,6 +967,10 @@ struct HostIO {
   EArr out[6];  // ldos, ndens, pdens, GRi1, GLi1, GGi1
   EArr cur, tr, tre, info;
 };
+
+}  // namespace
+
+extern "C" {
 
 static int rs_core(const rgf_params* p, const double temp[2], int nm, int nx, const double mu[2], const double* Sii,
                    const double* Hii, const double* H1i, const HostIO& io);
@@ -1398,6 +1450,104 @@ int rgf_zinv_batched(int n, int nbatch, const double* A, double* Ainv, int* info
       break;
   }
   HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int rgf_w_solve_batched(const rgf_w_batch* b, void* stream) {
+  g_err.clear();
+  if (!b) return fail(RGF_EINVAL, "rgf_w_solve_batched: batch is required");
+  if (b->nm < 1 || b->nx < 2 || b->nE < 0) return fail(RGF_EINVAL, "rgf_w_solve_batched: need nm >= 1, nx >= 2, nE >= 0");
+  if (b->nbc != 0)
+    return fail(RGF_EUNSUP, "rgf_w_solve_batched: open-boundary blocks (NBC > 0) are not supported (every driver passes 0)");
+  if (b->nm > kNmaxBig) return fail(RGF_EUNSUP, "rgf_w_solve_batched: nm > 512");
+  if (!b->Vii || !b->V1i || !b->PL || !b->PG || !b->PR || !b->WL || !b->WG || !b->WR)
+    return fail(RGF_EINVAL, "rgf_w_solve_batched: Vii, V1i, PL, PG, PR, WL, WG, WR are required");
+  if (b->nE == 0) return 0;
+  const int n = b->nm, np = w_block(n), nx = b->nx;
+  WVariant v;
+  if (!pick_w(np, &v)) return fail(RGF_EUNSUP, "rgf_w_solve_batched: unsupported block size");
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDev) return fail(RGF_EINVAL, "device index out of range");
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, dev));
+  int per_cu = 0;
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, v.fn, v.nth, 0));
+  if (per_cu < 1) return fail(RGF_EHIP, "W sweep kernel does not fit on a CU");
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  const size_t nn = (size_t)np * np;
+  const long long nblk = (long long)nx * b->nE;
+  // device layout: [conj(V1i) | padded V (2 nx) | padded P, W (6 or 7 nx nE) | energy slots]
+  const bool pad = np != n;
+  const size_t vc_el = nn * nx;
+  const size_t pad_el = pad ? nn * (2 * (size_t)nx + (b->WRi1 ? 7 : 6) * (size_t)nblk) : 0;
+  const long long slot_el = (long long)rgf::kWTmpBlocks * (long long)nn;
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  Buf& ws = g_ws[dev];
+  size_t fr = 0, tot = 0;
+  HIPCHK(hipMemGetInfo(&fr, &tot));
+  const size_t fixed = (vc_el + pad_el) * sizeof(double2);
+  const size_t avail = (size_t)((double)(fr + ws.bytes) * 0.9);
+  if (avail <= fixed) return fail(RGF_ENOMEM, "rgf_w_solve_batched: not enough device memory for the padded blocks");
+  long long slots = std::min<long long>(b->nE, (long long)prop.multiProcessorCount * per_cu);
+  slots = std::min<long long>(slots, (long long)((avail - fixed) / (slot_el * sizeof(double2))));
+  if (slots < 1) return fail(RGF_ENOMEM, "rgf_w_solve_batched: not enough device memory for one energy slot");
+  if (int rc = ws.ensure(fixed + (size_t)slots * slot_el * sizeof(double2))) return rc;
+  double2* base = reinterpret_cast<double2*>(ws.p);
+  rgf::WArgs a{};
+  a.n = np;
+  a.nx = nx;
+  a.nE = b->nE;
+  a.info = b->info;
+  double2* Vc = base;
+  double2* q = base + vc_el;
+  const unsigned g = 2048;
+  const double2 zero = make_double2(0, 0);
+  auto padded = [&](const double* src, long long nb) -> const double2* {
+    double2* d = q;
+    q += nn * nb;
+    hipLaunchKernelGGL(pad_blocks, dim3(g), dim3(256), 0, s, reinterpret_cast<const double2*>(src), n, d, np, nb, zero);
+    return d;
+  };
+  if (pad) {
+    a.Vii = padded(b->Vii, nx);
+    a.V1i = padded(b->V1i, nx);
+    a.PL = padded(b->PL, nblk);
+    a.PG = padded(b->PG, nblk);
+    a.PR = padded(b->PR, nblk);
+    a.WL = q; q += nn * nblk;
+    a.WG = q; q += nn * nblk;
+    a.WR = q; q += nn * nblk;
+    a.WRi1 = nullptr;
+    if (b->WRi1) { a.WRi1 = q; q += nn * nblk; }
+  } else {
+    a.Vii = reinterpret_cast<const double2*>(b->Vii);
+    a.V1i = reinterpret_cast<const double2*>(b->V1i);
+    a.PL = reinterpret_cast<const double2*>(b->PL);
+    a.PG = reinterpret_cast<const double2*>(b->PG);
+    a.PR = reinterpret_cast<const double2*>(b->PR);
+    a.WL = reinterpret_cast<double2*>(b->WL);
+    a.WG = reinterpret_cast<double2*>(b->WG);
+    a.WR = reinterpret_cast<double2*>(b->WR);
+    a.WRi1 = reinterpret_cast<double2*>(b->WRi1);
+  }
+  hipLaunchKernelGGL(conj_blocks, dim3(256), dim3(256), 0, s, a.V1i, Vc, vc_el);
+  a.Vc = Vc;
+  a.ws = base + vc_el + pad_el;
+  a.ws_slot = slot_el;
+  HIPCHK(hipGetLastError());
+  v.launch(a, (int)slots, s);
+  HIPCHK(hipGetLastError());
+  if (pad) {
+    const std::pair<const double2*, double*> outs[4] = {
+        {a.WL, b->WL}, {a.WG, b->WG}, {a.WR, b->WR}, {a.WRi1, b->WRi1}};
+    for (const auto& o : outs) {
+      if (!o.second) continue;
+      hipLaunchKernelGGL(unpad_blocks<double2>, dim3(g), dim3(256), 0, s, o.first, np, reinterpret_cast<double2*>(o.second),
+                         n, nblk);
+    }
+    HIPCHK(hipGetLastError());
+  }
   return 0;
 }
 

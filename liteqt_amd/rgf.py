@@ -61,13 +61,21 @@ class Scba(ctypes.Structure):
     _fields_ = [("D", ctypes.c_double), ("mix", ctypes.c_double)]
 
 
+class WBatch(ctypes.Structure):
+    _fields_ = [("nm", ctypes.c_int), ("nx", ctypes.c_int), ("nE", ctypes.c_int), ("nbc", ctypes.c_int),
+                ("Vii", ctypes.c_void_p), ("V1i", ctypes.c_void_p), ("PL", ctypes.c_void_p),
+                ("PG", ctypes.c_void_p), ("PR", ctypes.c_void_p), ("WL", ctypes.c_void_p),
+                ("WG", ctypes.c_void_p), ("WR", ctypes.c_void_p), ("WRi1", ctypes.c_void_p),
+                ("info", ctypes.c_void_p)]
+
+
 # every function declared in include/rgf.h
 EXPORTS = ("rgf_params_reference", "rgf_last_error", "rgf_version", "rgf_rs", "rgf_rs_ex",
            "rgf_rs_energies", "rgf_solve_batched", "rgf_workspace_bytes", "rgf_release",
            "rgf_zgemm_batched", "rgf_zinv_batched", "rgf_adjoint_batched", "rgf_adjoint_workspace_bytes",
            "rgf_scba_step", "rgf_graph_create", "rgf_graph_launch", "rgf_graph_destroy", "rgf_solve_devices",
            "rgf_graph_create_devices", "rgf_cms", "rgf_cms_ex", "rgf_cms_energies", "rgf_solve_batched_noalloc",
-           "rgf_integrate")
+           "rgf_integrate", "rgf_w_solve_batched")
 
 _lib = None
 
@@ -111,6 +119,7 @@ def lib():
     L.rgf_adjoint_batched.argtypes = [ctypes.POINTER(Params), ctypes.POINTER(Batch), vp, vp, vp, vp, vp, vp, vp]
     L.rgf_adjoint_workspace_bytes.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.rgf_adjoint_workspace_bytes.restype = ctypes.c_longlong
+    L.rgf_w_solve_batched.argtypes = [ctypes.POINTER(WBatch), vp]
     L.rgf_workspace_bytes.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.rgf_workspace_bytes.restype = ctypes.c_longlong
     _lib = L
@@ -443,6 +452,31 @@ def adjoint_batched(p, E, temp, mu, Hii, H1i, sigL=None, sigR=None, w_tr=None, w
 E0_REF = 1.6022e-19
 HBAR_REF = 1.0546e-34
 TPI_REF = 3.14159265359 * 2.0
+
+
+def w_solve_batched(Vii, V1i, PL, PG, PR, want_i1=False, stream=None):
+    """Screened interaction W of green_rgf_calc_w (src/green_rgf.f90:3399-3831, NBC = 0)
+    for a batch of energies on the GPU.  Device tensors in the reference layout (see
+    to_device_blocks): Vii, V1i (nx, nm, nm); PL, PG, PR (nE, nx, nm, nm).  Returns
+    dict WL, WG, WR (and WRi1) of shape (nE, nx, nm, nm) and info (nE,), asynchronous
+    on the current stream."""
+    import torch
+    nE, nx, nm = int(PR.shape[0]), int(PR.shape[1]), int(PR.shape[2])
+    dev = PR.device
+    c = torch.complex128
+    for t, name, numel in ((Vii, "Vii", nx * nm * nm), (V1i, "V1i", nx * nm * nm), (PL, "PL", nE * nx * nm * nm),
+                           (PG, "PG", nE * nx * nm * nm), (PR, "PR", nE * nx * nm * nm)):
+        _req(t, name, c, numel, dev)
+    out = {k: torch.empty_like(PR) for k in ("WL", "WG", "WR")}
+    if want_i1:
+        out["WRi1"] = torch.empty_like(PR)
+    out["info"] = torch.zeros(nE, dtype=torch.int32, device=dev)
+    b = WBatch(nm, nx, nE, 0, Vii.data_ptr(), V1i.data_ptr(), PL.data_ptr(), PG.data_ptr(), PR.data_ptr(),
+               out["WL"].data_ptr(), out["WG"].data_ptr(), out["WR"].data_ptr(),
+               out["WRi1"].data_ptr() if want_i1 else None, out["info"].data_ptr())
+    s = stream or torch.cuda.current_stream()
+    _check(lib().rgf_w_solve_batched(ctypes.byref(b), ctypes.c_void_p(s.cuda_stream)))
+    return out
 
 
 def current_scale(spindeg=2.0, nk=1):

@@ -8,6 +8,7 @@
 #include <string>
 
 #include "rgf_sweep.hpp"
+#include "rgf_wsweep.hpp"
 
 extern "C" int host_sweep(int cms, int mode, int nm, int nx, int nE, const double* E, const double* temp,
                           const double* mu, double eta_dev, double eta_lead, double dec_tol, int dec_nmax, double boltz,
@@ -48,6 +49,43 @@ extern "C" int host_sweep(int cms, int mode, int nm, int nx, int nE, const doubl
     rgf_sweep_cms<1, 4096>(a);
   else
     rgf_sweep<1, 4096>(a);
+  std::string all;
+  for (auto& v : hostcheck::violations()) all += v + "\n";
+  if (log && loglen > 0) {
+    const size_t m = std::min<size_t>(all.size(), (size_t)loglen - 1);
+    std::memcpy(log, all.data(), m);
+    log[m] = 0;
+  }
+  if (nops) *nops = hostcheck::ops();
+  return (int)hostcheck::violations().size();
+}
+
+// The screened-interaction program (rgf_wsweep.hpp) on the host: every energy in one
+// workspace slot, block pointers checked against Vii, V1i, conj(V1i), P, W and the slot.
+extern "C" int host_w(int nm, int nx, int nE, const double* Vii, const double* V1i, const double* Vc,
+                      const double* PL, const double* PG, const double* PR, double* WL, double* WG, double* WR,
+                      double* WRi1, int* info, long long* nops, char* log, int loglen) {
+  using namespace rgf;
+  hostcheck::reset();
+  const size_t nn = (size_t)nm * nm, vb = nn * 16 * nx, pb = vb * nE;
+  const long long slot = (long long)kWTmpBlocks * (long long)nn;
+  std::vector<double2> ws((size_t)slot);
+  hostcheck::add(Vii, vb, "Vii");
+  hostcheck::add(V1i, vb, "V1i");
+  hostcheck::add(Vc, vb, "Vc");
+  for (auto pp : {PL, PG, PR}) hostcheck::add(pp, pb, "P");
+  for (auto o : {WL, WG, WR, WRi1}) hostcheck::add(o, pb, "W");
+  hostcheck::add(ws.data(), (size_t)slot * 16, "workspace slot");
+  WArgs a;
+  std::memset(&a, 0, sizeof a);
+  a.n = nm; a.nx = nx; a.nE = nE;
+  a.Vii = (const double2*)Vii; a.V1i = (const double2*)V1i; a.Vc = (const double2*)Vc;
+  a.PL = (const double2*)PL; a.PG = (const double2*)PG; a.PR = (const double2*)PR;
+  a.WL = (double2*)WL; a.WG = (double2*)WG; a.WR = (double2*)WR; a.WRi1 = (double2*)WRi1;
+  a.info = info;
+  a.ws = ws.data();
+  a.ws_slot = slot;
+  rgf_wsweep<1, 4096>(a);
   std::string all;
   for (auto& v : hostcheck::violations()) all += v + "\n";
   if (log && loglen > 0) {

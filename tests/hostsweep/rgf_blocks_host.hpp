@@ -99,7 +99,7 @@ typedef double ldsd;
 typedef int ldsi;
 typedef std::complex<double> cd;
 
-enum { OP_N = 0, OP_C = 1 };
+enum { OP_N = 0, OP_C = 1, OP_T = 2 };
 
 template <int T>
 struct Cfg {
@@ -162,8 +162,10 @@ inline double wg_gemm(int n, int opA, const double2* A, int opB, const double2* 
   std::vector<cd> a(nn), b(nn), c(nn, cd(0, 0));
   for (int i = 0; i < n; ++i)
     for (int k = 0; k < n; ++k) {
-      a[i + (size_t)k * n] = opA == OP_N ? C(A[i + (size_t)k * n]) : std::conj(C(A[k + (size_t)i * n]));
-      b[i + (size_t)k * n] = opB == OP_N ? C(B[i + (size_t)k * n]) : std::conj(C(B[k + (size_t)i * n]));
+      a[i + (size_t)k * n] = opA == OP_N ? C(A[i + (size_t)k * n])
+                                         : (opA == OP_T ? C(A[k + (size_t)i * n]) : std::conj(C(A[k + (size_t)i * n])));
+      b[i + (size_t)k * n] = opB == OP_N ? C(B[i + (size_t)k * n])
+                                         : (opB == OP_T ? C(B[k + (size_t)i * n]) : std::conj(C(B[k + (size_t)i * n])));
     }
   for (int j = 0; j < n; ++j)
     for (int k = 0; k < n; ++k) {
@@ -191,6 +193,29 @@ inline double wg_gemm(int n, int opA, const double2* A, int opB, const double2* 
         ((double2*)Cout)[u] = D(v[i + (size_t)j * n]);
     }
   return ep.abs_sum ? asum : 0.0;
+}
+
+template <int T, int OPA, int OPB, int CRM>
+inline double wg_gemm_k(int n, const double2* A, const double2* B, void* Cout, Epi ep, ldsd* lds) {
+  return wg_gemm<T>(n, OPA, A, OPB, B, Cout, CRM, ep, lds);
+}
+
+inline void wg_imag_part(int n, const double2* X, double2* Y, double2* Z) {
+  using namespace hostcheck;
+  ++ops();
+  const size_t nn = (size_t)n * n, bytes = nn * 16;
+  need("wg_imag_part", "X", X, bytes);
+  need("wg_imag_part", "Y", Y, bytes);
+  no_alias("wg_imag_part", "Y aliases X", X, Y, bytes);
+  if (Z) {
+    need("wg_imag_part", "Z", Z, bytes);
+    no_alias("wg_imag_part", "Z aliases X", X, Z, bytes);
+  }
+  if (!X || !Y) return;
+  for (size_t q = 0; q < nn; ++q) {
+    Y[q] = make_double2(0.0, X[q].y);
+    if (Z) Z[q] = X[q];
+  }
 }
 
 inline void wg_ew(int n, double2* out, int out_rowmajor, double2 diag, const double2* D1, double2 s1,

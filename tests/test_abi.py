@@ -116,3 +116,21 @@ def test_overlap_solve_is_switched_off_before_any_device_call():
     p = r.params(r.MODE_COMPAT)
     assert L.rgf_solve_batched(ctypes.byref(p), ctypes.byref(b), None) == 4
     assert b"switched off" in L.rgf_last_error()
+
+
+def test_w_solve_argument_errors_before_any_device_call():
+    """rgf_w_solve_batched: open-boundary blocks are unsupported, nx >= 2, every array
+    required; all reported before the device is touched."""
+    from liteqt_amd import rgf as r
+    L = r.lib()
+    d = ctypes.c_void_p(16).value
+    b = r.WBatch(32, 4, 2, 1, d, d, d, d, d, d, d, d, None, None)
+    assert L.rgf_w_solve_batched(ctypes.byref(b), None) == 4  # RGF_EUNSUP
+    assert b"NBC" in L.rgf_last_error()
+    b.nbc, b.nx = 0, 1
+    assert L.rgf_w_solve_batched(ctypes.byref(b), None) == 1
+    b.nx, b.PG = 4, None
+    assert L.rgf_w_solve_batched(ctypes.byref(b), None) == 1
+    assert b"required" in L.rgf_last_error()
+    b.PG, b.nm = d, 600
+    assert L.rgf_w_solve_batched(ctypes.byref(b), None) == 4

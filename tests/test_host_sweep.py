@@ -27,6 +27,7 @@ HERE = os.path.join(ROOT, "tests", "hostsweep")
 def hostlib(tmp_path_factory):
     d = tmp_path_factory.mktemp("hostsweep")
     shutil.copy(os.path.join(ROOT, "liteqt_amd", "csrc", "rgf_sweep.hpp"), d / "rgf_sweep.hpp")
+    shutil.copy(os.path.join(ROOT, "liteqt_amd", "csrc", "rgf_wsweep.hpp"), d / "rgf_wsweep.hpp")
     shutil.copy(os.path.join(HERE, "rgf_blocks_host.hpp"), d / "rgf_blocks.hpp")
     shutil.copy(os.path.join(HERE, "host_sweep.cpp"), d / "host_sweep.cpp")
     so = d / "libhostsweep.so"
@@ -36,6 +37,7 @@ def hostlib(tmp_path_factory):
     vp, ci, cd = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
     L.host_sweep.argtypes = ([ci, ci, ci, ci, ci, vp, vp, vp, cd, cd, cd, ci, cd] + [vp] * 5 + [ctypes.c_longlong]
                              + [vp] * 13 + [ctypes.c_char_p, ci, ci])
+    L.host_w.argtypes = [ci, ci, ci] + [vp] * 12 + [ctypes.c_char_p, ci]
     return L
 
 
@@ -129,3 +131,26 @@ def test_checkpointed_program_in_bounds_and_equal_to_keep_all(hostlib, nx, K):
             assert n1 > n0  # segments were recomputed
         for k in ("ldos", "ndens", "pdens", "GRi1", "GLi1", "GGi1", "cur", "tr", "tre", "info"):
             assert np.array_equal(g[k], g0[k]), (k, nx, K, mode)
+
+
+@pytest.mark.parametrize("nm,nx,nE,sym", [(3, 2, 2, True), (4, 5, 3, False), (5, 7, 2, True)])
+def test_w_program_in_bounds_and_equal_to_oracle(hostlib, nm, nx, nE, sym):
+    """The screened-interaction program of rgf_wsweep.hpp: every block in bounds, the
+    allowed aliasing only, results equal to the W oracle."""
+    from oracle import w_oracle as wo
+    from tests.test_gpu_w import _inputs
+    rng = np.random.default_rng(nm + nx)
+    Vii, V1i, PL, PG, PR = (np.asfortranarray(x, np.complex128) for x in _inputs(rng, nm, nx, nE, sym))
+    Vc = np.asfortranarray(V1i.conj())
+    W = {k: np.zeros((nm, nm, nx, nE), np.complex128, order="F") for k in ("WL", "WG", "WR", "WRi1")}
+    info = np.zeros(nE, np.int32)
+    nops = ctypes.c_longlong(0)
+    log = ctypes.create_string_buffer(1 << 16)
+    nv = hostlib.host_w(nm, nx, nE, _p(Vii), _p(V1i), _p(Vc), _p(PL), _p(PG), _p(PR), _p(W["WL"]), _p(W["WG"]),
+                        _p(W["WR"]), _p(W["WRi1"]), _p(info), ctypes.byref(nops), log, len(log))
+    assert nv == 0, log.value.decode()
+    assert np.all(info == 0) and nops.value > 0
+    for e in range(nE):
+        ref = dict(zip(("WL", "WG", "WR", "WRi1"), wo.rgf_w(Vii, V1i, PL[..., e], PG[..., e], PR[..., e])))
+        for k, r in ref.items():
+            assert relerr(W[k][..., e], r) < 1e-12, (k, e)
