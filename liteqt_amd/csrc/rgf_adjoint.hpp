@@ -46,12 +46,23 @@ struct AdjArgs {
   double2* ws;
   long long ws_slot;
   int keep;  // 1: the products the reverse passes reuse are kept per slab (larger slot)
+  // segment recompute of the forward state (g, g<, S): 0 keeps it for every slab; K > 0
+  // keeps it at the slabs 0, K, 2K, ... <= nx-2 and nx-1 plus one K-slab segment, which
+  // the backward sweep and the two reverse passes refill from its checkpoint (keep = 0)
+  int ckpt;
 };
 
-// Blocks of a workspace slot: per slab g, g<, S, G, G<, gb, glb (+ R, R d, g S when
-// keep), then kAdjTmp temporaries.
-__host__ __device__ inline long long adj_slot_blocks(int nx, int keep) {
-  return (long long)(keep ? 10 : 7) * nx + kAdjTmp;
+// Forward-state slabs of a slot (g, g<, S each): nx, or the checkpoints, the last slab
+// and one segment.
+__host__ __device__ inline long long adj_state_slabs(int nx, int ckpt) {
+  if (ckpt <= 0) return nx;
+  return (long long)((nx - 2) / ckpt + 1) + 1 + ckpt;
+}
+
+// Blocks of a workspace slot: g, g<, S per state slab, G, G<, gb, glb per slab (+ R,
+// R d, g S per slab when keep), then kAdjTmp temporaries.
+__host__ __device__ inline long long adj_slot_blocks(int nx, int keep, int ckpt = 0) {
+  return 3 * adj_state_slabs(nx, ckpt) + (long long)(keep ? 7 : 4) * nx + kAdjTmp;
 }
 
 // Per energy, with d_l = H1i_l + Sigma^R_{l,l-1}, u_l = H1i_l^H + Sigma^R_{l-1,l},
@@ -75,18 +86,21 @@ __device__ void adjoint_energy(const AdjArgs& a, int e, double2* ws, double2* dH
   const double E = a.E[e];
   const double2 z = make_double2(E, a.eta_dev);
   const double2 one = make_double2(1, 0), mone = make_double2(-1, 0), zero = make_double2(0, 0);
+  const int KS = a.ckpt > 0 ? a.ckpt : 0;  // segment length (0: every slab kept)
+  const size_t NS = (size_t)adj_state_slabs(N, KS);
+  const int ncp = KS ? (N - 2) / KS + 1 : 0;  // checkpoint slabs 0, KS, .. <= N-2
   double2* gv = ws;
-  double2* glv = ws + (size_t)N * nn;
-  double2* Sv = ws + 2 * (size_t)N * nn;
-  double2* Gv = ws + 3 * (size_t)N * nn;
-  double2* Gnv = ws + 4 * (size_t)N * nn;
-  double2* gbv = ws + 5 * (size_t)N * nn;
-  double2* glbv = ws + 6 * (size_t)N * nn;
-  const bool keep = a.keep != 0;
-  double2* Rv = ws + 7 * (size_t)N * nn;   // keep only
-  double2* Rdv = ws + 8 * (size_t)N * nn;  // keep only
-  double2* Mv = ws + 9 * (size_t)N * nn;   // keep only
-  double2* tb = ws + (keep ? 10 : 7) * (size_t)N * nn;
+  double2* glv = ws + NS * nn;
+  double2* Sv = ws + 2 * NS * nn;
+  double2* Gv = ws + 3 * NS * nn;
+  double2* Gnv = Gv + (size_t)N * nn;
+  double2* gbv = Gnv + (size_t)N * nn;
+  double2* glbv = gbv + (size_t)N * nn;
+  const bool keep = a.keep != 0 && KS == 0;
+  double2* Rv = glbv + (size_t)N * nn;   // keep only
+  double2* Rdv = Rv + (size_t)N * nn;    // keep only
+  double2* Mv = Rdv + (size_t)N * nn;    // keep only
+  double2* tb = glbv + (size_t)(keep ? 4 : 1) * N * nn;
   auto B_ = [&](double2* base, int l) { return base + (size_t)l * nn; };
   auto Tm = [&](int k) { return tb + (size_t)k * nn; };
   double2 *H00 = Tm(0), *W = Tm(1), *T0 = Tm(2), *T1 = Tm(3), *T2 = Tm(4), *SL = Tm(5), *SR = Tm(6);
@@ -103,8 +117,16 @@ __device__ void adjoint_energy(const AdjArgs& a, int e, double2* ws, double2* dH
   auto sRlo = [&](int l) -> const double2* { return a.sigR_lo ? a.sigR_lo + so + (size_t)l * nn : nullptr; };
   auto sRup = [&](int l) -> const double2* { return a.sigR_up ? a.sigR_up + so + (size_t)l * nn : nullptr; };
   auto sLup = [&](int l) -> const double2* { return a.sigL_up ? a.sigL_up + so + (size_t)l * nn : nullptr; };
-  auto g = [&](int l) { return B_(gv, l); };
-  auto gl = [&](int l) { return B_(glv, l); };
+  // state slot of slab l: itself (K = 0), its checkpoint, the last-slab slot or the segment
+  auto st_ix = [&](int l) -> int {
+    if (!KS) return l;
+    if (l == N - 1) return ncp;
+    if (l % KS == 0) return l / KS;
+    return ncp + 1 + l % KS;
+  };
+  auto g = [&](int l) { return B_(gv, st_ix(l)); };
+  auto gl = [&](int l) { return B_(glv, st_ix(l)); };
+  auto Sst = [&](int l) { return B_(Sv, st_ix(l)); };
   auto Gf = [&](int l) { return l == N - 1 ? g(l) : B_(Gv, l); };    // G_ll
   auto Gnf = [&](int l) { return l == N - 1 ? gl(l) : B_(Gnv, l); };  // G<_ll
   const double wtr = a.w_tr ? a.w_tr[e] : 0.0;
@@ -163,13 +185,15 @@ __device__ void adjoint_energy(const AdjArgs& a, int e, double2* ws, double2* dH
   wg_ew(n, T1, 0, zero, Hb(0), one, sR(0), one, nullptr, one);
   wg_ew(n, W, 1, z, T1, mone, SL, mone, nullptr, one);  // z - (H_0 + sigR_0) - Sigma_L
   if (wg_inverse<T, NMAX>(n, W, g(0), lds)) ++fails;
-  wg_ew(n, B_(Sv, 0), 0, zero, SL, make_double2(-fL, 0), sL(0), one, SL, make_double2(fL, 0));
+  wg_ew(n, Sst(0), 0, zero, SL, make_double2(-fL, 0), sL(0), one, SL, make_double2(fL, 0));
   {
     double2* Mp = keep ? B_(Mv, 0) : T0;  // g S
-    wg_gemm<T>(n, OP_N, g(0), OP_N, B_(Sv, 0), Mp, 0, st, lds);
+    wg_gemm<T>(n, OP_N, g(0), OP_N, Sst(0), Mp, 0, st, lds);
     wg_gemm<T>(n, OP_N, Mp, OP_C, g(0), gl(0), 0, st, lds);
   }
-  for (int l = 1; l < N; ++l) {
+  // one forward step: g_l, g<_l, S_l from g_{l-1}, g<_{l-1} (also the segment refill;
+  // a singular slab is counted on the first pass only)
+  auto fwd_step = [&](int l, bool count) {
     const bool last = (l == N - 1);
     couple(l);
     wg_gemm<T>(n, OP_N, dP, OP_N, g(l - 1), T0, 0, st, lds);  // B = d g_{l-1}
@@ -180,7 +204,7 @@ __device__ void adjoint_energy(const AdjArgs& a, int e, double2* ws, double2* dH
       ew.D2 = T1;
     }
     wg_gemm<T>(n, OP_N, T0, uo(OP_N), uP, W, 1, ew, lds);         // z - H - Sigma^R - B u
-    if (wg_inverse<T, NMAX>(n, W, g(l), lds)) ++fails;
+    if (wg_inverse<T, NMAX>(n, W, g(l), lds) && count) ++fails;
     Epi es = epi_store();
     es.D1 = sL(l);
     if (sLup(l)) {
@@ -192,10 +216,26 @@ __device__ void adjoint_energy(const AdjArgs& a, int e, double2* ws, double2* dH
       es.D1 = T2;
     }
     wg_gemm<T>(n, OP_N, dP, OP_N, gl(l - 1), T1, 0, st, lds);  // Y = d g<_{l-1}
-    wg_gemm<T>(n, OP_N, T1, OP_C, dP, B_(Sv, l), 0, es, lds);  // S
+    wg_gemm<T>(n, OP_N, T1, OP_C, dP, Sst(l), 0, es, lds);     // S
     double2* Mp = keep ? B_(Mv, l) : T0;  // g S
-    wg_gemm<T>(n, OP_N, g(l), OP_N, B_(Sv, l), Mp, 0, st, lds);
+    wg_gemm<T>(n, OP_N, g(l), OP_N, Sst(l), Mp, 0, st, lds);
     wg_gemm<T>(n, OP_N, Mp, OP_C, g(l), gl(l), 0, st, lds);
+  };
+  for (int l = 1; l < N; ++l) fwd_step(l, true);
+  // segment refill: the forward state of slab l available (K > 0); the segment buffer
+  // holds one segment at a time, refilled from its checkpoint
+  int loaded = -1;
+  auto ensure = [&](int l) {
+    if (!KS || l == N - 1 || l % KS == 0) return;
+    const int c = l / KS;
+    if (c == loaded) return;
+    const int hi = c * KS + KS - 1 < N - 2 ? c * KS + KS - 1 : N - 2;
+    for (int m = c * KS + 1; m <= hi; ++m) fwd_step(m, false);
+    loaded = c;
+  };
+  if (KS) {  // the forward sweep leaves the segment of its last non-checkpoint slab <= N-2
+    const int lm = (N - 2) % KS == 0 ? N - 3 : N - 2;
+    loaded = lm >= 1 ? lm / KS : -1;
   }
   // tr = -Re Tr(KR Gn_N) - fR Re Tr(KR (G_N - G_N^H)),  KR = -(SR - SR^H)
   wg_ew(n, K, 0, zero, SR, mone, nullptr, one, SR, one);
@@ -211,6 +251,7 @@ __device__ void adjoint_energy(const AdjArgs& a, int e, double2* ws, double2* dH
 
   // ---- backward sweep, storing G_ll and G<_ll
   for (int l = N - 2; l >= 0; --l) {
+    ensure(l);
     const double2* s = sLup(l + 1);
     couple(l + 1);
     double2* Qp = keep ? B_(glbv, l) : Q;
@@ -238,6 +279,7 @@ __device__ void adjoint_energy(const AdjArgs& a, int e, double2* ws, double2* dH
   wg_ew(n, Gb, 0, zero, nullptr, one, nullptr, one, nullptr, one);
   wg_ew(n, Lb, 0, make_double2(0, wn(0)), nullptr, one, nullptr, one, nullptr, one);
   for (int l = 0; l < N - 1; ++l) {
+    ensure(l);
     const double2* s = sLup(l + 1);
     double2* Tbar = dH + (size_t)(l + 1) * nn;
     couple(l + 1);
@@ -288,9 +330,10 @@ __device__ void adjoint_energy(const AdjArgs& a, int e, double2* ws, double2* dH
 
   // ---- pass B: reverse of the forward sweep, l = N-1 .. 0
   for (int l = N - 1; l >= 0; --l) {
+    ensure(l);
     double2* gbl = (l == N - 1) ? Gb : B_(gbv, l);
     const double2* glbl = (l == N - 1) ? Lb : B_(glbv, l);
-    const double2* Sl = B_(Sv, l);
+    const double2* Sl = Sst(l);
     const double2* M = keep ? B_(Mv, l) : Tm(25);
     if (!keep) wg_gemm<T>(n, OP_N, g(l), OP_N, Sl, Tm(25), 0, st, lds);  // M = g S
     wg_gemm<T>(n, OP_N, glbl, OP_N, g(l), Mb, 0, st, lds);           // Mb = glb g
@@ -306,6 +349,7 @@ __device__ void adjoint_energy(const AdjArgs& a, int e, double2* ws, double2* dH
       if (threadIdx.x == 0) a.dU[(size_t)e * N + l] = -tr_ab;
     }
     if (l >= 1) {
+      ensure(l - 1);  // may replace slab l's segment: slab l's own blocks are no longer read
       const double2* s = sLup(l);
       double2* Tbar = dH + (size_t)l * nn;
       couple(l);

@@ -1317,6 +1317,8 @@ int rgf_adjoint_batched(const rgf_params* p, const rgf_batch* b, const double* w
   if (b && b->Sii) return fail(RGF_EUNSUP, "the adjoint does not take overlaps");
   if (!b || !b->E || !b->Hii || !b->H1i || !J || !dU) return fail(RGF_EINVAL, "missing arguments");
   if (b->nx < 2 || b->nE < 0) return fail(RGF_EINVAL, "need nx >= 2, nE >= 0");
+  if (b->ckpt < -1 || b->ckpt == 1)
+    return fail(RGF_EINVAL, "ckpt must be -1 (keep every slab), 0 (automatic) or a segment length >= 2");
   if (b->nE == 0) return 0;
   AdjVariant v;
   if (!pick_adj(b->nm, &v)) return fail(RGF_EUNSUP, "rgf_adjoint_batched: nm must be a multiple of 32 in [32, 256]");
@@ -1342,7 +1344,11 @@ int rgf_adjoint_batched(const rgf_params* p, const rgf_batch* b, const double* w
   // LITEQT_ADJ_RECOMPUTE=1 forces the small slot (the path taken under memory pressure)
   const char* rc_env = std::getenv("LITEQT_ADJ_RECOMPUTE");
   int keep = (rc_env && rc_env[0] == '1') ? 0 : 1;
-  long long slot_bytes = keep ? keep_bytes : small_bytes;
+  // segment recompute of the forward state: forced (ckpt > 0), or chosen below when
+  // even the small slot leaves CUs idle
+  int K = b->ckpt > 0 ? b->ckpt : 0;
+  if (K) keep = 0;
+  long long slot_bytes = K ? rgf::adj_slot_blocks(b->nx, 0, K) * nn * 16LL : keep ? keep_bytes : small_bytes;
   if ((size_t)(slots * slot_bytes) > ws.bytes || (size_t)(slots * dh_bytes) > dh.bytes) {
     size_t fr = 0, tot = 0;
     HIPCHK(hipMemGetInfo(&fr, &tot));
@@ -1351,7 +1357,17 @@ int rgf_adjoint_batched(const rgf_params* p, const rgf_batch* b, const double* w
       keep = 0;
       slot_bytes = small_bytes;
     }
-    const long long fit = (long long)(avail / (size_t)(slot_bytes + dh_bytes));
+    long long fit = (long long)(avail / (size_t)(slot_bytes + dh_bytes));
+    if (!K && b->ckpt == 0 && fit < slots) {
+      const int Ka = std::max(2, (int)std::ceil(std::sqrt((double)b->nx)));
+      const long long kb = rgf::adj_slot_blocks(b->nx, 0, Ka) * nn * 16LL;
+      const long long fit_k = (long long)(avail / (size_t)(kb + dh_bytes));
+      if (fit_k > fit) {
+        K = Ka;
+        slot_bytes = kb;
+        fit = fit_k;
+      }
+    }
     if (fit < 1) return fail(RGF_ENOMEM, "not enough device memory for one adjoint slot");
     slots = std::min(slots, fit);
     if ((size_t)(slots * slot_bytes) > ws.bytes || (size_t)(slots * dh_bytes) > dh.bytes) {
@@ -1383,6 +1399,7 @@ int rgf_adjoint_batched(const rgf_params* p, const rgf_batch* b, const double* w
   a.ws = reinterpret_cast<double2*>(ws.p);
   a.ws_slot = slot_bytes / 16LL;
   a.keep = keep;
+  a.ckpt = K;
   v.launch(a, (int)slots, s);
   HIPCHK(hipGetLastError());
   if (dH1i) {

@@ -9,6 +9,7 @@
 
 #include "rgf_sweep.hpp"
 #include "rgf_wsweep.hpp"
+#include "rgf_adjoint.hpp"
 
 extern "C" int host_sweep(int cms, int mode, int nm, int nx, int nE, const double* E, const double* temp,
                           const double* mu, double eta_dev, double eta_lead, double dec_tol, int dec_nmax, double boltz,
@@ -86,6 +87,51 @@ extern "C" int host_w(int nm, int nx, int nE, const double* Vii, const double* V
   a.ws = ws.data();
   a.ws_slot = slot;
   rgf_wsweep<1, 4096>(a);
+  std::string all;
+  for (auto& v : hostcheck::violations()) all += v + "\n";
+  if (log && loglen > 0) {
+    const size_t m = std::min<size_t>(all.size(), (size_t)loglen - 1);
+    std::memcpy(log, all.data(), m);
+    log[m] = 0;
+  }
+  if (nops) *nops = hostcheck::ops();
+  return (int)hostcheck::violations().size();
+}
+
+// The adjoint program (rgf_adjoint.hpp) on the host, physical mode, with the forward
+// state kept per slab (ckpt = 0) or at checkpoints with segment refills (ckpt = K).
+extern "C" int host_adjoint(int nm, int nx, int nE, const double* E, const double* temp, const double* mu,
+                            double eta_dev, double eta_lead, double dec_tol, int dec_nmax, double boltz,
+                            const double* Hii, const double* H1i, const double* sigL, const double* sigR,
+                            const double* w_tr, const double* w_n, double* J, double* tr, double* dU, double* dH,
+                            int keep, int ckpt, long long* nops, char* log, int loglen) {
+  using namespace rgf;
+  hostcheck::reset();
+  const size_t nn = (size_t)nm * nm, blk = nn * 16 * nx;
+  const long long slot = adj_slot_blocks(nx, keep && ckpt == 0, ckpt) * (long long)nn;
+  std::vector<double2> ws((size_t)slot);
+  hostcheck::add(Hii, blk, "Hii");
+  hostcheck::add(H1i, blk, "H1i");
+  hostcheck::add(sigL, blk * nE, "sigL");
+  hostcheck::add(sigR, blk * nE, "sigR");
+  hostcheck::add(dH, blk, "dH");
+  hostcheck::add(ws.data(), (size_t)slot * 16, "workspace slot");
+  AdjArgs a;
+  std::memset(&a, 0, sizeof a);
+  a.n = nm; a.nx = nx; a.nE = nE; a.dec_nmax = dec_nmax;
+  a.eta_dev = eta_dev; a.eta_lead = eta_lead; a.dec_tol = dec_tol; a.boltz = boltz;
+  a.temp[0] = temp[0]; a.temp[1] = temp[1]; a.mu[0] = mu[0]; a.mu[1] = mu[1];
+  a.E = E;
+  a.Hii = (const double2*)Hii; a.H1i = (const double2*)H1i;
+  a.sigL = (const double2*)sigL; a.sigR = (const double2*)sigR;
+  a.sig_stride = sigL ? (long long)nn * nx : 0;
+  a.w_tr = w_tr; a.w_n = w_n; a.J = J; a.tr = tr; a.dU = dU;
+  a.dH = (double2*)dH;
+  a.ws = ws.data();
+  a.ws_slot = slot;
+  a.keep = keep;
+  a.ckpt = ckpt;
+  rgf_adjoint<1, 4096>(a);
   std::string all;
   for (auto& v : hostcheck::violations()) all += v + "\n";
   if (log && loglen > 0) {

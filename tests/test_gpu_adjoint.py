@@ -226,3 +226,36 @@ def test_adjoint_recompute_slot_is_bit_identical(offdiag, monkeypatch):
         outs.append((J.clone(), dU.clone(), dH.clone()))
     for a, b in zip(*outs):
         assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("offdiag,nx,K", [(False, 11, 3), (True, 9, 4), (False, 7, 2), (True, 13, 5)])
+def test_adjoint_checkpointed_forward_state_is_bit_identical(offdiag, nx, K):
+    """The forward state (g, g^<, S) kept only at checkpoints every K slabs, refilled a
+    segment at a time by the backward sweep and both reverse passes: J, dU and dH equal
+    the keep-every-slab adjoint bit for bit."""
+    torch, rgf = _env()
+    dev = torch.device("cuda:0")
+    nm = 64
+    E = np.array([-0.25, 0.05, 0.3, 0.45])
+    print(f"adjoint checkpoints offdiag={offdiag} nx={nx} K={K}", flush=True)
+    if offdiag:
+        Hii, H1i, sig = _od_case(nm, nx, 95, 0.0)
+        names = ("sigL", "sigR", "sigR_lo", "sigR_up", "sigL_up")
+        sd = {k: rgf.to_device_blocks(v, dev) for k, v in zip(names, sig)}
+    else:
+        Hii, H1i, sL, sR = _case(nm, nx, 97, scatter=True)
+        sd = {"sigL": rgf.to_device_blocks(sL, dev), "sigR": rgf.to_device_blocks(sR, dev)}
+    p = rgf.params(rgf.MODE_PHYSICAL, eta_dev=1e-4, eta_lead=1e-4, dec_tol=1e-12)
+    rng = np.random.default_rng(7)
+    w_tr = torch.tensor(rng.standard_normal(len(E)), device=dev)
+    w_n = torch.tensor(rng.standard_normal((len(E), nx)), device=dev)
+    args = (p, torch.tensor(E, dtype=torch.float64, device=dev), (300.0, 250.0), (0.1, -0.1),
+            rgf.to_device_blocks(Hii, dev), rgf.to_device_blocks(H1i, dev))
+    outs = []
+    for ck in (-1, K):
+        rgf.lib().rgf_release()  # the slot size follows ckpt
+        J, dU, dH = rgf.adjoint_batched(*args, w_tr=w_tr, w_n=w_n, ckpt=ck, **sd)
+        torch.cuda.synchronize()
+        outs.append((J.clone(), dU.clone(), dH.clone()))
+    for a, b in zip(*outs):
+        assert torch.equal(a, b)

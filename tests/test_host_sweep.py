@@ -28,6 +28,7 @@ def hostlib(tmp_path_factory):
     d = tmp_path_factory.mktemp("hostsweep")
     shutil.copy(os.path.join(ROOT, "liteqt_amd", "csrc", "rgf_sweep.hpp"), d / "rgf_sweep.hpp")
     shutil.copy(os.path.join(ROOT, "liteqt_amd", "csrc", "rgf_wsweep.hpp"), d / "rgf_wsweep.hpp")
+    shutil.copy(os.path.join(ROOT, "liteqt_amd", "csrc", "rgf_adjoint.hpp"), d / "rgf_adjoint.hpp")
     shutil.copy(os.path.join(HERE, "rgf_blocks_host.hpp"), d / "rgf_blocks.hpp")
     shutil.copy(os.path.join(HERE, "host_sweep.cpp"), d / "host_sweep.cpp")
     so = d / "libhostsweep.so"
@@ -38,6 +39,8 @@ def hostlib(tmp_path_factory):
     L.host_sweep.argtypes = ([ci, ci, ci, ci, ci, vp, vp, vp, cd, cd, cd, ci, cd] + [vp] * 5 + [ctypes.c_longlong]
                              + [vp] * 13 + [ctypes.c_char_p, ci, ci])
     L.host_w.argtypes = [ci, ci, ci] + [vp] * 12 + [ctypes.c_char_p, ci]
+    L.host_adjoint.argtypes = ([ci, ci, ci, vp, vp, vp, cd, cd, cd, ci, cd] + [vp] * 10 + [ci, ci, vp,
+                               ctypes.c_char_p, ci])
     return L
 
 
@@ -154,3 +157,37 @@ def test_w_program_in_bounds_and_equal_to_oracle(hostlib, nm, nx, nE, sym):
         ref = dict(zip(("WL", "WG", "WR", "WRi1"), wo.rgf_w(Vii, V1i, PL[..., e], PG[..., e], PR[..., e])))
         for k, r in ref.items():
             assert relerr(W[k][..., e], r) < 1e-12, (k, e)
+
+
+@pytest.mark.parametrize("nx,K", [(7, 2), (9, 3), (10, 4), (6, 4)])
+def test_adjoint_checkpointed_program_in_bounds_and_equal_to_keep_all(hostlib, nx, K):
+    """The adjoint with its forward state kept only at checkpoints every K slabs (the
+    backward sweep and both reverse passes refill one segment at a time): every block
+    in bounds, and J, dU, dH bit-identical to the keep-all program (small and large
+    slot)."""
+    nm, nE = 4, 2
+    Hii, H1i = configs.random_device(nm, nx, 30 + nx)
+    sL, sG, sR = configs.scattering_self_energies(nm, nx, 31, E=np.array([-0.3, 0.2]), scale=0.05)
+    E = np.array([-0.3, 0.2])
+    temp, mu = np.array([300.0, 250.0]), np.array([0.1, -0.1])
+    w_tr = np.array([1.0, 0.5])
+    w_n = np.random.default_rng(1).standard_normal((nE, nx))
+    outs = {}
+    for keep, ck in ((1, 0), (0, 0), (0, K)):
+        J, tr, dU = np.zeros(nE), np.zeros(nE), np.zeros((nE, nx))
+        dH = np.zeros((nm, nm, nx), np.complex128, order="F")
+        nops = ctypes.c_longlong(0)
+        log = ctypes.create_string_buffer(1 << 16)
+        nv = hostlib.host_adjoint(nm, nx, nE, _p(E), _p(temp), _p(mu), 1e-4, 1e-2, 1e-10, 100, 8.61734e-5,
+                                  _p(Hii), _p(H1i), _p(sL), _p(sR), _p(w_tr), _p(w_n), _p(J), _p(tr), _p(dU), _p(dH),
+                                  keep, ck, ctypes.byref(nops), log, len(log))
+        assert nv == 0, log.value.decode()
+        outs[(keep, ck)] = (J, tr, dU, dH, nops.value)
+    base = outs[(0, 0)]
+    for key in ((1, 0), (0, K)):
+        for a, b in zip(outs[key][:4], base[:4]):
+            assert np.array_equal(a, b), key
+    if (nx - 3) // K >= 1:  # more than one segment: the refills ran
+        assert outs[(0, K)][4] > base[4]
+    else:  # one segment stays in the buffer from the forward sweep on
+        assert outs[(0, K)][4] == base[4]
