@@ -371,6 +371,54 @@ def random_device_torch(torch, nm, nx, gen, dev, u_range=0.3, coupling_sigma=0.5
     return H.contiguous(), C.contiguous()
 
 
+class _Progress:
+    """Long launches (the 64 devices at their full energy counts take minutes): a
+    thread counts the energies whose decimation count is already written (info[:, 0]
+    >= 1, zero before the solve) and prints the count every `every` seconds.  It reads
+    the info arrays with asynchronous copies into pinned host memory on a stream of its
+    own created non-blocking (it does not wait for the launch on the default stream)."""
+
+    def __init__(self, torch, infos, label, every=45.0):
+        import ctypes
+        import threading
+        self.torch, self.infos, self.label, self.every = torch, infos, label, every
+        self.hip = ctypes.CDLL("libamdhip64.so")
+        self.hip.hipStreamCreateWithFlags.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint]
+        self.hip.hipMemcpyAsync.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int,
+                                            ctypes.c_void_p]
+        self.hip.hipStreamSynchronize.argtypes = [ctypes.c_void_p]
+        self.hip.hipStreamDestroy.argtypes = [ctypes.c_void_p]
+        self.host = [torch.empty(i.shape, dtype=i.dtype, pin_memory=True) for i in infos]
+        self.stop = threading.Event()
+        self.t = threading.Thread(target=self._run, daemon=True)
+
+    def _run(self):
+        import ctypes
+        st = ctypes.c_void_p()
+        if self.hip.hipStreamCreateWithFlags(ctypes.byref(st), 1) != 0:  # hipStreamNonBlocking
+            return
+        total = sum(int(i.shape[0]) for i in self.infos)
+        while not self.stop.wait(self.every):
+            for d, h in zip(self.infos, self.host):  # hipMemcpyDeviceToHost = 2
+                self.hip.hipMemcpyAsync(ctypes.c_void_p(h.data_ptr()), ctypes.c_void_p(d.data_ptr()),
+                                        d.numel() * d.element_size(), 2, st)
+            self.hip.hipStreamSynchronize(st)
+            done = sum(int((h[:, 0] > 0).sum()) for h in self.host)
+            print(f"bench: {self.label}: {done} of {total} energies solved", file=sys.stderr, flush=True)
+        self.hip.hipStreamDestroy(st)
+
+    def __enter__(self):
+        for i in self.infos:
+            i.zero_()
+        self.torch.cuda.synchronize()
+        self.t.start()
+        return self
+
+    def __exit__(self, *exc):
+        self.stop.set()
+        self.t.join()
+
+
 def devices_cost(rgf, torch, dev, ndev, e_div, compare, graph=True, seed=3):
     """64 uneven devices in one persistent launch: nm in {64, 96, 128, 192, 256},
     nx ~ U[50, 400], energy count ~ U[256, 2048] divided by e_div (energies in
@@ -400,14 +448,18 @@ def devices_cost(rgf, torch, dev, ndev, e_div, compare, graph=True, seed=3):
         devs.append({"E": E, "temp": temp, "mu": mu, "Hii": Hd, "H1i": Cd, "out": out})
         shapes.append((nm, nx, nE))
     items = sum(s[2] for s in shapes)
-    t0 = time.perf_counter()
-    tiles, rep = rgf.solve_devices(p, devs, autotune=True)  # includes the autotuner's first timing
+    infos = [d["out"]["info"] for d in devs]
     torch.cuda.synchronize()
-    t_first = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    tiles, rep = rgf.solve_devices(p, devs, autotune=True)
-    torch.cuda.synchronize()
-    t_one = time.perf_counter() - t0
+    with _Progress(torch, infos, "devices first call"):
+        t0 = time.perf_counter()
+        tiles, rep = rgf.solve_devices(p, devs, autotune=True)  # includes the autotuner's first timing
+        torch.cuda.synchronize()
+        t_first = time.perf_counter() - t0
+    with _Progress(torch, infos, "devices one launch"):
+        t0 = time.perf_counter()
+        tiles, rep = rgf.solve_devices(p, devs, autotune=True)
+        torch.cuda.synchronize()
+        t_one = time.perf_counter() - t0
     print(f"bench: {ndev} devices, {items} energies, one launch {t_one:.2f}s (first call {t_first:.2f}s)",
           file=sys.stderr, flush=True)
     flops = 0.0
@@ -421,12 +473,14 @@ def devices_cost(rgf, torch, dev, ndev, e_div, compare, graph=True, seed=3):
         torch.cuda.synchronize()
         rgf.lib().rgf_release()  # the graph holds its own workspace
         g = rgf.DevicesGraph(p, devs, gs, autotune=True)
-        g.launch(1)
-        gs.synchronize()
-        t0 = time.perf_counter()
-        g.launch(1)
-        gs.synchronize()
-        t_graph = time.perf_counter() - t0
+        with _Progress(torch, infos, "devices graph warm-up replay"):
+            g.launch(1)
+            gs.synchronize()
+        with _Progress(torch, infos, "devices graph replay"):
+            t0 = time.perf_counter()
+            g.launch(1)
+            gs.synchronize()
+            t_graph = time.perf_counter() - t0
         g.close()
         graph_same = bool(all(torch.equal(a, d["out"]["tr"]) for a, d in zip(tr_one, devs)))
         print(f"bench: {ndev} devices, graph replay {t_graph:.2f}s", file=sys.stderr, flush=True)
@@ -441,6 +495,8 @@ def devices_cost(rgf, torch, dev, ndev, e_div, compare, graph=True, seed=3):
             torch.cuda.synchronize()
             dt = time.perf_counter() - t1
             per_device.append({"nm": nm, "nx": nx, "energies": nE, "s_alone": dt, "energy_points_per_s_alone": nE / dt})
+            print(f"bench: device {len(per_device)}/{ndev} nm={nm} nx={nx} {nE} energies alone {dt:.2f}s",
+                  file=sys.stderr, flush=True)
         t_seq = time.perf_counter() - t0
         print(f"bench: {ndev} devices one launch each {t_seq:.2f}s", file=sys.stderr, flush=True)
         same = bool(all(torch.equal(a, d["out"]["tr"]) for a, d in zip(tr_one, devs)))

@@ -84,6 +84,17 @@ int gw_polarization_fft(const gw_params* p, const double* Gl, const double* Gg, 
 int gw_selfenergy_fft(const gw_params* p, const double* Gl, const double* Gg, const double* Gr, const double* Wl,
                       const double* Wg, const double* Wr, double* Sl, double* Sg, double* Sr, void* stream);
 
+/* Dense screened interaction of the one-dimensional driver, green_calc_w_full
+ * (src/green_rgf.f90:3347-3396, called per energy at 3235), for nE energies at once:
+ * V the block-tridiagonal bare interaction (Vii diagonal blocks, V1i block ix =
+ * V_{ix+1,ix}, its conjugate transpose above the diagonal), P^{<,>,r} block-diagonal
+ * (nm, nm, nx, nE), M = I - V P^r, W^r = M^-1 V (solved from the LU factors),
+ * W^{<,>} = W^r P^{<,>} W^r^H; the diagonal blocks (nm, nm, nx, nE) are written.  Dense
+ * NT x NT algebra (NT = nm nx <= 16384) by rocBLAS / rocSOLVER over batches of energies;
+ * workspace 3 NT^2 complex per energy of a batch (up to 64), kept per device. */
+int gw_w_full(int nm, int nx, int nE, const double* Vii, const double* V1i, const double* PL, const double* PG,
+              const double* PR, double* WL, double* WG, double* WR, void* stream);
+
 /* Valid (ie, nop) pairs summed per band element (algorithmic work): for P and for Sigma. */
 long long gw_pairs_polarization(int nen, int nopmax);
 long long gw_pairs_selfenergy(int nen, int nopmax);

@@ -974,6 +974,12 @@ int fft_step(const gw_params* p, bool pol, const double* Gl, const double* Gg, c
 
 }  // namespace
 
+// The error state shared with the other sources of this library (gw_wfull.hip).
+namespace gwerr {
+int fail(int code, const std::string& msg) { return ::fail(code, msg); }
+void clear() { g_err.clear(); }
+}  // namespace gwerr
+
 extern "C" {
 
 void gw_params_reference(gw_params* p, int nm, int nx, int nen, int ndiag, double dE, double spindeg,

@@ -24,7 +24,8 @@ def read_example_input(path):
     """Values of the example input file by their labels."""
     out = {}
     labels = {"num cell in sancho": "ns", "length num cells": "length", "emin emax num_en": "grid",
-              "read potential from pot_dat": "lreadpot", "mu_s mu_d": "mu", "temperature": "temp"}
+              "read potential from pot_dat": "lreadpot", "mu_s mu_d": "mu", "temperature": "temp",
+              "num iter": "niter", "epsilon": "eps", "r0": "r0", "alpha mix": "alpha", "only diag": "ldiag"}
     lines = open(path).read().splitlines()
     for k, ln in enumerate(lines):
         if "!" not in ln:
@@ -39,7 +40,13 @@ def read_example_input(path):
     return {"ns": int(out["ns"][0]), "length": int(out["length"][0]),
             "emin": float(out["grid"][0]), "emax": float(out["grid"][1]), "nen": int(out["grid"][2]),
             "lreadpot": out["lreadpot"][0].upper().startswith("T"), "potscale": out.get("potscale", 1.0),
-            "mu": (float(out["mu"][0]), float(out["mu"][1])), "temp": (float(out["temp"][0]), float(out["temp"][1]))}
+            "mu": (float(out["mu"][0]), float(out["mu"][1])), "temp": (float(out["temp"][0]), float(out["temp"][1])),
+            # the GW loop's values (src/main.f90:58-66)
+            "niter": int(out["niter"][0]) if "niter" in out else 0,
+            "eps": float(out["eps"][0]) if "eps" in out else 1.0,
+            "r0": float(out["r0"][0]) if "r0" in out else 1.0,
+            "alpha": float(out["alpha"][0]) if "alpha" in out else 0.5,
+            "ldiag": out["ldiag"][0].upper().startswith("T") if "ldiag" in out else True}
 
 
 def run(example_dir, out_dir, nen=None, chunk=250):

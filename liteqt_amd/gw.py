@@ -40,6 +40,7 @@ def lib():
     L.gw_selfenergy.argtypes = [P] + [vp] * 10
     L.gw_polarization_fft.argtypes = [P] + [vp] * 7
     L.gw_selfenergy_fft.argtypes = [P] + [vp] * 10
+    L.gw_w_full.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int] + [vp] * 9
     L.gw_pairs_polarization.argtypes = [ctypes.c_int, ctypes.c_int]
     L.gw_pairs_polarization.restype = ctypes.c_longlong
     L.gw_pairs_selfenergy.argtypes = [ctypes.c_int, ctypes.c_int]
@@ -103,6 +104,28 @@ def selfenergy(p, Gl, Gg, Gr, Wl, Wg, Wr, Sl, Sg, Sr, method="auto"):
                                     (Sl, "Sl"), (Sg, "Sg"), (Sr, "Sr"))]
     fn = {"auto": lib().gw_selfenergy, "fft": lib().gw_selfenergy_fft}[method]
     _check(fn(ctypes.byref(p), *a, _stream()))
+
+
+def w_full(Vii, V1i, PL, PG, PR, WL=None, WG=None, WR=None):
+    """Dense screened interaction of the one-dimensional driver, green_calc_w_full
+    (src/green_rgf.f90:3347-3396), for every energy of the (nen, nx, nm, nm) device
+    tensors PL, PG, PR; Vii, V1i are (nx, nm, nm) device tensors (the memory of Fortran
+    (nm, nm, nx) arrays).  Returns (WL, WG, WR) diagonal blocks, shaped like P."""
+    import torch
+    nen, nx, nm = int(PR.shape[0]), int(PR.shape[1]), int(PR.shape[2])
+    for t, name in ((PL, "PL"), (PG, "PG"), (PR, "PR")):
+        if t.dtype != torch.complex128 or not t.is_cuda or not t.is_contiguous() or tuple(t.shape) != (nen, nx, nm, nm):
+            raise GwError(f"{name}: need a contiguous complex128 CUDA tensor of shape {(nen, nx, nm, nm)}")
+    for t, name in ((Vii, "Vii"), (V1i, "V1i")):
+        if t.dtype != torch.complex128 or not t.is_cuda or not t.is_contiguous() or tuple(t.shape) != (nx, nm, nm):
+            raise GwError(f"{name}: need a contiguous complex128 CUDA tensor of shape {(nx, nm, nm)}")
+    WL = torch.empty_like(PR) if WL is None else WL
+    WG = torch.empty_like(PR) if WG is None else WG
+    WR = torch.empty_like(PR) if WR is None else WR
+    ptr = lambda t: ctypes.c_void_p(t.data_ptr())
+    _check(lib().gw_w_full(nm, nx, nen, ptr(Vii), ptr(V1i), ptr(PL), ptr(PG), ptr(PR), ptr(WL), ptr(WG), ptr(WR),
+                           _stream()))
+    return WL, WG, WR
 
 
 def to_device(a, device="cuda:0"):

@@ -14,6 +14,10 @@
   slabs, the potential pot[l] * potscale added to the first nb diagonal entries of
   slab l (the reference's loop runs over nb, not over the nb*ns orbitals of a slab).
 
+* bare_coulomb / coulomb_blocks / device_coulomb: the bare Coulomb interaction between
+  Wannier centres (src/wannierHam.f90:464-488), its slab blocks (w90_bare_coulomb_blocks,
+  368-397) and the RGF driver's device copies (src/main.f90:531-537), the V of the GW loop.
+
 The transverse wave vector is an argument (the reference's driver passes kt_CBM from a
 band-structure scan, which is 0 for a one-dimensional system such as the (8,0) tube of
 examples/cnt_8-0).
@@ -135,3 +139,57 @@ def device_blocks(w, ns, length, pot=None, potscale=1.0, kx=0.0, ky=0.0):
         idx = np.arange(w.nb)
         Hii[idx, idx, :] += (np.asarray(pot[:length]) * potscale)[None, :]
     return Hii, H1i
+
+
+# constants of bare_coulomb (src/wannierHam.f90:468-471); epsilon0 is a default-real
+# (single precision) literal there, so its value is that of the float32 rounding
+_PI_COUL = 3.14159265359
+_E_COUL = 1.6e-19
+_EPS0_COUL = float(np.float32(8.85e-12))
+
+
+def bare_coulomb(w, a1, a2, eps, r0, ldiag):
+    """(nb, nb) real bare Coulomb matrix between the centres of cell 0 and the cell at
+    a1 alpha + a2 beta, in eV (src/wannierHam.f90:464-488): e / (4 pi eps0 eps |r| 1e-10)
+    tanh(|r| / r0) with r = a1 alpha + a2 beta + c_i - c_j (Angstrom); at r = 0 the
+    self-interaction e / (4 pi eps0 eps 1e-10) / r0 when ldiag, else 0.1."""
+    alpha, beta = w.cell[:, 0], w.cell[:, 1]
+    r = (a1 * alpha + a2 * beta)[:, None, None] + w.centers[:, :, None] - w.centers[:, None, :]
+    normr = np.sqrt((r * r).sum(axis=0))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        v = _E_COUL / (4.0 * _PI_COUL * _EPS0_COUL * eps * normr * 1.0e-10) * np.tanh(normr / r0)
+    self_v = _E_COUL / (4.0 * _PI_COUL * _EPS0_COUL * eps * 1.0e-10) * (1.0 / r0) if ldiag else 0.1
+    return np.where(normr > 0.0, v, self_v)
+
+
+def coulomb_blocks(w, ns, eps, r0, ldiag):
+    """(Vii, V1i) of a slab of ns cells (w90_bare_coulomb_blocks at kx = ky = 0,
+    src/wannierHam.f90:368-397): block (i, k) of Vii sums bare_coulomb(i-k, j) over the
+    Hamiltonian's y offsets j when i-k lies in its x range; block (i, k) of V1i sums the
+    same bare_coulomb(i-k, j) when i-k+ns lies in that range (the reference passes i-k,
+    not i-k+ns, there)."""
+    nb = w.nb
+    nm = nb * ns
+    Vii = np.zeros((nm, nm), np.complex128)
+    V1i = np.zeros((nm, nm), np.complex128)
+    for i in range(1, ns + 1):
+        for k in range(1, ns + 1):
+            bi, bk = slice((i - 1) * nb, i * nb), slice((k - 1) * nb, k * nb)
+            for j in range(w.ymin, w.ymax + 1):
+                if w.xmin <= i - k <= w.xmax:
+                    Vii[bi, bk] += bare_coulomb(w, i - k, j, eps, r0, ldiag)
+                if w.xmin <= i - k + ns <= w.xmax:
+                    V1i[bi, bk] += bare_coulomb(w, i - k, j, eps, r0, ldiag)
+    return Vii, V1i
+
+
+def device_coulomb(w, ns, length, eps, r0, ldiag):
+    """Vii, V1i (Fortran (nm, nm, length)) of the RGF driver: the slab blocks copied to
+    every slab (src/main.f90:531-537)."""
+    v00, v10 = coulomb_blocks(w, ns, eps, r0, ldiag)
+    nm = v00.shape[0]
+    Vii = np.empty((nm, nm, length), np.complex128, order="F")
+    V1i = np.empty((nm, nm, length), np.complex128, order="F")
+    Vii[:, :, :] = v00[:, :, None]
+    V1i[:, :, :] = v10[:, :, None]
+    return Vii, V1i

@@ -85,3 +85,44 @@ def test_w_symmetric_inputs_give_the_dense_blocks():
         for i in range(nx):
             r = D[i * nm:(i + 1) * nm, i * nm:(i + 1) * nm]
             assert np.linalg.norm(WR[:, :, i, e] - r) <= TOL * np.linalg.norm(r)
+
+
+@pytest.mark.parametrize("nm,nx,nE", [(8, 4, 3), (20, 3, 5), (64, 6, 2), (128, 4, 2)])
+def test_w_full_matches_dense_oracle(nm, nx, nE):
+    """green_calc_w_full on the GPU (gw_w_full: dense LU solve per energy) against the
+    oracle's dense numpy solve, Hermitian V and random P, 1e-10 per block."""
+    import torch
+    from liteqt_amd import gw
+    _rgf()
+    print(f"W full nm={nm} nx={nx} nE={nE}", flush=True)
+    rng = np.random.default_rng(3 * nm + nx)
+    Vii, V1i, PL, PG, PR = _inputs(rng, nm, nx, nE, False)
+    dev = torch.device("cuda:0")
+    from liteqt_amd import rgf
+    tb = lambda a: rgf.to_device_blocks(a, dev)
+    WL, WG, WR = gw.w_full(tb(Vii), tb(V1i), tb(PL), tb(PG), tb(PR))
+    torch.cuda.synchronize()
+    got = [rgf.from_device_blocks(x) for x in (WL, WG, WR)]
+    for e in range(nE):
+        ref = wo.w_full(Vii, V1i, PL[..., e], PG[..., e], PR[..., e])
+        for name, g, r in zip(("WL", "WG", "WR"), got, ref):
+            for i in range(nx):
+                err = np.linalg.norm(g[:, :, i, e] - r[:, :, i])
+                assert err <= TOL * np.linalg.norm(r[:, :, i]), (name, e, i, err / np.linalg.norm(r[:, :, i]))
+
+
+def test_w_full_symmetric_inputs_equal_the_block_recursion():
+    """For symmetric V and P^r the dense form (green_calc_w_full) and the block recursion
+    (green_rgf_calc_w) give the same W^r diagonal blocks."""
+    import torch
+    from liteqt_amd import gw
+    rgf = _rgf()
+    rng = np.random.default_rng(8)
+    nm, nx, nE = 32, 5, 3
+    Vii, V1i, PL, PG, PR = _inputs(rng, nm, nx, nE, True)
+    dev = torch.device("cuda:0")
+    tb = lambda a: rgf.to_device_blocks(a, dev)
+    _, _, WRf = gw.w_full(tb(Vii), tb(V1i), tb(PL), tb(PG), tb(PR))
+    WRr = rgf.w_solve_batched(tb(Vii), tb(V1i), tb(PL), tb(PG), tb(PR))["WR"]
+    torch.cuda.synchronize()
+    assert torch.linalg.vector_norm(WRf - WRr).item() <= TOL * torch.linalg.vector_norm(WRf).item()

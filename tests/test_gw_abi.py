@@ -23,7 +23,8 @@ def test_library_exports_every_declared_symbol():
         pytest.fail("libgw_hip.so not built (run __graft_entry__.build())")
     names = header_functions()
     assert names == ["gw_last_error", "gw_pairs_polarization", "gw_pairs_selfenergy", "gw_params_reference",
-                     "gw_polarization", "gw_polarization_fft", "gw_release", "gw_selfenergy", "gw_selfenergy_fft"]
+                     "gw_polarization", "gw_polarization_fft", "gw_release", "gw_selfenergy", "gw_selfenergy_fft",
+                     "gw_w_full"]
     L = ctypes.CDLL(gw.LIB_PATH)
     for name in names:
         assert hasattr(L, name), name

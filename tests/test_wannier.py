@@ -112,3 +112,30 @@ def test_oracle_transmission_of_the_pristine_tube_counts_modes():
         assert M == expect
         o = orc.rgf(po, (300.0, 300.0), E, (100.0, -100.0), Hii, H1i, offdiag=False)
         assert abs(o["tr"] - M) < 1e-6 and abs(o["tre"] - M) < 1e-6, (E, o["tr"], M)
+
+
+def test_coulomb_blocks_of_the_example():
+    """w90_bare_coulomb_blocks on examples/cnt_8-0 (ns = 4, epsilon = 1, r0 = 3 A, only
+    diag): real symmetric blocks; the self-interaction e/(4 pi eps0 r0) and the distant
+    pairs' V |r| / tanh(|r|/r0) equal the Coulomb constant (14.40 eV A) to the reference's
+    rounded constants (e = 1.6e-19 C, eps0 = 8.85e-12: 14.387); V scales as 1/epsilon;
+    the 0.1 eV self-term without ldiag; and V1i takes bare_coulomb(i-k) as the reference
+    writes it (so here, with every offset of a 4-cell slab in range, V1i = Vii)."""
+    w = wannier.load_ham_dat(os.path.join(EX, "ham_dat"))
+    Vii, V1i = wannier.coulomb_blocks(w, 4, 1.0, 3.0, True)
+    assert Vii.shape == (128, 128) and np.all(Vii.imag == 0)
+    assert np.allclose(Vii, Vii.T, rtol=0, atol=1e-13)
+    k = 14.3996  # e^2 / (4 pi eps0) in eV A
+    assert abs(Vii[0, 0].real * 3.0 - k) / k < 1e-3
+    alpha = w.cell[:, 0]
+    bc = wannier.bare_coulomb(w, 4, 0, 1.0, 3.0, True)
+    r = 4 * alpha[:, None, None] + w.centers[:, :, None] - w.centers[:, None, :]
+    nr = np.sqrt((r * r).sum(0))
+    assert np.all(np.abs(bc * nr / np.tanh(nr / 3.0) - k) / k < 1e-3)
+    V2, _ = wannier.coulomb_blocks(w, 4, 2.0, 3.0, True)
+    assert np.allclose(V2, Vii / 2.0, rtol=1e-15, atol=0)
+    Vn, _ = wannier.coulomb_blocks(w, 4, 1.0, 3.0, False)
+    assert Vn[0, 0] == 0.1 and np.array_equal(Vn[0, 1], Vii[0, 1])
+    assert np.array_equal(V1i, Vii)
+    Vd, V1d = wannier.device_coulomb(w, 4, 18, 1.0, 3.0, True)
+    assert Vd.shape == (128, 128, 18) and np.array_equal(Vd[:, :, 17], Vii) and np.array_equal(V1d[:, :, 0], V1i)
