@@ -371,41 +371,58 @@ def random_device_torch(torch, nm, nx, gen, dev, u_range=0.3, coupling_sigma=0.5
     return H.contiguous(), C.contiguous()
 
 
+class HostInfo:
+    """info arrays (nE, 4) int32 in coherent pinned host memory (hipHostMalloc with the
+    coherent and mapped flags), wrapped as CPU tensors.  The solve's kernels write each
+    energy's counters there when the energy is done; the host reads them without a HIP
+    call.  A copy from device memory would not do: a device-to-host copy runs as a
+    kernel, which cannot start while the persistent launch holds every CU's register
+    file."""
+
+    def __init__(self, torch):
+        import ctypes
+        self.torch = torch
+        self.hip = ctypes.CDLL("libamdhip64.so")
+        self.hip.hipHostMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t, ctypes.c_uint]
+        self.hip.hipHostFree.argtypes = [ctypes.c_void_p]
+        self.ptrs = []
+
+    def new(self, nE):
+        import ctypes
+        p = ctypes.c_void_p()
+        nbytes = max(1, nE) * 16
+        if self.hip.hipHostMalloc(ctypes.byref(p), nbytes, 0x40000000 | 0x2) != 0:  # coherent | mapped
+            raise RuntimeError("hipHostMalloc failed")
+        self.ptrs.append(p.value)
+        arr = np.ctypeslib.as_array((ctypes.c_int32 * (4 * max(1, nE))).from_address(p.value))
+        t = self.torch.from_numpy(arr).view(max(1, nE), 4)[:nE]
+        t.zero_()
+        return t
+
+    def close(self):
+        import ctypes
+        for p in self.ptrs:
+            self.hip.hipHostFree(ctypes.c_void_p(p))
+        self.ptrs = []
+
+
 class _Progress:
     """Long launches (the 64 devices at their full energy counts take minutes): a
     thread counts the energies whose decimation count is already written (info[:, 0]
-    >= 1, zero before the solve) and prints the count every `every` seconds.  It reads
-    the info arrays with asynchronous copies into pinned host memory on a stream of its
-    own created non-blocking (it does not wait for the launch on the default stream)."""
+    >= 1, zero before the solve) and prints the count every `every` seconds.  The info
+    arrays are HostInfo tensors, which the kernels write as they go."""
 
     def __init__(self, torch, infos, label, every=45.0):
-        import ctypes
         import threading
         self.torch, self.infos, self.label, self.every = torch, infos, label, every
-        self.hip = ctypes.CDLL("libamdhip64.so")
-        self.hip.hipStreamCreateWithFlags.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint]
-        self.hip.hipMemcpyAsync.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int,
-                                            ctypes.c_void_p]
-        self.hip.hipStreamSynchronize.argtypes = [ctypes.c_void_p]
-        self.hip.hipStreamDestroy.argtypes = [ctypes.c_void_p]
-        self.host = [torch.empty(i.shape, dtype=i.dtype, pin_memory=True) for i in infos]
         self.stop = threading.Event()
         self.t = threading.Thread(target=self._run, daemon=True)
 
     def _run(self):
-        import ctypes
-        st = ctypes.c_void_p()
-        if self.hip.hipStreamCreateWithFlags(ctypes.byref(st), 1) != 0:  # hipStreamNonBlocking
-            return
         total = sum(int(i.shape[0]) for i in self.infos)
         while not self.stop.wait(self.every):
-            for d, h in zip(self.infos, self.host):  # hipMemcpyDeviceToHost = 2
-                self.hip.hipMemcpyAsync(ctypes.c_void_p(h.data_ptr()), ctypes.c_void_p(d.data_ptr()),
-                                        d.numel() * d.element_size(), 2, st)
-            self.hip.hipStreamSynchronize(st)
-            done = sum(int((h[:, 0] > 0).sum()) for h in self.host)
+            done = sum(int((h[:, 0] > 0).sum()) for h in self.infos)
             print(f"bench: {self.label}: {done} of {total} energies solved", file=sys.stderr, flush=True)
-        self.hip.hipStreamDestroy(st)
 
     def __enter__(self):
         for i in self.infos:
@@ -435,6 +452,7 @@ def devices_cost(rgf, torch, dev, ndev, e_div, compare, graph=True, seed=3):
     T = kT / 8.61734e-05
     temp, mu = (T, T), (0.1, -0.1)
     p = rgf.params(rgf.MODE_COMPAT, eta_dev=1e-5, eta_lead=1e-5, dec_tol=1e-12, dec_nmax=100)
+    hinfo = HostInfo(torch)
     devs, shapes = [], []
     for j in range(ndev):
         nm = int(rng.choice([64, 96, 128, 192, 256]))
@@ -444,7 +462,7 @@ def devices_cost(rgf, torch, dev, ndev, e_div, compare, graph=True, seed=3):
         E = torch.tensor(np.linspace(-2.0, 2.0, nE, endpoint=False) + 4.0 / nE, dtype=torch.float64, device=dev)
         out = {"tr": torch.zeros(nE, dtype=torch.float64, device=dev), "tre": torch.zeros(nE, dtype=torch.float64, device=dev),
                "dens_n": torch.zeros(nE, nx, dtype=torch.float64, device=dev),
-               "info": torch.zeros(nE, 4, dtype=torch.int32, device=dev)}
+               "info": hinfo.new(nE)}
         devs.append({"E": E, "temp": temp, "mu": mu, "Hii": Hd, "H1i": Cd, "out": out})
         shapes.append((nm, nx, nE))
     items = sum(s[2] for s in shapes)
@@ -501,7 +519,9 @@ def devices_cost(rgf, torch, dev, ndev, e_div, compare, graph=True, seed=3):
         print(f"bench: {ndev} devices one launch each {t_seq:.2f}s", file=sys.stderr, flush=True)
         same = bool(all(torch.equal(a, d["out"]["tr"]) for a, d in zip(tr_one, devs)))
     rgf.lib().rgf_release()
-    del devs
+    del devs, infos
+    torch.cuda.synchronize()
+    hinfo.close()
     torch.cuda.empty_cache()
     achieved = flops / t_one / 1e12
     sizes = sorted(set(s[0] for s in shapes))

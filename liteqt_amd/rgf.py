@@ -241,16 +241,17 @@ def _tp(t):
     return None if t is None else t.data_ptr()
 
 
-def _req(t, name, dtype, numel, device):
+def _req(t, name, dtype, numel, device, host_ok=False):
     """Validate a tensor handed to the C ABI as a raw pointer: right dtype,
-    contiguous, on `device`, with at least `numel` elements.  None passes."""
+    contiguous, on `device` (or, with host_ok, in pinned host memory, which the
+    device addresses directly), with at least `numel` elements.  None passes."""
     if t is None:
         return None
     if t.dtype != dtype:
         raise RgfError(f"{name}: expected {dtype}, got {t.dtype}")
     if not t.is_contiguous():
         raise RgfError(f"{name}: tensor must be contiguous")
-    if t.device != device:
+    if t.device != device and not (host_ok and t.device.type == "cpu" and t.is_pinned()):
         raise RgfError(f"{name}: tensor on {t.device}, expected {device}")
     if t.numel() < numel:
         raise RgfError(f"{name}: {t.numel()} elements, need {numel}")
@@ -319,7 +320,10 @@ def _out_batch(torch, b, out, dev):
              "x": (torch.float64, nE * nx), "i": (torch.int32, 4 * nE)}
     for k, kind in _OUT_KINDS.items():
         dt, cnt = sizes[kind]
-        setattr(b, k, _req(out.get(k), k, dt, cnt, dev))
+        # info may sit in pinned host memory: the kernels write each energy's four
+        # counters there as it finishes, so the host can watch a long launch without
+        # queueing work on the device
+        setattr(b, k, _req(out.get(k), k, dt, cnt, dev, host_ok=(kind == "i")))
 
 
 class Graph:

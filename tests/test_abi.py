@@ -56,6 +56,13 @@ def test_tensor_checks_reject_wrong_dtype_layout_and_size():
                      (torch.zeros(3, dtype=torch.float64), "elements")):
         with pytest.raises(rgf.RgfError, match=why):
             rgf._req(bad, "w", torch.float64, 4, cpu)
+    # a host tensor for a device output: refused unless the output may live in pinned
+    # host memory and the tensor is pinned (an ordinary host tensor is not)
+    gpu = torch.device("cuda:0")
+    with pytest.raises(rgf.RgfError, match="expected cuda"):
+        rgf._req(ok, "w", torch.float64, 4, gpu)
+    with pytest.raises(rgf.RgfError, match="expected cuda"):
+        rgf._req(torch.zeros(8, dtype=torch.int32), "info", torch.int32, 8, gpu, host_ok=True)
 
 
 def test_offdiag_and_scba_argument_errors_before_any_device_call():

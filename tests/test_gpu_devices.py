@@ -139,3 +139,34 @@ def test_devices_graph_replays_match_eager():
             torch.cuda.synchronize()
     finally:
         g.close()
+
+
+def test_devices_info_in_pinned_host_memory():
+    """info may be a pinned host tensor (the kernels write it through the device's
+    mapping of host memory): the counters equal those of a device-resident info, and
+    the other outputs are unchanged by where info lives."""
+    torch, rgf = _env()
+    dev = torch.device("cuda:0")
+    host, devs = _devices(torch, rgf, dev, scatter_every=2)
+    p = rgf.params(rgf.MODE_COMPAT, **KW)
+    rgf.solve_devices(p, devs, autotune=False)
+    torch.cuda.synchronize()
+    want = [{k: v.clone() for k, v in d["out"].items()} for d in devs]
+    for d in devs:
+        for v in d["out"].values():
+            v.zero_()
+        d["out"]["info"] = torch.zeros(d["out"]["info"].shape, dtype=torch.int32, pin_memory=True)
+    torch.cuda.synchronize()
+    rgf.solve_devices(p, devs, autotune=False)
+    torch.cuda.synchronize()
+    for j, (d, w) in enumerate(zip(devs, want)):
+        assert torch.equal(d["out"]["info"], w["info"].cpu()), j
+        for k in w:
+            if k != "info":
+                assert torch.equal(d["out"][k], w[k]), (j, k)
+    # the batched entry point takes it as well
+    d, w = devs[3], want[3]
+    o = {k: v.clone().zero_() for k, v in d["out"].items()}
+    rgf.solve_batched(p, d["E"], d["temp"], d["mu"], d["Hii"], d["H1i"], sigL=d.get("sigL"), sigR=d.get("sigR"), out=o)
+    torch.cuda.synchronize()
+    assert torch.equal(o["info"], w["info"].cpu())
