@@ -251,7 +251,7 @@ def _req(t, name, dtype, numel, device, host_ok=False):
         raise RgfError(f"{name}: expected {dtype}, got {t.dtype}")
     if not t.is_contiguous():
         raise RgfError(f"{name}: tensor must be contiguous")
-    if t.device != device and not (host_ok and t.device.type == "cpu" and t.is_pinned()):
+    if t.device != device and not (host_ok and t.device.type == "cpu" and (t.numel() == 0 or t.is_pinned())):
         raise RgfError(f"{name}: tensor on {t.device}, expected {device}")
     if t.numel() < numel:
         raise RgfError(f"{name}: {t.numel()} elements, need {numel}")

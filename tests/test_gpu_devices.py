@@ -166,7 +166,8 @@ def test_devices_info_in_pinned_host_memory():
                 assert torch.equal(d["out"][k], w[k]), (j, k)
     # the batched entry point takes it as well
     d, w = devs[3], want[3]
-    o = {k: v.clone().zero_() for k, v in d["out"].items()}
+    o = {k: v.clone().zero_() for k, v in d["out"].items() if k != "info"}
+    o["info"] = torch.zeros(d["out"]["info"].shape, dtype=torch.int32, pin_memory=True)
     rgf.solve_batched(p, d["E"], d["temp"], d["mu"], d["Hii"], d["H1i"], sigL=d.get("sigL"), sigR=d.get("sigR"), out=o)
     torch.cuda.synchronize()
     assert torch.equal(o["info"], w["info"].cpu())
