@@ -436,7 +436,7 @@ class _Progress:
         self.t.join()
 
 
-def devices_cost(rgf, torch, dev, ndev, e_div, compare, graph=True, seed=3):
+def devices_cost(rgf, torch, dev, ndev, e_div, compare, graph=True, graph_warmup=True, seed=3):
     """64 uneven devices in one persistent launch: nm in {64, 96, 128, 192, 256},
     nx ~ U[50, 400], energy count ~ U[256, 2048] divided by e_div (energies in
     [-2, 2] eV), eta 1e-5, Fermi leads, compat mode, tr/tre/dens per energy.
@@ -490,10 +490,11 @@ def devices_cost(rgf, torch, dev, ndev, e_div, compare, graph=True, seed=3):
         gs = torch.cuda.Stream()
         torch.cuda.synchronize()
         rgf.lib().rgf_release()  # the graph holds its own workspace
-        g = rgf.DevicesGraph(p, devs, gs, autotune=True)
-        with _Progress(torch, infos, "devices graph warm-up replay"):
-            g.launch(1)
-            gs.synchronize()
+        g = rgf.DevicesGraph(p, devs, gs, autotune=True)  # instantiated here, outside the timed replay
+        if graph_warmup:
+            with _Progress(torch, infos, "devices graph warm-up replay"):
+                g.launch(1)
+                gs.synchronize()
         with _Progress(torch, infos, "devices graph replay"):
             t0 = time.perf_counter()
             g.launch(1)
@@ -634,6 +635,8 @@ def main():
     ap.add_argument("--devices", type=int, default=64)
     ap.add_argument("--no-devices-graph", action="store_true",
                     help="skip the graph-captured replay of the 64-device launch")
+    ap.add_argument("--no-devices-graph-warmup", action="store_true",
+                    help="time the first replay of the devices graph (no untimed replay before it)")
     ap.add_argument("--devices-compare", action="store_true",
                     help="also time the devices one rgf_solve_batched launch after another")
     ap.add_argument("--devices-energy-div", type=int, default=32,
@@ -751,7 +754,7 @@ def main():
     devices = None
     if not args.no_devices:
         devices = devices_cost(rgf, torch, dev, args.devices, args.devices_energy_div, args.devices_compare,
-                               graph=not args.no_devices_graph)
+                               graph=not args.no_devices_graph, graph_warmup=not args.no_devices_graph_warmup)
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tfile):
