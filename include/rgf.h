@@ -257,21 +257,25 @@ int rgf_graph_create_devices(const rgf_params *p, const rgf_batch *devs, int nde
  * flight at block size nm and nx slabs. */
 long long rgf_workspace_bytes(int nm, int nx, int slots);
 
-/* Adjoint (reverse-mode) RGF of the physical-mode device with the contact
- * self-energies and the scattering self-energy held fixed (diagonal sigL/sigR
- * and, when given, the off-diagonal sigR_lo/sigR_up/sigL_up of b).  Per energy e the objective is
+/* Adjoint (reverse-mode) RGF of the physical-mode device with the scattering
+ * self-energy held fixed (diagonal sigL/sigR and, when given, the off-diagonal
+ * sigR_lo/sigR_up/sigL_up of b).  Per energy e the objective is
  *   J_e = w_tr[e] * tr_e + sum_l w_n[e*nx + l] * Im Tr G^<_ll
  * (tr: current spectrum into the right contact, src/green_rgf.f90:4058-4066;
  * Im Tr G^<_ll: the density integrand of src/green_rgf.f90:3919-3945); w_tr
  * or w_n may be NULL (zero).  Outputs (device arrays): J[e]; dU[e*nx + l] =
  * dJ_e/dU_l for a potential U_l added to the diagonal of slab l; dH1i (nx
- * blocks, may be NULL) = sum_e (dJ_e/dRe H1i_l + i dJ_e/dIm H1i_l) for the
- * device-internal couplings l = 2..nx-1 (1-based), zero for l = 1 and nx,
- * whose blocks also define the leads.  lead_H00 (2 blocks, may be NULL) gives
- * the onsite blocks of the left and right lead; NULL takes them from the edge
- * slabs, Hii + sigR at l = 1 and nx, as the reference does
- * (src/green_rgf.f90:3995, 4035), so that a potential on an edge slab then
- * moves only the device slab, not the lead.  b supplies E, Hii, H1i, sigL,
+ * blocks, may be NULL) = sum_e (dJ_e/dRe H1i_l + i dJ_e/dIm H1i_l).
+ * Contacts: lead_H00 NULL builds the leads from the edge slabs, Hii + sigR at
+ * l = 1 and nx with the couplings H1i_1 and H1i_nx, as the reference does
+ * (src/green_rgf.f90:3995, 4035), and differentiates through them: a potential
+ * on an edge slab moves its lead too, and dH1i_1, dH1i_nx include the contact
+ * couplings' role in the decimation (the surface function's fixed point,
+ * differentiated implicitly: a Stein equation summed by doubling; info bit 2 /
+ * bit 3 of info[4e+2] flag a left / right sum that did not converge within
+ * dec_nmax doublings).  lead_H00 given (2 blocks: left and right lead onsite)
+ * holds the contacts fixed at those blocks: dH1i is then zero for l = 1 and nx
+ * and dU_1, dU_nx move only the device slabs.  b supplies E, Hii, H1i, sigL,
  * sigR, sig_stride, temp, mu, nE, nm (multiple of 32, <= 256), nx, max_slots
  * and optionally tr and info.  Asynchronous on `stream`.  No counterpart in
  * the reference (which has no derivatives); it serves optimisation and

@@ -6,9 +6,13 @@ for the physical-mode device whose slab l carries an extra potential U[l] on
 its diagonal and whose device-internal couplings H1i[..., l] (l = 1..nx-2,
 0-based) are replaced by T[l-1]; sig_off optionally holds the first
 off-diagonal scattering blocks (sigR_lo, sigR_up, sigL_up), held fixed like
-the diagonal ones.  The leads stay those of the unperturbed
+the diagonal ones.  By default the leads stay those of the unperturbed
 device: their onsite blocks are the edge slabs of Hii (plus sigR there), so a
-potential on an edge slab moves the device slab only.  Forward and gradients
+potential on an edge slab moves the device slab only.  With follow_leads the
+leads repeat the perturbed edge slabs, as in the reference's device: T then
+replaces every coupling (T[l] = H1i[..., l], l = 0..nx-1, the first and last
+being the contact couplings) and the gradients run through the decimation.
+Forward and gradients
 come from one call of rgf_adjoint_batched; the backward only scales the stored
 gradients.  Used as a test harness (gradcheck against finite differences).
 """
@@ -19,27 +23,33 @@ from liteqt_amd import rgf
 
 class RGFObjective(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, U, Tint, p, E, temp, mu, Hii, H1i, sigL, sigR, w_tr, w_n, sig_off=None):
+    def forward(ctx, U, Tint, p, E, temp, mu, Hii, H1i, sigL, sigR, w_tr, w_n, sig_off=None, follow_leads=False):
         # Hii, H1i: device tensors (nx, nm, nm) holding column-major blocks (to_device_blocks)
         nx, nm = Hii.shape[0], Hii.shape[-1]
-        if sigR is not None and sigR.dim() == 4:
-            raise NotImplementedError("RGFObjective: fixed leads need an energy-independent sigR")
-        lead = torch.stack([Hii[0], Hii[nx - 1]])
-        if sigR is not None:
-            lead = lead + torch.stack([sigR[0], sigR[nx - 1]])
+        lead = None
+        if not follow_leads:
+            if sigR is not None and sigR.dim() == 4:
+                raise NotImplementedError("RGFObjective: fixed leads need an energy-independent sigR")
+            lead = torch.stack([Hii[0], Hii[nx - 1]])
+            if sigR is not None:
+                lead = lead + torch.stack([sigR[0], sigR[nx - 1]])
+            lead = lead.contiguous()
         Hm = Hii.clone()
         idx = torch.arange(nm, device=Hii.device)
         Hm[:, idx, idx] += U.to(Hm.dtype)[:, None]
         Cm = H1i.clone()
-        # Tint[k] is the column-major block of H1i at slab k+1 (same memory layout)
-        Cm[1:nx - 1] = Tint
+        # Tint[k] is the column-major block of H1i at slab k+1 (all slabs: slab k) in the same layout
+        if follow_leads:
+            Cm[:] = Tint
+        else:
+            Cm[1:nx - 1] = Tint
         J, dU, dH = rgf.adjoint_batched(p, E, temp, mu, Hm, Cm, sigL=sigL, sigR=sigR, w_tr=w_tr, w_n=w_n,
-                                        lead_H00=lead.contiguous(), **(sig_off or {}))
-        ctx.save_for_backward(dU.sum(0), dH[1:nx - 1].clone())
+                                        lead_H00=lead, **(sig_off or {}))
+        ctx.save_for_backward(dU.sum(0), dH.clone() if follow_leads else dH[1:nx - 1].clone())
         return J.sum()
 
     @staticmethod
     def backward(ctx, gJ):
         dU, dT = ctx.saved_tensors
         # dT holds dJ/dRe + i dJ/dIm of column-major blocks, the layout of Tint
-        return gJ * dU, gJ * dT, None, None, None, None, None, None, None, None, None, None, None
+        return gJ * dU, gJ * dT, None, None, None, None, None, None, None, None, None, None, None, None

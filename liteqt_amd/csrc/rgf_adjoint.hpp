@@ -79,6 +79,86 @@ __host__ __device__ inline long long adj_slot_blocks(int nx, int keep, int ckpt 
 // With keep, Q_l and (Q G<)_l sit in the slab-l blocks of glb and gb, which pass A
 // writes only after their last use in its step l.
 // Coupling adjoints d-bar + u-bar^H accumulate per slab in dH.
+//
+// Contacts that repeat the edge slabs (a.lead null, the reference's device): after pass
+// B the adjoints of the contact self-energies, Sigma_L-bar = -Abar_0 + f_L (Sbar_0^H -
+// Sbar_0) and Sigma_R-bar = -Abar_N + f_R (Sbar_N^H - Sbar_N) + w_tr ((G<_N)^H - G<_N)
+// + 2 w_tr f_R (G_N^H - G_N) (the last two from tr's own K_R), are carried through
+// Sigma_L = C g C^H (C = H1i_1) and Sigma_R = C^H g C (C = H1i_nx) and through the
+// decimation's fixed point g = (z - H00 - Sigma)^-1 (lead_adjoint): dU_1, dU_nx gain
+// Re Tr H00-bar, dH_1 and dH_nx the contact couplings' adjoints.
+
+// Sum of |P_ij| below which the doubling of lead_adjoint stops: P is dimensionless
+// (a coupling times a surface function), and the next term is then below 1e-16 of X.
+constexpr double kSteinTol = 1e-8;
+
+// Adjoint of a contact given Sigma-bar Sb (left: Sigma = C g C^H with g = (z - H00 -
+// C g C^H)^-1; right: Sigma = C^H g C with g = (z - H00 - C^H g C)^-1).  C-bar is added
+// to Cbar; returns Re Tr H00-bar (uniform).  The total g-bar X solves the Stein
+// equation X = g-bar + P X P2 (left: P = C^H g^H, P2 = g^H C; right: P = C g^H,
+// P2 = g^H C^H), summed by doubling (X += P X P2, P <- P^2, P2 <- P2^2); steps is the
+// number of doublings, or -1 when kSteinTol is not reached within nmax.
+// t: 8 temporary blocks.
+template <int T>
+__device__ double lead_adjoint(int n, bool left, const double2* C, const double2* g, const double2* Sb,
+                               double2* Cbar, double2* t, int nmax, int& steps, ldsd* lds, ldsd* red) {
+  const size_t nn = (size_t)n * n;
+  double2 *X = t, *P = t + nn, *P2 = t + 2 * nn, *Pn = t + 3 * nn, *P2n = t + 4 * nn, *Tt = t + 5 * nn;
+  double2 *U = t + 6 * nn, *Hb = t + 7 * nn;
+  const Epi st = epi_store();
+  Epi ac = epi_store();
+  ac.D1 = Cbar;
+  if (left) {
+    wg_gemm<T>(n, OP_N, Sb, OP_N, C, U, 0, st, lds);      // Sb C
+    wg_gemm<T>(n, OP_C, C, OP_N, U, X, 0, st, lds);       // g-bar = C^H Sb C
+    wg_gemm<T>(n, OP_N, U, OP_C, g, Cbar, 0, ac, lds);    // C-bar += Sb C g^H
+    wg_gemm<T>(n, OP_C, Sb, OP_N, C, U, 0, st, lds);
+    wg_gemm<T>(n, OP_N, U, OP_N, g, Cbar, 0, ac, lds);    // C-bar += Sb^H C g
+    wg_gemm<T>(n, OP_C, C, OP_C, g, P, 0, st, lds);       // P = C^H g^H
+    wg_gemm<T>(n, OP_C, g, OP_N, C, P2, 0, st, lds);      // P2 = g^H C
+  } else {
+    wg_gemm<T>(n, OP_N, C, OP_N, Sb, U, 0, st, lds);      // C Sb
+    wg_gemm<T>(n, OP_N, U, OP_C, C, X, 0, st, lds);       // g-bar = C Sb C^H
+    wg_gemm<T>(n, OP_C, g, OP_N, C, U, 0, st, lds);
+    wg_gemm<T>(n, OP_N, U, OP_N, Sb, Cbar, 0, ac, lds);   // C-bar += g^H C Sb
+    wg_gemm<T>(n, OP_N, g, OP_N, C, U, 0, st, lds);
+    wg_gemm<T>(n, OP_N, U, OP_C, Sb, Cbar, 0, ac, lds);   // C-bar += g C Sb^H
+    wg_gemm<T>(n, OP_N, C, OP_C, g, P, 0, st, lds);       // P = C g^H
+    wg_gemm<T>(n, OP_C, g, OP_C, C, P2, 0, st, lds);      // P2 = g^H C^H
+  }
+  steps = -1;
+  Epi ax = epi_store();
+  ax.D1 = X;
+  Epi as = epi_store();
+  as.abs_sum = 1;
+  for (int k = 1; k <= nmax; ++k) {
+    wg_gemm<T>(n, OP_N, P, OP_N, X, Tt, 0, st, lds);
+    wg_gemm<T>(n, OP_N, Tt, OP_N, P2, X, 0, ax, lds);     // X += P X P2
+    const double sp = wg_gemm<T>(n, OP_N, P, OP_N, P, Pn, 0, as, lds);
+    wg_gemm<T>(n, OP_N, P2, OP_N, P2, P2n, 0, st, lds);
+    double2* q = P; P = Pn; Pn = q;
+    q = P2; P2 = P2n; P2n = q;
+    if (sp <= kSteinTol) { steps = k; break; }
+  }
+  wg_gemm<T>(n, OP_C, g, OP_N, X, U, 0, st, lds);
+  wg_gemm<T>(n, OP_N, U, OP_C, g, Hb, 0, st, lds);        // H00-bar = g^H X g^H
+  double trh = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) trh += gc(Hb)[i + (size_t)i * n].x;
+  trh = block_sum(trh, red);
+  if (left) {
+    wg_gemm<T>(n, OP_N, Hb, OP_N, C, U, 0, st, lds);
+    wg_gemm<T>(n, OP_N, U, OP_C, g, Cbar, 0, ac, lds);    // C-bar += H00b C g^H
+    wg_gemm<T>(n, OP_C, Hb, OP_N, C, U, 0, st, lds);
+    wg_gemm<T>(n, OP_N, U, OP_N, g, Cbar, 0, ac, lds);    // C-bar += H00b^H C g
+  } else {
+    wg_gemm<T>(n, OP_C, g, OP_N, C, U, 0, st, lds);
+    wg_gemm<T>(n, OP_N, U, OP_N, Hb, Cbar, 0, ac, lds);   // C-bar += g^H C H00b
+    wg_gemm<T>(n, OP_N, g, OP_N, C, U, 0, st, lds);
+    wg_gemm<T>(n, OP_N, U, OP_C, Hb, Cbar, 0, ac, lds);   // C-bar += g C H00b^H
+  }
+  return trh;
+}
+
 template <int T, int NMAX>
 __device__ void adjoint_energy(const AdjArgs& a, int e, double2* ws, double2* dH, ldsd* lds, ldsd* red) {
   const int n = a.n, N = a.nx;
@@ -108,6 +188,7 @@ __device__ void adjoint_energy(const AdjArgs& a, int e, double2* ws, double2* dH
   double2 *gs = Tm(14), *Wb = Tm(15), *gsb = Tm(16), *Rb = Tm(17), *Rdb = Tm(18), *QGnb = Tm(19), *Qb = Tm(20);
   double2 *Gb = Tm(21), *Lb = Tm(22), *Gb2 = Tm(23), *Lb2 = Tm(24), *Mb = Tm(26), *Sb = Tm(27);  // 25: g S when not kept
   double2 *Ab = Tm(28), *Bb = Tm(29), *Yb = Tm(30), *Bm = Tm(31), *Y = Tm(32), *K = Tm(33), *Gs = Tm(34);
+  double2 *GsL = Tm(35), *SRb = Tm(0);  // left surface function; Sigma_R-bar (after the contacts)
   double2* stmp = Tm(36);  // 11 blocks for the decimation
   const size_t so = (size_t)e * a.sig_stride;
   auto Hb = [&](int l) { return a.Hii + (size_t)l * nn; };
@@ -166,9 +247,9 @@ __device__ void adjoint_energy(const AdjArgs& a, int e, double2* ws, double2* dH
     wg_ew(n, H00, 0, zero, a.lead, one, nullptr, one, nullptr, one);
   else
     wg_ew(n, H00, 0, zero, Hb(0), one, sR(0), one, nullptr, one);
-  const int itL = wg_sancho<T, NMAX>(n, E, a.eta_lead, a.dec_tol, a.dec_nmax, H00, Cb(0), 1, Gs, stmp, lds, red,
+  const int itL = wg_sancho<T, NMAX>(n, E, a.eta_lead, a.dec_tol, a.dec_nmax, H00, Cb(0), 1, GsL, stmp, lds, red,
                                      hitL, fails);
-  wg_gemm<T>(n, OP_N, Cb(0), OP_N, Gs, T0, 0, st, lds);
+  wg_gemm<T>(n, OP_N, Cb(0), OP_N, GsL, T0, 0, st, lds);
   wg_gemm<T>(n, OP_N, T0, OP_C, Cb(0), SL, 0, st, lds);
   if (a.lead)
     wg_ew(n, H00, 0, zero, a.lead + nn, one, nullptr, one, nullptr, one);
@@ -348,6 +429,13 @@ __device__ void adjoint_energy(const AdjArgs& a, int e, double2* ws, double2* dH
       tr_ab = block_sum(tr_ab, red);
       if (threadIdx.x == 0) a.dU[(size_t)e * N + l] = -tr_ab;
     }
+    if (!a.lead && l == N - 1) {
+      // Sigma_R-bar = -Abar + fR (Sbar^H - Sbar) + w_tr (G<^H - G<) + 2 w_tr fR (G^H - G)
+      wg_ew(n, SRb, 0, zero, Ab, mone, Sb, make_double2(-fR, 0), Sb, make_double2(fR, 0));
+      wg_ew(n, SRb, 0, zero, SRb, one, gl(N - 1), make_double2(-wtr, 0), gl(N - 1), make_double2(wtr, 0));
+      wg_ew(n, SRb, 0, zero, SRb, one, g(N - 1), make_double2(-2.0 * wtr * fR, 0), g(N - 1),
+            make_double2(2.0 * wtr * fR, 0));
+    }
     if (l >= 1) {
       ensure(l - 1);  // may replace slab l's segment: slab l's own blocks are no longer read
       const double2* s = sLup(l);
@@ -371,12 +459,27 @@ __device__ void adjoint_energy(const AdjArgs& a, int e, double2* ws, double2* dH
       wg_gemm<T>(n, OP_C, dP, OP_N, Bb, B_(gbv, l - 1), 0, acc(B_(gbv, l - 1)), lds);
     }
   }
+  int lead_flags = 0;
+  if (!a.lead) {
+    // contacts that repeat the edge slabs: Ab, Sb hold slab 0's adjoints
+    double2* SLb = Tm(1);
+    wg_ew(n, SLb, 0, zero, Ab, mone, Sb, make_double2(-fL, 0), Sb, make_double2(fL, 0));
+    int stL = 0, stR = 0;
+    const double hL = lead_adjoint<T>(n, true, Cb(0), GsL, SLb, dH, Tm(2), a.dec_nmax, stL, lds, red);
+    const double hR = lead_adjoint<T>(n, false, Cb(N - 1), Gs, SRb, dH + (size_t)(N - 1) * nn, Tm(2), a.dec_nmax,
+                                      stR, lds, red);
+    if (threadIdx.x == 0) {
+      a.dU[(size_t)e * N] += hL;
+      a.dU[(size_t)e * N + N - 1] += hR;
+    }
+    lead_flags = (stL < 0 ? 4 : 0) | (stR < 0 ? 8 : 0);
+  }
   if (threadIdx.x == 0) {
     a.J[e] = Jv;
     if (a.info) {
       a.info[4 * (size_t)e + 0] = itL;
       a.info[4 * (size_t)e + 1] = itR;
-      a.info[4 * (size_t)e + 2] = hitL | (hitR << 1);
+      a.info[4 * (size_t)e + 2] = hitL | (hitR << 1) | lead_flags;
       a.info[4 * (size_t)e + 3] = fails;
     }
   }
@@ -391,13 +494,15 @@ __global__ __launch_bounds__(Cfg<T>::NTH, Cfg<T>::OCC) void rgf_adjoint(AdjArgs 
   for (int e = blockIdx.x; e < a.nE; e += gridDim.x) adjoint_energy<T, NMAX>(a, e, ws, dH, (ldsd*)lds, (ldsd*)lds);
 }
 
-// Sum the per-slot coupling adjoints: out[l] = sum_s dH[s][l] for l = 1..nx-2, zero at the edges.
-__global__ void k_sum_slots(const double2* __restrict__ dH, int slots, int nx, int n, double2* __restrict__ out) {
+// Sum the per-slot coupling adjoints: out[l] = sum_s dH[s][l] for l = 1..nx-2, and at
+// the edges when `edges` (contacts that repeat the edge slabs), else zero there.
+__global__ void k_sum_slots(const double2* __restrict__ dH, int slots, int nx, int n, int edges,
+                            double2* __restrict__ out) {
   const size_t nn = (size_t)n * n, tot = (size_t)nx * nn;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
     const size_t l = i / nn;
     double2 s = make_double2(0, 0);
-    if (l >= 1 && l + 1 < (size_t)nx)
+    if (edges || (l >= 1 && l + 1 < (size_t)nx))
       for (int k = 0; k < slots; ++k) {
         const double2 v = dH[(size_t)k * tot + i];
         s.x += v.x;

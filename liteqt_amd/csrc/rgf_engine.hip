@@ -1404,7 +1404,7 @@ int rgf_adjoint_batched(const rgf_params* p, const rgf_batch* b, const double* w
   HIPCHK(hipGetLastError());
   if (dH1i) {
     hipLaunchKernelGGL(rgf::k_sum_slots, dim3(1024), dim3(256), 0, s, reinterpret_cast<const double2*>(dh.p),
-                       (int)slots, b->nx, b->nm, reinterpret_cast<double2*>(dH1i));
+                       (int)slots, b->nx, b->nm, lead_H00 ? 0 : 1, reinterpret_cast<double2*>(dH1i));
     HIPCHK(hipGetLastError());
   }
   return 0;

@@ -429,10 +429,11 @@ def zinv_batched(A, stream=None):
 def adjoint_batched(p, E, temp, mu, Hii, H1i, sigL=None, sigR=None, w_tr=None, w_n=None, max_slots=0,
                     stream=None, want_dH=True, tr=None, info=None, lead_H00=None, sigR_lo=None, sigR_up=None,
                     sigL_up=None, ckpt=0):
-    """Adjoint RGF (physical mode, contacts fixed).  Tensors as in solve_batched;
-    w_tr float64 (nE,), w_n float64 (nE, nx); lead_H00 optional complex128
-    (2, nm, nm): onsite blocks of the left and right lead (default: the edge
-    slabs plus sigR there).  ckpt: the forward state kept for every slab (-1),
+    """Adjoint RGF (physical mode).  Tensors as in solve_batched; w_tr float64
+    (nE,), w_n float64 (nE, nx).  Without lead_H00 the contacts repeat the edge
+    slabs (the reference's device) and are differentiated with them; lead_H00
+    complex128 (2, nm, nm), the left and right lead onsite blocks, holds them
+    fixed.  ckpt: the forward state kept for every slab (-1),
     at checkpoints every ckpt >= 2 slabs with segments recomputed, or chosen by
     device memory (0).  Returns (J[nE], dU[nE, nx], dH1i[nx, nm, nm]
     column-major blocks or None)."""

@@ -55,3 +55,58 @@ def test_adjoint_reference_matches_oracle_fd(scatter):
               - _J(p, E, temp, mu, Hii, Cm, sL, sR, leadL, leadR, w_tr, w_n)) / (2 * h)
         ad = np.real(np.vdot(Hb[:, :, l], D))
         assert abs(fd - ad) < 1e-6 * max(1, abs(fd)), (l, fd, ad)
+
+
+def _J_leads(p, E, temp, mu, Hii, H1i, sL, sR, w_tr, w_n):
+    """J of the device whose contacts repeat its edge slabs (orc_rgf, physical mode)."""
+    o = orc.rgf(p, temp, E, mu, Hii, H1i, sL, None, sR, offdiag=False)
+    return w_tr * o["tr"] + sum(w_n[l] * np.imag(np.trace(o["ndens"][:, :, l])) for l in range(Hii.shape[2]))
+
+
+@pytest.mark.parametrize("scatter", [False, True])
+def test_adjoint_with_leads_matches_oracle_fd(scatter):
+    """Contacts that repeat the edge slabs: dJ/dU_l on every slab (the edge potentials
+    shift the leads too) and dJ/dH1i_l on every coupling (the first and last also set
+    the contacts) against central differences of the oracle's own device."""
+    nm, nx, E = 3, 5, 0.15
+    Hii, H1i, sL, sR = _setup(nm, nx, 11, scatter)
+    p = orc.params(orc.PHYSICAL, eta_dev=1e-3, eta_lead=1e-3, dec_tol=1e-14, dec_nmax=200)
+    temp, mu = (300.0, 300.0), (0.2, -0.1)
+    rng = np.random.default_rng(5)
+    w_tr, w_n = 0.7, rng.standard_normal(nx)
+    fL, fR = fermi(E, mu[0], p.boltz, temp[0]), fermi(E, mu[1], p.boltz, temp[1])
+    J, dU, Hb = adjoint_ref.adjoint_with_leads(E, p.eta_dev, p.eta_lead, Hii, H1i, sL, sR, fL, fR, w_tr, w_n,
+                                               orc.sancho, p.dec_tol, p.dec_nmax)
+    J0 = _J_leads(p, E, temp, mu, Hii, H1i, sL, sR, w_tr, w_n)
+    assert abs(J - J0) < 1e-10 * max(1, abs(J0))
+    h = 1e-5
+    for l in range(nx):
+        Hp, Hm = Hii.copy(order="F"), Hii.copy(order="F")
+        Hp[:, :, l] += h * np.eye(nm)
+        Hm[:, :, l] -= h * np.eye(nm)
+        fd = (_J_leads(p, E, temp, mu, Hp, H1i, sL, sR, w_tr, w_n)
+              - _J_leads(p, E, temp, mu, Hm, H1i, sL, sR, w_tr, w_n)) / (2 * h)
+        assert abs(fd - dU[l]) < 1e-6 * max(1, abs(fd)), (l, fd, dU[l])
+    for l in range(nx):
+        D = rng.standard_normal((nm, nm)) + 1j * rng.standard_normal((nm, nm))
+        Cp, Cm = H1i.copy(order="F"), H1i.copy(order="F")
+        Cp[:, :, l] += h * D
+        Cm[:, :, l] -= h * D
+        fd = (_J_leads(p, E, temp, mu, Hii, Cp, sL, sR, w_tr, w_n)
+              - _J_leads(p, E, temp, mu, Hii, Cm, sL, sR, w_tr, w_n)) / (2 * h)
+        ad = np.real(np.vdot(Hb[:, :, l], D))
+        assert abs(fd - ad) < 1e-6 * max(1, abs(fd)), (l, fd, ad)
+
+
+def test_stein_doubling_matches_the_direct_solve():
+    """The doubling sum of X = Q + P X P2 against the Kronecker-product solve."""
+    rng = np.random.default_rng(2)
+    n = 4
+    P = 0.3 * (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    P2 = 0.3 * (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    Q = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    X, k = adjoint_ref.stein(Q, P, P2)
+    # vec(P X P2) = (P2^T kron P) vec(X), column-major vec
+    A = np.eye(n * n) - np.kron(P2.T, P)
+    Xd = np.linalg.solve(A, Q.reshape(-1, order="F")).reshape(n, n, order="F")
+    assert k < 20 and np.abs(X - Xd).max() < 1e-12 * np.abs(Xd).max()

@@ -30,8 +30,21 @@ def _case(nm, nx, seed, scatter=True):
     return Hii, H1i, sL, sR
 
 
-@pytest.mark.parametrize("nm,nx,scatter", [(32, 4, True), (64, 5, False), (32, 2, True), (128, 3, True)])
-def test_adjoint_kernel_matches_reference_reverse_pass(nm, nx, scatter):
+def _fixed_leads(rgf, Hii, sR, dev):
+    """lead_H00 of the unperturbed device (edge slabs plus sigR there): contacts held fixed."""
+    nx = Hii.shape[2]
+    L = Hii[:, :, 0] + (sR[:, :, 0] if sR is not None else 0)
+    R = Hii[:, :, nx - 1] + (sR[:, :, nx - 1] if sR is not None else 0)
+    return rgf.to_device_blocks(np.stack([L, R], axis=2), dev)
+
+
+@pytest.mark.parametrize("nm,nx,scatter,leads", [(32, 4, True, "follow"), (64, 5, False, "follow"),
+                                                 (32, 2, True, "follow"), (128, 3, True, "follow"),
+                                                 (32, 4, True, "fixed"), (64, 5, False, "fixed")])
+def test_adjoint_kernel_matches_reference_reverse_pass(nm, nx, scatter, leads):
+    """Contacts that repeat the edge slabs (lead_H00 absent: their derivatives through
+    the decimation included, tests/adjoint_ref.adjoint_with_leads) and contacts held
+    fixed (lead_H00 given, tests/adjoint_ref.adjoint)."""
     torch, rgf = _env()
     dev = torch.device("cuda:0")
     Hii, H1i, sL, sR = _case(nm, nx, 40 + nm + nx, scatter)
@@ -47,15 +60,20 @@ def test_adjoint_kernel_matches_reference_reverse_pass(nm, nx, scatter):
         p, torch.tensor(E, device=dev), temp, mu, rgf.to_device_blocks(Hii, dev), rgf.to_device_blocks(H1i, dev),
         sigL=None if sL is None else rgf.to_device_blocks(sL, dev),
         sigR=None if sR is None else rgf.to_device_blocks(sR, dev),
-        w_tr=torch.tensor(w_tr, device=dev), w_n=torch.tensor(w_n, device=dev))
+        w_tr=torch.tensor(w_tr, device=dev), w_n=torch.tensor(w_n, device=dev),
+        lead_H00=_fixed_leads(rgf, Hii, sR, dev) if leads == "fixed" else None)
     torch.cuda.synchronize()
     Jg, dUg = Jg.cpu().numpy(), dUg.cpu().numpy()
     dHg = rgf.from_device_blocks(dHg)
     dH_ref = np.zeros((nm, nm, nx), complex)
     for e, En in enumerate(E):
-        SigL, SigR = lead_self_energies(orc.sancho, En, Hii, H1i, sR, po.eta_lead, po.dec_tol, po.dec_nmax)
         fL, fR = fermi(En, mu[0], po.boltz, temp[0]), fermi(En, mu[1], po.boltz, temp[1])
-        J, dU, dH = adjoint_ref.adjoint(En, po.eta_dev, Hii, H1i, sL, sR, SigL, SigR, fL, fR, w_tr[e], w_n[e])
+        if leads == "fixed":
+            SigL, SigR = lead_self_energies(orc.sancho, En, Hii, H1i, sR, po.eta_lead, po.dec_tol, po.dec_nmax)
+            J, dU, dH = adjoint_ref.adjoint(En, po.eta_dev, Hii, H1i, sL, sR, SigL, SigR, fL, fR, w_tr[e], w_n[e])
+        else:
+            J, dU, dH = adjoint_ref.adjoint_with_leads(En, po.eta_dev, po.eta_lead, Hii, H1i, sL, sR, fL, fR, w_tr[e],
+                                                       w_n[e], orc.sancho, po.dec_tol, po.dec_nmax)
         assert abs(Jg[e] - J) <= 1e-10 * max(1.0, abs(J)), (e, Jg[e], J)
         assert np.linalg.norm(dUg[e] - dU) <= 1e-9 * max(1.0, np.linalg.norm(dU)), (e, dUg[e], dU)
         dH_ref += dH
@@ -64,7 +82,9 @@ def test_adjoint_kernel_matches_reference_reverse_pass(nm, nx, scatter):
 
 def test_adjoint_kernel_matches_oracle_central_fd():
     """dJ/dU_l and directional dJ/dH1i_l vs central differences (h = 1e-6) of the
-    CPU oracle with the contacts held fixed, on random slabs."""
+    CPU oracle's own device, whose contacts repeat the edge slabs: every slab's
+    potential (the edge ones move the leads too) and every coupling, the contact
+    couplings included."""
     torch, rgf = _env()
     dev = torch.device("cuda:0")
     nm, nx = 32, 12
@@ -81,26 +101,30 @@ def test_adjoint_kernel_matches_oracle_central_fd():
     torch.cuda.synchronize()
     dU = dU.cpu().numpy()[0]
     dH = rgf.from_device_blocks(dH)
-    leadL, leadR = Hii[:, :, 0].copy(), Hii[:, :, nx - 1].copy()
 
     def Jo(Hm, Cm):
-        o = orc.rgf_fixed_leads(po, temp, E0, mu, Hm, Cm, None, None, leadL, leadR)
+        o = orc.rgf(po, temp, E0, mu, Hm, Cm, offdiag=False)
         return o["tr"] + sum(w_n[l] * np.imag(np.trace(o["ndens"][:, :, l])) for l in range(nx))
 
     h = 1e-6
     rng = np.random.default_rng(5)
-    for l in rng.choice(nx, 8, replace=False):
+    for l in [0, nx - 1] + list(rng.choice(np.arange(1, nx - 1), 6, replace=False)):
         Hp, Hm = Hii.copy(order="F"), Hii.copy(order="F")
         Hp[:, :, l] += h * np.eye(nm)
         Hm[:, :, l] -= h * np.eye(nm)
         fd = (Jo(Hp, H1i) - Jo(Hm, H1i)) / (2 * h)
         assert abs(fd - dU[l]) <= 1e-6 * max(1.0, abs(fd)), (l, fd, dU[l])
-    for l in rng.choice(np.arange(1, nx - 1), 4, replace=False):
-        D = rng.standard_normal((nm, nm)) + 1j * rng.standard_normal((nm, nm))
+    def fd_coupling(l, D, step):
         Cp, Cm = H1i.copy(order="F"), H1i.copy(order="F")
-        Cp[:, :, l] += h * D
-        Cm[:, :, l] -= h * D
-        fd = (Jo(Hii, Cp) - Jo(Hii, Cm)) / (2 * h)
+        Cp[:, :, l] += step * D
+        Cm[:, :, l] -= step * D
+        return (Jo(Hii, Cp) - Jo(Hii, Cm)) / (2 * step)
+
+    for l in [0, nx - 1] + list(rng.choice(np.arange(1, nx - 1), 3, replace=False)):
+        D = rng.standard_normal((nm, nm)) + 1j * rng.standard_normal((nm, nm))
+        # Richardson: J is strongly curved along the contact couplings at eta = 1e-4 (the
+        # O(h^2) term of one central difference reaches 1e-5 relative at h = 1e-6)
+        fd = (4.0 * fd_coupling(l, D, h) - fd_coupling(l, D, 2 * h)) / 3.0
         ad = np.real(np.vdot(dH[:, :, l], D))
         assert abs(fd - ad) <= 1e-6 * max(1.0, abs(fd)), (l, fd, ad)
 
@@ -121,6 +145,11 @@ def test_autograd_function_gradcheck():
     Tint = Cd[1:nx - 1].clone().requires_grad_(True)
     f = lambda U, Tint: RGFObjective.apply(U, Tint, p, E, (300.0, 300.0), (0.1, -0.1), Hd, Cd, sLd, sRd, w_tr, w_n)
     assert torch.autograd.gradcheck(f, (U, Tint), eps=1e-6, atol=1e-6, rtol=1e-5, fast_mode=True)
+    # leads that repeat the edge slabs: every potential and every coupling, the contact ones included
+    Tall = Cd.clone().requires_grad_(True)
+    g = lambda U, T: RGFObjective.apply(U, T, p, E, (300.0, 300.0), (0.1, -0.1), Hd, Cd, sLd, sRd, w_tr, w_n, None,
+                                        True)
+    assert torch.autograd.gradcheck(g, (U, Tall), eps=1e-6, atol=1e-6, rtol=1e-5, fast_mode=True)
 
 
 def _od_case(nm, nx, seed, E0):
@@ -147,7 +176,8 @@ def test_offdiag_adjoint_kernel_matches_reference_reverse_pass(nm, nx):
     sd = {k: rgf.to_device_blocks(v, dev) for k, v in zip(names, sig)}
     Jg, dUg, dHg = rgf.adjoint_batched(p, torch.tensor(E, dtype=torch.float64, device=dev), temp, mu,
                                        rgf.to_device_blocks(Hii, dev), rgf.to_device_blocks(H1i, dev),
-                                       w_tr=torch.tensor(w_tr, device=dev), w_n=torch.tensor(w_n, device=dev), **sd)
+                                       w_tr=torch.tensor(w_tr, device=dev), w_n=torch.tensor(w_n, device=dev),
+                                       lead_H00=_fixed_leads(rgf, Hii, sig[1], dev), **sd)
     torch.cuda.synchronize()
     Jg, dUg, dHg = Jg.cpu().numpy(), dUg.cpu().numpy(), rgf.from_device_blocks(dHg)
     dH_ref = np.zeros((nm, nm, nx), complex)
@@ -177,7 +207,8 @@ def test_offdiag_adjoint_density_gradient_matches_oracle_fd():
     sd = {k: rgf.to_device_blocks(v, dev) for k, v in zip(names, sig)}
     _, dU, _ = rgf.adjoint_batched(p, torch.tensor([E0], dtype=torch.float64, device=dev), temp, mu,
                                    rgf.to_device_blocks(Hii, dev), rgf.to_device_blocks(H1i, dev),
-                                   w_n=torch.tensor(w_n[None, :], device=dev), **sd)
+                                   w_n=torch.tensor(w_n[None, :], device=dev),
+                                   lead_H00=_fixed_leads(rgf, Hii, sig[1], dev), **sd)
     torch.cuda.synchronize()
     dU = dU.cpu().numpy()[0]
 

@@ -191,3 +191,40 @@ def test_adjoint_checkpointed_program_in_bounds_and_equal_to_keep_all(hostlib, n
         assert outs[(0, K)][4] > base[4]
     else:  # one segment stays in the buffer from the forward sweep on
         assert outs[(0, K)][4] == base[4]
+
+
+def test_adjoint_with_contacts_program_equal_to_reference(hostlib):
+    """The adjoint program with contacts that repeat the edge slabs (no fixed lead
+    blocks): the contacts' derivatives through the decimation's fixed point (edge dU,
+    edge couplings) equal tests/adjoint_ref.adjoint_with_leads, which central
+    differences of the oracle pin (tests/test_adjoint_ref.py)."""
+    from tests import adjoint_ref
+    from tests.dense_ref import fermi
+    nm, nx, nE = 4, 6, 2
+    Hii, H1i = configs.random_device(nm, nx, 41, coupling_sigma=1.0, identical_leads=False)
+    E = np.array([-0.3, 0.2])
+    sL, sG, sR = configs.scattering_self_energies(nm, nx, 42, E=E, scale=0.05)
+    temp, mu = np.array([300.0, 250.0]), np.array([0.1, -0.1])
+    w_tr = np.array([1.0, 0.5])
+    w_n = np.random.default_rng(2).standard_normal((nE, nx))
+    eta_dev, eta_lead, tol, nmax, boltz = 1e-4, 1e-2, 1e-12, 100, 8.61734e-5
+    J, tr, dU = np.zeros(nE), np.zeros(nE), np.zeros((nE, nx))
+    dH = np.zeros((nm, nm, nx), np.complex128, order="F")
+    nops = ctypes.c_longlong(0)
+    log = ctypes.create_string_buffer(1 << 16)
+    f = lambda a: np.asfortranarray(a, np.complex128)
+    Hf, Cf, sLf, sRf = f(Hii), f(H1i), f(sL), f(sR)
+    nv = hostlib.host_adjoint(nm, nx, nE, _p(E), _p(temp), _p(mu), eta_dev, eta_lead, tol, nmax, boltz,
+                              _p(Hf), _p(Cf), _p(sLf), _p(sRf), _p(w_tr), _p(w_n), _p(J), _p(tr), _p(dU), _p(dH),
+                              0, 0, ctypes.byref(nops), log, len(log))
+    assert nv == 0, log.value.decode()
+    dH_ref = np.zeros_like(dH)
+    for e in range(nE):
+        fL, fR = fermi(E[e], mu[0], boltz, temp[0]), fermi(E[e], mu[1], boltz, temp[1])
+        Jr, dUr, Hr = adjoint_ref.adjoint_with_leads(E[e], eta_dev, eta_lead, Hii, H1i, sL[..., e], sR[..., e], fL, fR,
+                                                     w_tr[e], w_n[e], orc.sancho, tol, nmax)
+        assert abs(J[e] - Jr) <= 1e-10 * max(1.0, abs(Jr)), e
+        assert np.abs(dU[e] - dUr).max() <= 1e-9 * max(1.0, np.abs(dUr).max()), (e, dU[e], dUr)
+        dH_ref += Hr
+    for l in range(nx):
+        assert relerr(dH[:, :, l], dH_ref[:, :, l]) <= 1e-9, l
