@@ -1,10 +1,12 @@
 // rgf_adjoint.hpp -- reverse-mode (adjoint) RGF: gradients of
 //   J(E) = w_tr(E) * tr(E) + sum_l w_n(E,l) * Im Tr G^<_ll(E)
-// with respect to a potential U_l on the diagonal of every slab and to the
-// device-internal coupling blocks H1i(:,:,l), l = 2..nx-1 (1-based), for the
-// consistent (physical) device with the contact self-energies and the
-// scattering self-energy (diagonal and, optionally, first off-diagonal
-// blocks) held fixed.
+// with respect to a potential U_l on the diagonal of every slab and to every
+// coupling block H1i(:,:,l), for the consistent (physical) device with the
+// scattering self-energy (diagonal and, optionally, first off-diagonal blocks)
+// held fixed.  The contacts either repeat the edge slabs (the reference's
+// device, differentiated through the decimation: lead_adjoint) or are held
+// fixed at given onsite blocks (AdjArgs.lead; the edge couplings then get no
+// gradient).
 // tr is the Meir-Wingreen current spectrum into the right contact
 // (src/green_rgf.f90:4058-4066): with f_L = 1, f_R = 0 it is T(E); with Fermi
 // factors and weights dE*e/h it gives dI/dU_l.  Densities are the integrands of
