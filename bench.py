@@ -616,6 +616,59 @@ def gw_bench(args):
     print(json.dumps(res))
 
 
+def gw_loop_bench(args):
+    """One iteration of green_rgf_solve_gw_1D (src/green_rgf.f90:3124-3305) at the size of
+    the reference's example (examples/cnt_8-0, fixtures in tests/golden/cnt_8-0: nm = 128,
+    18 slabs, 2000 energies in [-15, 5] eV, only-diag Coulomb blocks): the batched solve
+    with the running Sigma, P, the dense W of green_calc_w_full and Sigma, each stage
+    timed (synchronised) on the GPU.  The example runs its `num iter` = 30 iterations of
+    this."""
+    import torch
+    from liteqt_amd import configs, example_gw, example_rgf, wannier
+    ex = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "cnt_8-0")
+    cfg = example_rgf.read_example_input(os.path.join(ex, "input"))
+    w = wannier.load_ham_dat(os.path.join(ex, "ham_dat"))
+    ns, length = cfg["ns"], cfg["length"]
+    pot = wannier.read_pot(os.path.join(ex, "pot_dat"), length) if cfg["lreadpot"] else None
+    Hii, H1i = wannier.device_blocks(w, ns, length, pot, cfg["potscale"])
+    Vii, V1i = wannier.device_coulomb(w, ns, length, cfg["eps"], cfg["r0"], cfg["ldiag"])
+    nm = w.nb * ns
+    ndiag = 1 if cfg["ldiag"] else nm
+    en = configs.energy_grid(cfg["emin"], cfg["emax"], cfg["nen"])
+    stages = {}
+    t0 = [0.0]
+
+    def cb(it, st):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        stages[st["stage"]] = t - t0[0]
+        print(f"bench: GW iteration {it} stage {st['stage']} {t - t0[0]:.3f}s", file=sys.stderr, flush=True)
+        t0[0] = time.perf_counter()
+
+    torch.cuda.synchronize()
+    t0[0] = time.perf_counter()
+    start = t0[0]
+    st = example_gw.run_loop(Hii, H1i, Vii, V1i, en, cfg["temp"], cfg["mu"], ndiag, float(w.spin_deg), cfg["alpha"],
+                             0, on_iteration=cb)
+    total = time.perf_counter() - start
+    NT = nm * length
+    # dense W per energy: LU (16/3 NT^3) and the solve with NT right-hand sides (8 NT^3) in real flops
+    w_flops = (16.0 / 3.0 + 8.0) * NT ** 3 * len(en)
+    res = {
+        "metric": "GW loop iteration (green_rgf_solve_gw_1D) at the example's size",
+        "value": 1.0 / total, "unit": "iterations/s", "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": total * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c128 (complex fp64)",
+        "data": "the reference's example input (cnt_8-0)",
+        "config": {"workload": "cnt_8-0 GW iteration 0", "nm": nm, "nx": length, "nen": len(en), "ndiag": ndiag,
+                   "iterations_in_the_example": cfg["niter"]},
+        "stages_s": stages,
+        "dense_w": {"NT": NT, "flops": w_flops, "achieved_tflops": w_flops / stages["W"] / 1e12,
+                    "frac": w_flops / stages["W"] / 1e12 / PEAK_FP64_MFMA},
+        "tr_sum": float(st["tr"].sum()),
+    }
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -643,6 +696,8 @@ def main():
                     help="divide each device's energy count (U[256,2048]) by this")
     ap.add_argument("--plain-energies", type=int, default=4,
                     help="energies timed on the plain per-energy GPU path (0: skip)")
+    ap.add_argument("--gw-loop", action="store_true",
+                    help="one iteration of the 1D GW loop at the example's size, per stage (instead of the RGF bench)")
     ap.add_argument("--gw", action="store_true",
                     help="only measure the GW self-energy step (polarization + self-energy correlations)")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
@@ -664,6 +719,9 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
+    if args.gw_loop:
+        gw_loop_bench(args)
+        return
     if args.gw:
         gw_bench(args)
         return
