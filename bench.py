@@ -620,9 +620,10 @@ def gw_loop_bench(args):
     """One iteration of green_rgf_solve_gw_1D (src/green_rgf.f90:3124-3305) at the size of
     the reference's example (examples/cnt_8-0, fixtures in tests/golden/cnt_8-0: nm = 128,
     18 slabs, 2000 energies in [-15, 5] eV, only-diag Coulomb blocks): the batched solve
-    with the running Sigma, P, the dense W of green_calc_w_full and Sigma, each stage
-    timed (synchronised) on the GPU.  The example runs its `num iter` = 30 iterations of
-    this."""
+    with the running Sigma, P, W (the blocks of green_calc_w_full by the super-block
+    solve) and Sigma, each stage timed (synchronised) on the GPU; then the dense LU form
+    of W on the same P, timed and compared.  The example runs its `num iter` = 30
+    iterations of this."""
     import torch
     from liteqt_amd import configs, example_gw, example_rgf, wannier
     ex = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "cnt_8-0")
@@ -651,6 +652,23 @@ def gw_loop_bench(args):
     st = example_gw.run_loop(Hii, H1i, Vii, V1i, en, cfg["temp"], cfg["mu"], ndiag, float(w.spin_deg), cfg["alpha"],
                              0, on_iteration=cb)
     total = time.perf_counter() - start
+    # the loop's W (the super-block solve, gw.w_blocks, where 2 nm <= 256) against the
+    # dense LU form of green_calc_w_full (gw.w_full) on this iteration's P: time and agreement
+    from liteqt_amd import gw, rgf
+    tb = lambda a: rgf.to_device_blocks(a, "cuda:0")
+    Vd, V1d = tb(Vii), tb(V1i)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    Wf = gw.w_full(Vd, V1d, st["PL"], st["PG"], st["PR"])
+    torch.cuda.synchronize()
+    dense_s = time.perf_counter() - t1
+    rel = {k: torch.linalg.vector_norm(st[k] - a).item() / max(torch.linalg.vector_norm(a).item(), 1e-300)
+           for k, a in zip(("WL", "WG", "WR"), Wf)}
+    pl = st["PL"]
+    asym = torch.linalg.vector_norm(pl + pl.conj().transpose(-1, -2)).item() / max(
+        torch.linalg.vector_norm(pl).item(), 1e-300)
+    print(f"bench: dense W {dense_s:.3f}s (loop's W {stages['W']:.3f}s), difference {rel}", file=sys.stderr, flush=True)
+    del Wf
     NT = nm * length
     # dense W per energy: LU (16/3 NT^3) and the solve with NT right-hand sides (8 NT^3) in real flops
     w_flops = (16.0 / 3.0 + 8.0) * NT ** 3 * len(en)
@@ -662,8 +680,11 @@ def gw_loop_bench(args):
         "config": {"workload": "cnt_8-0 GW iteration 0", "nm": nm, "nx": length, "nen": len(en), "ndiag": ndiag,
                    "iterations_in_the_example": cfg["niter"]},
         "stages_s": stages,
-        "dense_w": {"NT": NT, "flops": w_flops, "achieved_tflops": w_flops / stages["W"] / 1e12,
-                    "frac": w_flops / stages["W"] / 1e12 / PEAK_FP64_MFMA},
+        "w_method": "blocks" if 2 * nm <= 256 else "dense",
+        "dense_w": {"NT": NT, "s": dense_s, "flops": w_flops, "achieved_tflops": w_flops / dense_s / 1e12,
+                    "frac": w_flops / dense_s / 1e12 / PEAK_FP64_MFMA,
+                    "loop_w_over_dense": stages["W"] / dense_s, "loop_w_relative_difference": rel,
+                    "P_lesser_anti_hermitian_defect": asym},
         "tr_sum": float(st["tr"].sum()),
     }
     print(json.dumps(res))

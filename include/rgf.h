@@ -171,6 +171,11 @@ typedef struct rgf_batch {
    * free device memory leave CUs idle and checkpoints every ceil(sqrt(nx)) slabs
    * run more energies per unit of time. */
   int ckpt;
+  /* Lower off-diagonal blocks G^R_{l+1,l} (l = 1..nx-1 at block l-1, as GRi1),
+   * written by the block-tridiagonal-Sigma solve only (sigR_lo, sigR_up or sigL_up
+   * given); NULL elsewhere.  With a non-Hermitian effective A = z - H - Sigma^R
+   * (Sigma^R_{l-1,l} unrelated to Sigma^R_{l,l-1}) it is not fixed by GRi1. */
+  double *GRi1_lo;
 } rgf_batch;
 
 int rgf_solve_batched(const rgf_params *p, const rgf_batch *b, void *stream);

@@ -317,6 +317,8 @@ int prepare(const rgf_params* p, const rgf_batch* b, const rgf_scba* scba, Prepa
   if (b->ckpt < -1) return fail(RGF_EINVAL, "ckpt must be -1 (keep every slab), 0 (automatic) or a segment length");
   if (b->ckpt > 0 && (od || cms))
     return fail(RGF_EUNSUP, "segment recompute (ckpt > 0) is for the diagonal-Sigma solve only");
+  if (b->GRi1_lo && !od)
+    return fail(RGF_EUNSUP, "GRi1_lo is written by the block-tridiagonal-Sigma solve only");
   if (scba) {
     if (!b->sigR || !b->sigL) return fail(RGF_EINVAL, "SCBA refresh: sigR and sigL are required");
     if (b->sig_stride != (long long)b->nm * b->nm * b->nx)
@@ -378,6 +380,7 @@ void fill_args(const rgf_params* p, const rgf_batch* b, long long se, rgf::Sweep
   a.GRi1 = reinterpret_cast<double2*>(b->GRi1);
   a.GLi1 = reinterpret_cast<double2*>(b->GLi1);
   a.GGi1 = reinterpret_cast<double2*>(b->GGi1);
+  a.GRi1_lo = reinterpret_cast<double2*>(b->GRi1_lo);
   a.cur = b->cur;
   a.tr = b->tr;
   a.tre = b->tre;

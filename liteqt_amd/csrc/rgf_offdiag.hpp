@@ -142,7 +142,7 @@ __device__ void solve_energy_od(const SweepArgs& a, int e, double2* ws, ldsd* ld
   refresh(sL(L1), Gn);
 
   // ---- backward sweep
-  const bool want_lo = a.GGi1 || (a.scba && a.sigR_lo);
+  const bool want_lo = a.GGi1 || a.GRi1_lo || (a.scba && a.sigR_lo);
   for (int l = N - 2; l >= 0; --l) {
     const double2* s = sLup(l + 1);
     couple(l + 1);
@@ -163,6 +163,7 @@ __device__ void solve_energy_od(const SweepArgs& a, int e, double2* ws, ldsd* ld
     if (a.GRi1) wg_ew(n, ob(a.GRi1, l), 0, zero, R, one, nullptr, one, nullptr, one);
     if (a.GLi1) wg_ew(n, ob(a.GLi1, l), 0, zero, Gnn, one, nullptr, one, nullptr, one);
     if (a.GGi1) wg_ew(n, ob(a.GGi1, l), 0, zero, Gnn, one, R, one, Glo, mone);
+    if (a.GRi1_lo) wg_ew(n, ob(a.GRi1_lo, l), 0, zero, Glo, one, nullptr, one, nullptr, one);
     if (a.cur) {
       // Re(H_{l,l+1} G<_{l+1,l}) - Re(H_{l+1,l} G<_{l,l+1}),  G<_{l+1,l} = -(G<_{l,l+1})^H
       wg_gemm<T>(n, OP_N, Gnn, OP_N, Cb(l + 1), K1, 0, st, lds);

@@ -58,6 +58,8 @@ struct SweepArgs {
   // checkpoint slabs 0, K, 2K, ... and one K-slab segment, recomputing a segment's
   // left-connected blocks from its checkpoint before the backward sweep enters it
   int ckpt;
+  // G^R_{l+1,l} per bond (rgf_offdiag.hpp only), or null
+  double2* GRi1_lo;
 };
 
 // Workspace blocks of one energy slot: the stored left-connected blocks (all slabs,

@@ -24,11 +24,14 @@ from .example_rgf import read_example_input
 
 
 def run_loop(Hii, H1i, Vii, V1i, en, temp, mu, ndiag, spindeg, alpha_mix, niter, on_iteration=None,
-             device="cuda:0"):
+             device="cuda:0", w_method="auto"):
     """The loop on numpy Fortran blocks (Hii, H1i, Vii, V1i: (nm, nm, nx)); returns the
     device tensors of the last iteration (G^<, G^>, G^r, P, W and the running Sigma,
     shaped (nen, nx, nm, nm)) and per-iteration (tr, tre).  on_iteration(it, state) is
-    called after each stage with state["stage"] in ("G", "P", "W", "Sigma")."""
+    called after each stage with state["stage"] in ("G", "P", "W", "Sigma").  w_method:
+    "dense" (gw.w_full, the LU solve of green_calc_w_full), "blocks" (gw.w_blocks, the
+    same blocks by the block-tridiagonal solve on two-slab super-blocks) or "auto"
+    (blocks where 2 nm <= 256, else dense)."""
     import torch
     dev = torch.device(device)
     nm, _, nx = Hii.shape
@@ -53,7 +56,9 @@ def run_loop(Hii, H1i, Vii, V1i, en, temp, mu, ndiag, spindeg, alpha_mix, niter,
         _stage(on_iteration, it, "G", st)
         gw.polarization(gp, st["GL"], st["GG"], st["GR"], st["PL"], st["PG"], st["PR"])
         _stage(on_iteration, it, "P", st)
-        gw.w_full(Vd, V1d, st["PL"], st["PG"], st["PR"], st["WL"], st["WG"], st["WR"])
+        blocks = w_method == "blocks" or (w_method == "auto" and 2 * nm <= 256 and (2 * nm) % 32 == 0)
+        wf = gw.w_blocks if blocks else gw.w_full
+        wf(Vd, V1d, st["PL"], st["PG"], st["PR"], st["WL"], st["WG"], st["WR"])
         _stage(on_iteration, it, "W", st)
         gw.selfenergy(gp, st["GL"], st["GG"], st["GR"], st["WL"], st["WG"], st["WR"], st["SL"], st["SG"], st["SR"])
         _stage(on_iteration, it, "Sigma", st)

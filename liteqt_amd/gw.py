@@ -128,6 +128,126 @@ def w_full(Vii, V1i, PL, PG, PR, WL=None, WG=None, WR=None):
     return WL, WG, WR
 
 
+def w_blocks(Vii, V1i, PL, PG, PR, WL=None, WG=None, WR=None, chunk=256):
+    """The same diagonal blocks as w_full, W^r = (I - V P^r)^-1 V and W^{<,>} = W^r P^{<,>}
+    W^r^H, at O(nx nm^3) per energy instead of O((nx nm)^3): M = I - V P^r is block-
+    tridiagonal and S = V P^< V^H block-pentadiagonal, so on super-blocks of two slabs
+    (2 nm orbitals, max(2, ceil(nx/2)) of them, padded with decoupled zero slabs) both are
+    block-tridiagonal and the engine's block-tridiagonal-Sigma solve gives X = M^-1 on its
+    diagonal and first off-diagonal super-blocks and X S X^H on its diagonal ones: it runs
+    with H = 0, z = 0, Sigma^R = -M (diagonal and off-diagonal super-blocks, so A = M and
+    the zero edge couplings leave the contacts out) and Sigma^< = S.  Then W^r_ii =
+    X_{i,i-1} V_{i-1,i} + X_ii V_ii + X_{i,i+1} V_{i+1,i}, W^<_ii = (X S X^H)_ii, and W^>
+    by a second solve with S = V P^> V^H.  The solve takes Sigma^<_{l,l-1} =
+    -(Sigma^<_{l-1,l})^H, so P^< and P^> must be anti-Hermitian, as the polarization step
+    makes them (then S is too); P^r and V are general.  Needs 2 nm a multiple of 32,
+    at most 256.  Tensors as in w_full; `chunk` energies per solve."""
+    import torch
+    from . import rgf
+    nen, nx, nm = int(PR.shape[0]), int(PR.shape[1]), int(PR.shape[2])
+    for t, name in ((PL, "PL"), (PG, "PG"), (PR, "PR")):
+        if t.dtype != torch.complex128 or not t.is_cuda or not t.is_contiguous() or tuple(t.shape) != (nen, nx, nm, nm):
+            raise GwError(f"{name}: need a contiguous complex128 CUDA tensor of shape {(nen, nx, nm, nm)}")
+    for t, name in ((Vii, "Vii"), (V1i, "V1i")):
+        if t.dtype != torch.complex128 or not t.is_cuda or not t.is_contiguous() or tuple(t.shape) != (nx, nm, nm):
+            raise GwError(f"{name}: need a contiguous complex128 CUDA tensor of shape {(nx, nm, nm)}")
+    n2 = 2 * nm
+    if n2 % 32 or n2 > 256 or nx < 1:
+        raise GwError("w_blocks: 2 nm must be a multiple of 32 and at most 256 (use w_full)")
+    WL = torch.empty_like(PR) if WL is None else WL
+    WG = torch.empty_like(PR) if WG is None else WG
+    WR = torch.empty_like(PR) if WR is None else WR
+    dev = PR.device
+    tm = lambda t: t.transpose(-1, -2)  # the device blocks hold A^T: these are the matrices
+    H_ = lambda t: t.conj().transpose(-1, -2)
+    nxp = max(2, (nx + 1) // 2)  # super-blocks (the solve needs two)
+    nx2 = 2 * nxp
+    Vd = torch.zeros((nx2, nm, nm), dtype=torch.complex128, device=dev)
+    V1 = torch.zeros_like(Vd)  # V1[i] = V_{i+1,i}
+    Vd[:nx] = tm(Vii)
+    V1[:nx - 1] = tm(V1i)[:nx - 1]
+    eye = torch.eye(nm, dtype=torch.complex128, device=dev)
+    p = rgf.params(rgf.MODE_PHYSICAL, eta_dev=0.0, eta_lead=1e-3)
+    zsup = torch.zeros((nxp, n2, n2), dtype=torch.complex128, device=dev)  # H = 0
+
+    def pad(P):  # (ce, nx, nm, nm) device blocks -> (ce, nx2, nm, nm) matrices
+        out = torch.zeros((P.shape[0], nx2, nm, nm), dtype=torch.complex128, device=dev)
+        out[:, :nx] = tm(P)
+        return out
+
+    def lesser_source(P):  # super-blocks of S = V P V^H: diagonal and (s-1, s)
+        ce = P.shape[0]
+        Vu = H_(V1)  # Vu[i] = V_{i,i+1}
+        Sii = Vd @ P @ H_(Vd)
+        Sii[:, 1:] += V1[:-1] @ P[:, :-1] @ H_(V1[:-1])
+        Sii[:, :-1] += Vu[:-1] @ P[:, 1:] @ V1[:-1]
+        Sup1 = Vd[:-1] @ P[:, :-1] @ H_(V1[:-1]) + Vu[:-1] @ P[:, 1:] @ H_(Vd[1:])  # S_{i,i+1}
+        Slo1 = V1[:-1] @ P[:, :-1] @ H_(Vd[:-1]) + Vd[1:] @ P[:, 1:] @ V1[:-1]     # S_{i+1,i}
+        Sup2 = Vu[:-2] @ P[:, 1:-1] @ H_(V1[1:-1]) if nx2 > 2 else None           # S_{i,i+2}
+        Sd = torch.zeros((ce, nxp, n2, n2), dtype=torch.complex128, device=dev)
+        Sd[..., :nm, :nm] = Sii[:, 0::2]
+        Sd[..., nm:, nm:] = Sii[:, 1::2]
+        Sd[..., :nm, nm:] = Sup1[:, 0::2]
+        Sd[..., nm:, :nm] = Slo1[:, 0::2]
+        Su = torch.zeros_like(Sd)
+        if nxp > 1:
+            Su[:, 1:, :nm, :nm] = Sup2[:, 0::2]   # S_{2s-2,2s}
+            Su[:, 1:, nm:, :nm] = Sup1[:, 1::2]   # S_{2s-1,2s}
+            Su[:, 1:, nm:, nm:] = Sup2[:, 1::2]   # S_{2s-1,2s+1}
+        return tm(Sd).contiguous(), tm(Su).contiguous()
+
+    for e0 in range(0, nen, chunk):
+        e1 = min(nen, e0 + chunk)
+        ce = e1 - e0
+        Pr = pad(PR[e0:e1])
+        Mii = eye - Vd @ Pr
+        Mlo = -(V1[:-1] @ Pr[:, :-1])       # M_{i+1,i}
+        Mup = -(H_(V1[:-1]) @ Pr[:, 1:])    # M_{i,i+1}
+        A = torch.zeros((ce, nxp, n2, n2), dtype=torch.complex128, device=dev)
+        A[..., :nm, :nm] = Mii[:, 0::2]
+        A[..., nm:, nm:] = Mii[:, 1::2]
+        A[..., :nm, nm:] = Mup[:, 0::2]
+        A[..., nm:, :nm] = Mlo[:, 0::2]
+        Lo = torch.zeros_like(A)
+        Up = torch.zeros_like(A)
+        if nxp > 1:
+            Lo[:, 1:, :nm, nm:] = Mlo[:, 1::2]  # M_{2s,2s-1}
+            Up[:, 1:, nm:, :nm] = Mup[:, 1::2]  # M_{2s-1,2s}
+        sig = {"sigR": tm(-A).contiguous(), "sigR_lo": tm(-Lo).contiguous(), "sigR_up": tm(-Up).contiguous()}
+        del Pr, Mii, Mlo, Mup, A, Lo, Up
+        E = torch.zeros(ce, dtype=torch.float64, device=dev)
+        shape = (ce, nxp, n2, n2)
+        X = torch.empty(shape, dtype=torch.complex128, device=dev)
+        Xu = torch.zeros(shape, dtype=torch.complex128, device=dev)
+        Xl = torch.zeros(shape, dtype=torch.complex128, device=dev)
+        for P, Wout, first in ((PL, WL, True), (PG, WG, False)):
+            Sd, Su = lesser_source(pad(P[e0:e1]))
+            Gn = torch.empty(shape, dtype=torch.complex128, device=dev)
+            out = {"ndens": Gn}
+            if first:
+                out.update(ldos=X, GRi1=Xu, GRi1_lo=Xl)
+            rgf.solve_batched(p, E, (300.0, 300.0), (0.0, 0.0), zsup, zsup, sigL=Sd, sigR=sig["sigR"],
+                              sigR_lo=sig["sigR_lo"], sigR_up=sig["sigR_up"], sigL_up=Su, out=out)
+            Gt = tm(Gn)
+            Wd = torch.stack([Gt[..., :nm, :nm], Gt[..., nm:, nm:]], dim=2).reshape(ce, nx2, nm, nm)
+            Wout[e0:e1] = tm(Wd[:, :nx])
+            del Sd, Su, Gn, Gt, Wd
+        Xt, Xut, Xlt = tm(X), tm(Xu), tm(Xl)
+        Wr = torch.zeros((ce, nx2, nm, nm), dtype=torch.complex128, device=dev)
+        Vu = H_(V1)
+        # even slabs 2s: X_{2s,2s-1} V_{2s-1,2s} + X_{2s,2s} V_{2s,2s} + X_{2s,2s+1} V_{2s+1,2s}
+        Wr[:, 0::2] = Xt[..., :nm, :nm] @ Vd[0::2] + Xt[..., :nm, nm:] @ V1[0::2]
+        if nxp > 1:
+            Wr[:, 2::2] += Xlt[:, :-1, :nm, nm:] @ Vu[1:-1:2]
+        # odd slabs 2s+1: X_{2s+1,2s} V_{2s,2s+1} + X_{2s+1,2s+1} V_{2s+1,2s+1} + X_{2s+1,2s+2} V_{2s+2,2s+1}
+        Wr[:, 1::2] = Xt[..., nm:, :nm] @ Vu[0::2] + Xt[..., nm:, nm:] @ Vd[1::2]
+        if nxp > 1:
+            Wr[:, 1:-1:2] += Xut[:, :-1, nm:, :nm] @ V1[1:-1:2]
+        WR[e0:e1] = tm(Wr[:, :nx])
+        del X, Xu, Xl, Xt, Xut, Xlt, Wr, sig
+    return WL, WG, WR
+
+
 def to_device(a, device="cuda:0"):
     """numpy (nm, nm, nx, nen) array -> device tensor (nen, nx, nm, nm) of the same memory order."""
     import torch

@@ -49,7 +49,7 @@ class Batch(ctypes.Structure):
                 ("cur", ctypes.c_void_p), ("tr", ctypes.c_void_p), ("tre", ctypes.c_void_p),
                 ("dens_n", ctypes.c_void_p), ("dens_p", ctypes.c_void_p), ("info", ctypes.c_void_p),
                 ("sigR_lo", ctypes.c_void_p), ("sigR_up", ctypes.c_void_p), ("sigL_up", ctypes.c_void_p),
-                ("Sii", ctypes.c_void_p), ("ckpt", ctypes.c_int)]
+                ("Sii", ctypes.c_void_p), ("ckpt", ctypes.c_int), ("GRi1_lo", ctypes.c_void_p)]
 
 
 class MultiReport(ctypes.Structure):
@@ -258,7 +258,7 @@ def _req(t, name, dtype, numel, device, host_ok=False):
     return t.data_ptr()
 
 
-_OUT_KINDS = {"ldos": "c", "ndens": "c", "pdens": "c", "GRi1": "c", "GLi1": "c", "GGi1": "c", "cur": "r",
+_OUT_KINDS = {"ldos": "c", "ndens": "c", "pdens": "c", "GRi1": "c", "GLi1": "c", "GGi1": "c", "GRi1_lo": "c", "cur": "r",
               "tr": "e", "tre": "e", "dens_n": "x", "dens_p": "x", "info": "i"}
 
 

@@ -126,3 +126,32 @@ def test_w_full_symmetric_inputs_equal_the_block_recursion():
     WRr = rgf.w_solve_batched(tb(Vii), tb(V1i), tb(PL), tb(PG), tb(PR))["WR"]
     torch.cuda.synchronize()
     assert torch.linalg.vector_norm(WRf - WRr).item() <= TOL * torch.linalg.vector_norm(WRf).item()
+
+
+@pytest.mark.parametrize("nm,nx,nE,chunk", [(16, 5, 3, 256), (32, 6, 5, 2), (64, 3, 2, 256), (128, 4, 2, 1),
+                                            (48, 7, 2, 256), (16, 1, 2, 256), (32, 2, 3, 256)])
+def test_w_blocks_match_dense_oracle(nm, nx, nE, chunk):
+    """The super-block form of the dense W (gw.w_blocks: the block-tridiagonal-Sigma solve
+    on two-slab super-blocks, odd nx padded) against the oracle's dense solve and the
+    GPU's dense LU form (gw.w_full): Hermitian V, general P^r, anti-Hermitian P^<, P^>,
+    1e-10 per block; energies in chunks smaller than nE included."""
+    import torch
+    from liteqt_amd import gw
+    rgf = _rgf()
+    print(f"W blocks nm={nm} nx={nx} nE={nE} chunk={chunk}", flush=True)
+    rng = np.random.default_rng(5 * nm + nx)
+    Vii, V1i, PL, PG, PR = _inputs(rng, nm, nx, nE, False)
+    dev = torch.device("cuda:0")
+    tb = lambda a: rgf.to_device_blocks(a, dev)
+    Wb = gw.w_blocks(tb(Vii), tb(V1i), tb(PL), tb(PG), tb(PR), chunk=chunk)
+    Wf = gw.w_full(tb(Vii), tb(V1i), tb(PL), tb(PG), tb(PR))
+    torch.cuda.synchronize()
+    got = [rgf.from_device_blocks(x) for x in Wb]
+    for e in range(nE):
+        ref = wo.w_full(Vii, V1i, PL[..., e], PG[..., e], PR[..., e])
+        for name, g, r in zip(("WL", "WG", "WR"), got, ref):
+            for i in range(nx):
+                err = np.linalg.norm(g[:, :, i, e] - r[:, :, i])
+                assert err <= TOL * np.linalg.norm(r[:, :, i]), (name, e, i, err / np.linalg.norm(r[:, :, i]))
+    for name, a, b in zip(("WL", "WG", "WR"), Wb, Wf):
+        assert torch.linalg.vector_norm(a - b).item() <= TOL * torch.linalg.vector_norm(b).item(), name
