@@ -605,6 +605,66 @@ __device__ __noinline__ void wg_ew(int n, double2* out, int out_rowmajor, double
   __syncthreads();
 }
 
+// out = (X - X^H) / 2, the anti-Hermitian part of X (n a multiple of 32, out must not
+// alias X).  The 32 x 32 tile pairs (I, J), (J, I), I <= J, are read down their
+// columns into LDS, the pair's two output tiles are written down their columns; the
+// next pair's elements are loaded into registers while the current pair is combined.
+// LDS: kAntihermLds doubles; 256 threads.
+constexpr int kAntihermLds = 4 * 32 * 33;
+__device__ __noinline__ void wg_antiherm(int n, const double2* X_, double2* out_, ldsd* lds) {
+  constexpr int TS = 32, LD = TS + 1, PER = TS * TS / 256;
+  static_assert(4 * TS * LD == kAntihermLds, "LDS size");
+  const gcd2* X = gc(X_);
+  gd2* out = gw(out_);
+  ldsd* Ar = lds;
+  ldsd* Ai = Ar + TS * LD;
+  ldsd* Br = Ai + TS * LD;
+  ldsd* Bi = Br + TS * LD;
+  const int nt = n / TS, npair = nt * (nt + 1) / 2;
+  const int tid = threadIdx.x;  // element k of a tile: row tid % 32, column tid / 32 + 8k
+  const int r = tid % TS, c0 = tid / TS;
+  auto pair_of = [&](int q, int& I, int& J) {  // q-th pair (I <= J), row by row
+    I = 0;
+    while (q >= nt - I) { q -= nt - I; ++I; }
+    J = I + q;
+  };
+  double2 ra[PER], rb[PER];
+  auto load = [&](int q) {
+    int I, J;
+    pair_of(q, I, J);
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int c = c0 + 8 * k;
+      ra[k] = X[(size_t)(I * TS + r) + (size_t)(J * TS + c) * n];
+      rb[k] = X[(size_t)(J * TS + r) + (size_t)(I * TS + c) * n];
+    }
+  };
+  load(0);
+  for (int q = 0; q < npair; ++q) {
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int c = c0 + 8 * k;
+      Ar[c * LD + r] = ra[k].x; Ai[c * LD + r] = ra[k].y;
+      Br[c * LD + r] = rb[k].x; Bi[c * LD + r] = rb[k].y;
+    }
+    __syncthreads();
+    if (q + 1 < npair) load(q + 1);
+    int I, J;
+    pair_of(q, I, J);
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int c = c0 + 8 * k;
+      // A = X[I-tile rows, J-tile cols] at (c, r) of the LDS tile, B likewise
+      out[(size_t)(I * TS + r) + (size_t)(J * TS + c) * n] =
+          make_double2(0.5 * (Ar[c * LD + r] - Br[r * LD + c]), 0.5 * (Ai[c * LD + r] + Bi[r * LD + c]));
+      if (I != J)
+        out[(size_t)(J * TS + r) + (size_t)(I * TS + c) * n] =
+            make_double2(0.5 * (Br[c * LD + r] - Ar[r * LD + c]), 0.5 * (Bi[c * LD + r] + Ai[r * LD + c]));
+    }
+    __syncthreads();
+  }
+}
+
 // Y = i Im X (real parts zeroed); Z = X as well when Z is given.  Y, Z must not alias X.
 __device__ __noinline__ void wg_imag_part(int n, const double2* X, double2* Y, double2* Z) {
   const int nn = n * n;

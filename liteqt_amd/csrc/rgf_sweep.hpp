@@ -218,6 +218,17 @@ __device__ void solve_energy(const SweepArgs& a, int e, double2* ws, ldsd* lds, 
     return sL(l);
   };
 
+  // Without a scattering Sigma^< every left-connected g^<_l = g S g^H is anti-Hermitian
+  // (S: contact terms -(Sigma - Sigma^H) f, and T g^<_{l-1} T^H); the GEMM result is
+  // projected onto the anti-Hermitian matrices, (X - X^H) / 2, which only removes
+  // rounding error, and the backward sweep then uses the identity
+  // g< H^H G_{l,l+1}^H = -(G_{l,l+1} H g<)^H (one GEMM less per slab).
+  const bool ahp = !CMS && !a.sigL;
+  auto lesser_store = [&](int l, const double2* Tm) {
+    wg_gemm<T>(n, OP_N, Tm, OP_C, gl(l), ahp ? T2 : gln(l), 0, st, lds);
+    if (ahp) wg_antiherm(n, T2, gln(l), lds);
+  };
+
   // ---- left contact
   onsite(0, H00);
   const int itL = wg_sancho<T, NMAX>(n, E, a.eta_lead, a.dec_tol, a.dec_nmax, H00, Cb(0), 1, G, stmp,
@@ -234,7 +245,7 @@ __device__ void solve_energy(const SweepArgs& a, int e, double2* ws, ldsd* lds, 
   }
   if (wg_inverse<T, NMAX>(n, W, gl(0), lds)) ++fails;
   wg_gemm<T>(n, OP_N, gl(0), OP_N, sig, T0, 0, st, lds);
-  wg_gemm<T>(n, OP_N, T0, OP_C, gl(0), gln(0), 0, st, lds);
+  lesser_store(0, T0);
 
   // ---- forward sweep
   // one step: g_l, g^<_l from g_{l-1}, g^<_{l-1} (also the segment recompute); a
@@ -255,7 +266,7 @@ __device__ void solve_energy(const SweepArgs& a, int e, double2* ws, ldsd* lds, 
     wg_gemm<T>(n, OP_N, Cb(l), OP_N, gln(l - 1), T0, 0, st, lds);
     wg_gemm<T>(n, OP_N, T0, OP_C, Cb(l), T1, 0, es, lds);
     wg_gemm<T>(n, OP_N, gl(l), OP_N, T1, T0, 0, st, lds);
-    wg_gemm<T>(n, OP_N, T0, OP_C, gl(l), gln(l), 0, st, lds);
+    lesser_store(l, T0);
   };
   for (int l = 1; l < nx - 1; ++l) forward_step(l, true);
 
@@ -348,12 +359,23 @@ __device__ void solve_energy(const SweepArgs& a, int e, double2* ws, ldsd* lds, 
       Epi ea = epi_store();
       ea.D1 = gl(l);
       wg_gemm<T>(n, OP_N, V, OP_N, gl(l), G, 0, ea, lds);     // G_ll
-      ea.D1 = gln(l);
-      wg_gemm<T>(n, OP_N, P7, OP_C, X, T0, 0, ea, lds);       // g< + X G< X^H
-      ea.D1 = T0;
-      wg_gemm<T>(n, OP_N, V, OP_N, gln(l), T1, 0, ea, lds);   // + G_{l,l+1} H g<
-      ea.D1 = T1;
-      wg_gemm<T>(n, OP_N, Wm, OP_C, GN0, Gn, 0, ea, lds);     // + g< H^H G_{l,l+1}^H
+      if (!ahp) {
+        ea.D1 = gln(l);
+        wg_gemm<T>(n, OP_N, P7, OP_C, X, T0, 0, ea, lds);       // g< + X G< X^H
+        ea.D1 = T0;
+        wg_gemm<T>(n, OP_N, V, OP_N, gln(l), T1, 0, ea, lds);   // + G_{l,l+1} H g<
+        ea.D1 = T1;
+        wg_gemm<T>(n, OP_N, Wm, OP_C, GN0, Gn, 0, ea, lds);     // + g< H^H G_{l,l+1}^H
+      } else {
+        // g^<_l anti-Hermitian: g< H^H G_{l,l+1}^H = -(G_{l,l+1} H g<)^H, the last
+        // term is the negated conjugate transpose of the third, read in the epilogue
+        wg_gemm<T>(n, OP_N, V, OP_N, gln(l), T1, 0, st, lds);   // Q = G_{l,l+1} H g<
+        Epi eq = epi_store();
+        eq.D1 = gln(l);
+        eq.D2 = T1;
+        eq.D3 = T1; eq.s3 = mone;
+        wg_gemm<T>(n, OP_N, P7, OP_C, X, Gn, 0, eq, lds);       // g< + X G< X^H + Q - Q^H
+      }
     }
     if (CMS) {
       // G^>_ll = g H^H G^>_{l+1} H g^H  (the left-connected G^> of the reference is zero)
@@ -398,7 +420,9 @@ template <int T, int NMAX>
 struct SweepLds {
   static constexpr int A = GemmLds<T>::DOUBLES;
   static constexpr int B = InvLds<T, NMAX>::DOUBLES;
-  static constexpr int DOUBLES = (A > B ? A : B);
+  static constexpr int C = kAntihermLds;
+  static constexpr int AB = (A > B ? A : B);
+  static constexpr int DOUBLES = (AB > C ? AB : C);
 };
 
 template <int T, int NMAX>

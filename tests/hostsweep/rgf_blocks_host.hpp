@@ -249,6 +249,20 @@ inline void wg_ew(int n, double2* out, int out_rowmajor, double2 diag, const dou
     for (int i = 0; i < n; ++i) out[out_rowmajor ? (size_t)i * n + j : i + (size_t)j * n] = D(v[i + (size_t)j * n]);
 }
 
+constexpr int kAntihermLds = 4 * 32 * 33;
+
+inline void wg_antiherm(int n, const double2* X, double2* out, ldsd*) {
+  using namespace hostcheck;
+  ++ops();
+  const size_t bytes = (size_t)n * n * 16;
+  need("wg_antiherm", "X", X, bytes);
+  need("wg_antiherm", "out", out, bytes);
+  no_alias("wg_antiherm", "output aliases X", out, X, bytes);
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i)
+      out[i + (size_t)j * n] = D(0.5 * (C(X[i + (size_t)j * n]) - std::conj(C(X[j + (size_t)i * n]))));
+}
+
 inline double wg_retrace_prod(int n, const double2* X, const double2* Y, ldsd*) {
   using namespace hostcheck;
   ++ops();

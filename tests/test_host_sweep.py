@@ -96,10 +96,15 @@ def test_overlap_solve_program_in_bounds_and_equal_to_oracle(hostlib, nm, nx):
 
 
 @pytest.mark.parametrize("mode", [0, 1])
-def test_plain_solve_program_in_bounds_and_equal_to_oracle(hostlib, mode):
+@pytest.mark.parametrize("scat", [True, False])
+def test_plain_solve_program_in_bounds_and_equal_to_oracle(hostlib, mode, scat):
+    """Without scattering Sigma the program takes its anti-Hermitian path (g^< projected,
+    the backward G^< update with one product fewer); with it, the reference's sequence."""
     nm, nx = 16, 5
     Hii, H1i = configs.random_device(nm, nx, 5)
     sL, sG, sR = configs.scattering_self_energies(nm, nx, 6, E=0.0, scale=0.02)
+    if not scat:
+        sL = sG = sR = None
     E = np.array([-0.4, 0.05, 0.7])
     temp, mu = (300.0, 250.0), (0.1, -0.1)
     po = orc.params(mode, eta_dev=1e-4, eta_lead=1e-4, dec_tol=1e-12)
@@ -111,11 +116,16 @@ def test_plain_solve_program_in_bounds_and_equal_to_oracle(hostlib, mode):
             nb = nx if k in ("ldos", "ndens") else nx - 1
             for l in range(nb):
                 assert relerr(g[k][..., l, e], r[k][..., l, e]) <= 1e-10, (k, l, e)
+        for l in range(nx):
+            assert np.linalg.norm(g["pdens"][..., l, e] - r["pdens"][..., l, e]) <= \
+                1e-9 * np.linalg.norm(r["pdens"][..., l, e])
         assert abs(g["tr"][e] - r["tr"][e]) <= 1e-9 * max(1.0, abs(r["tr"][e]))
+        assert abs(g["tre"][e] - r["tre"][e]) <= 1e-9 * max(1.0, abs(r["tre"][e]))
 
 
 @pytest.mark.parametrize("nx,K", [(11, 3), (12, 5), (9, 4), (7, 1), (6, 6), (10, 8)])
-def test_checkpointed_program_in_bounds_and_equal_to_keep_all(hostlib, nx, K):
+@pytest.mark.parametrize("scat", [True, False])
+def test_checkpointed_program_in_bounds_and_equal_to_keep_all(hostlib, nx, K, scat):
     """Segment recompute: with the left-connected blocks kept only every K slabs and
     in one K-slab segment (a workspace slot of 2 ((nx-2)/K + 1) + 2K + 30 blocks),
     every block pointer stays inside that smaller slot and every output equals the
@@ -123,6 +133,8 @@ def test_checkpointed_program_in_bounds_and_equal_to_keep_all(hostlib, nx, K):
     nm = 8
     Hii, H1i = configs.random_device(nm, nx, 90 + nx)
     sL, sG, sR = configs.scattering_self_energies(nm, nx, 91 + nx, E=0.0, scale=0.02)
+    if not scat:
+        sL = sR = None
     E = np.array([-0.3, 0.2])
     temp, mu = (300.0, 250.0), (0.1, -0.1)
     for mode in (0, 1):
