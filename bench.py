@@ -280,14 +280,15 @@ def adjoint_cost(rgf, torch, dev, stream, nE):
             "current_e_over_h": float(J.sum().item()), "max_abs_dI_dU": float(dU.sum(0).abs().max().item())}
 
 
-def scba_cost(rgf, torch, dev, nE, iters):
+def scba_cost(rgf, torch, dev, nE, iters, eager=False):
     """SCBA step on the block-tridiagonal-Sigma path: random device nm = 64,
     nx = 100, nE energies in [-3, 3] eV, eta 1e-4, Fermi leads mu = +-0.1 eV,
     kT = 0.0259 eV; diagonal and first off-diagonal Sigma^R / Sigma^< blocks of
     size ~1e-3 (Sigma^R = Lambda - i Gamma/2, Sigma^< = i f Gamma), per energy,
     resident in HBM.  One captured launch (HIP graph) solves every energy and
     refreshes Sigma in place (elastic, Sigma <- (1-mix) Sigma + mix D G); it is
-    replayed `iters` times."""
+    replayed `iters` times (eager: `iters` plain launches of the same solve instead,
+    the form a counter profile attributes per dispatch)."""
     from liteqt_amd import configs
     rgf.lib().rgf_release()
     torch.cuda.empty_cache()
@@ -315,7 +316,7 @@ def scba_cost(rgf, torch, dev, nE, iters):
     D, mix = 1e-3, 0.5
     stream = torch.cuda.Stream()
     t0 = time.perf_counter()
-    g = rgf.Graph(p, Ed, temp, mu, Hd, Cd, stream, out=out, scba=(D, mix), **sig)
+    g = None if eager else rgf.Graph(p, Ed, temp, mu, Hd, Cd, stream, out=out, scba=(D, mix), **sig)
     t_capture = time.perf_counter() - t0
     current = []
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -323,23 +324,29 @@ def scba_cost(rgf, torch, dev, nE, iters):
     with torch.cuda.stream(stream):
         ev0.record(stream)
         for _ in range(iters):
-            g.launch(1)
+            if eager:
+                rgf.solve_batched(p, Ed, temp, mu, Hd, Cd, out=out, stream=stream, scba=(D, mix), **sig)
+            else:
+                g.launch(1)
             current.append((out["tr"].sum() * dE).clone())  # Landauer current per iteration, e/h units
         ev1.record(stream)
     stream.synchronize()
     ms = ev0.elapsed_time(ev1)
     info = out["info"].cpu().numpy()
-    g.close()
+    if g is not None:
+        g.close()
     cur = [float(c.item()) for c in current]
     n3 = 8.0 * nm ** 3
     flops = float(np.sum(n3 * (21 * nx + 7 * (info[:, 0] + info[:, 1]))))
     achieved = flops / (ms * 1e-3 / iters) / 1e12
-    print(f"bench: scba {iters} graph replays (nm={nm} nx={nx} nE={nE}) {ms / 1e3:.2f}s", file=sys.stderr, flush=True)
+    form = "eager launches" if eager else "graph replays"
+    print(f"bench: scba {iters} {form} (nm={nm} nx={nx} nE={nE}) {ms / 1e3:.2f}s", file=sys.stderr, flush=True)
     del sig, out
     torch.cuda.empty_cache()
     res = {"workload": f"random device nm={nm} nx={nx}, {nE} energies in [-3,3] eV, eta 1e-4, Fermi leads; "
                        "diagonal + first off-diagonal Sigma^R/Sigma^< (~1e-3) per energy in HBM; elastic SCBA "
-                       f"refresh D={D} mix={mix} in place; one HIP graph replayed per iteration",
+                       f"refresh D={D} mix={mix} in place; "
+                       + ("one eager launch per iteration" if eager else "one HIP graph replayed per iteration"),
            "iterations": iters, "s_per_iteration": ms / 1e3 / iters, "capture_s": t_capture,
            "energy_points_per_s": nE * iters / (ms / 1e3), "achieved_tflops": achieved,
            "frac": achieved / PEAK_FP64_MFMA, "flops_per_iteration": flops,
@@ -759,6 +766,8 @@ def main():
     ap.add_argument("--no-scba", action="store_true", help="skip the SCBA-step (off-diagonal Sigma) measurement")
     ap.add_argument("--scba-energies", type=int, default=4096)
     ap.add_argument("--scba-iterations", type=int, default=10)
+    ap.add_argument("--scba-eager", action="store_true",
+                    help="SCBA leg as plain launches instead of graph replays (for counter profiles)")
     ap.add_argument("--no-small", action="store_true", help="skip the small-block (CNT nm=32) measurement")
     ap.add_argument("--no-devices", action="store_true", help="skip the 64-device one-launch measurement")
     ap.add_argument("--devices", type=int, default=64)
@@ -881,7 +890,7 @@ def main():
         adjoint = adjoint_cost(rgf, torch, dev, stream, args.adjoint_energies)
     scba = None
     if not args.no_scba:
-        scba = scba_cost(rgf, torch, dev, args.scba_energies, args.scba_iterations)
+        scba = scba_cost(rgf, torch, dev, args.scba_energies, args.scba_iterations, args.scba_eager)
     small = None
     if not args.no_small and args.config != "cnt32":
         small = small_block_cost(rgf, torch, dev)
