@@ -223,6 +223,55 @@ def cpu_baseline(cfg, params_kw, temp, mu, threads, rgf=None, p_gpu=None):
     return res
 
 
+def dropin_cost(rgf, nE, nx, threads, nm=256):
+    """The literal drop-in as the reference's OpenMP drivers use it: the Fortran program
+    tests/fortran/rgf_omp_driver (built by __graft_entry__.build) calls green_RGF_RS_gpu
+    once per energy from `threads` OpenMP threads (src/green_rgf.f90:1301-1318), reference
+    parameters, ballistic, outputs ndens/pdens/ldos/cur/tr/tre in host arrays; the same
+    energies then go through the batched host entry point rgf_rs_energies with the same
+    parameters and outputs.  Both include the host<->device copies of their arrays.
+    Workload: the random device nm = 256 at nx slabs, nE energies spread over [-2, 2] eV."""
+    import subprocess
+    import tempfile
+    from liteqt_amd import configs
+    drv = os.path.join(ROOT, "tests", "fortran", "rgf_omp_driver")
+    if not os.path.exists(drv):
+        raise SystemExit("bench.py --dropin: tests/fortran/rgf_omp_driver not built (run __graft_entry__.build())")
+    Hii, H1i = configs.random_device(nm, nx, 2)
+    E = configs.energy_grid(-2.0, 2.0, 2048)[np.linspace(0, 2047, nE).astype(int)]
+    temp, mu = np.array([300.0, 300.0]), np.array([0.1, -0.1])
+    with tempfile.TemporaryDirectory() as d:
+        with open(os.path.join(d, "in.bin"), "wb") as f:
+            np.array([nm, nx, nE], np.int32).tofile(f)
+            for a in (temp, mu, E):
+                np.asarray(a, np.float64).tofile(f)
+            for a in (Hii, H1i):
+                np.asfortranarray(a, np.complex128).ravel(order="F").tofile(f)
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+        r = subprocess.run([drv], cwd=d, capture_output=True, text=True, timeout=900, env=env)
+        if r.returncode != 0:
+            raise SystemExit(f"bench.py --dropin: driver failed: {r.stdout[-400:]} {r.stderr[-400:]}")
+        t_drv = float(r.stdout.split("seconds")[-1])
+        tr_drv = np.fromfile(os.path.join(d, "out.bin"), dtype=np.float64, count=nE)
+    p = rgf.params(rgf.MODE_COMPAT)
+    outs = ("ndens", "pdens", "ldos", "cur")
+    rgf.rgf_rs_energies(p, tuple(temp), E[:1], tuple(mu), Hii, H1i, outputs=outs)  # library and workspace
+    t0 = time.perf_counter()
+    g = rgf.rgf_rs_energies(p, tuple(temp), E, tuple(mu), Hii, H1i, outputs=outs)
+    t_bat = time.perf_counter() - t0
+    print(f"bench: drop-in {threads} OpenMP threads {t_drv:.2f}s, batched host entry {t_bat:.2f}s "
+          f"(nm={nm} nx={nx} nE={nE})", file=sys.stderr, flush=True)
+    return {"workload": f"random device nm={nm} nx={nx}, {nE} energies in [-2,2] eV, reference parameters "
+                        "(eta_dev 0, eta_lead 1e-3, 100 decimation steps), ballistic; ndens, pdens, ldos, cur, "
+                        "tr, tre in host arrays",
+            "omp_threads": threads, "s_omp_driver": t_drv, "s_rs_energies": t_bat,
+            "driver_over_batched": t_drv / t_bat,
+            # the driver passes zero-filled scattering arrays, which take the reference's
+            # product sequence; without them the batched entry takes the anti-Hermitian one
+            "tr_max_relative_difference": float(np.max(np.abs(tr_drv - np.asarray(g["tr"]).reshape(-1))
+                                                       / np.maximum(1.0, np.abs(tr_drv))))}
+
+
 def adjoint_cost(rgf, torch, dev, stream, nE):
     """Adjoint vs forward cost on the differentiation workload: random device
     nm = 128, nx = 300, nE energies in [-2, 2] eV, eta 1e-5, Fermi leads
@@ -781,6 +830,10 @@ def main():
                     help="divide each device's energy count (U[256,2048]) by this")
     ap.add_argument("--plain-energies", type=int, default=4,
                     help="energies timed on the plain per-energy GPU path (0: skip)")
+    ap.add_argument("--dropin", action="store_true",
+                    help="measure the per-energy drop-in (Fortran OpenMP driver) against the batched host entry")
+    ap.add_argument("--dropin-energies", type=int, default=64)
+    ap.add_argument("--dropin-nx", type=int, default=20)
     ap.add_argument("--gw-loop", action="store_true",
                     help="one iteration of the 1D GW loop at the example's size, per stage (instead of the RGF bench)")
     ap.add_argument("--gw", action="store_true",
@@ -804,6 +857,11 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
+    if args.dropin:
+        from liteqt_amd import rgf
+        threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count())
+        print(json.dumps({"dropin": dropin_cost(rgf, args.dropin_energies, args.dropin_nx, threads)}))
+        return
     if args.gw_loop:
         gw_loop_bench(args)
         return
