@@ -259,13 +259,19 @@ def dropin_cost(rgf, nE, nx, threads, nm=256):
     t0 = time.perf_counter()
     g = rgf.rgf_rs_energies(p, tuple(temp), E, tuple(mu), Hii, H1i, outputs=outs)
     t_bat = time.perf_counter() - t0
+    # the same call with the driver's zero-filled scattering arrays (its exact inputs)
+    z = np.zeros_like(Hii)
+    t0 = time.perf_counter()
+    rgf.rgf_rs_energies(p, tuple(temp), E, tuple(mu), Hii, H1i, z, z, z, outputs=outs)
+    t_bat_sig = time.perf_counter() - t0
     print(f"bench: drop-in {threads} OpenMP threads {t_drv:.2f}s, batched host entry {t_bat:.2f}s "
           f"(nm={nm} nx={nx} nE={nE})", file=sys.stderr, flush=True)
     return {"workload": f"random device nm={nm} nx={nx}, {nE} energies in [-2,2] eV, reference parameters "
                         "(eta_dev 0, eta_lead 1e-3, 100 decimation steps), ballistic; ndens, pdens, ldos, cur, "
                         "tr, tre in host arrays",
             "omp_threads": threads, "s_omp_driver": t_drv, "s_rs_energies": t_bat,
-            "driver_over_batched": t_drv / t_bat,
+            "s_rs_energies_zero_sigma_arrays": t_bat_sig,
+            "driver_over_batched": t_drv / t_bat, "driver_over_batched_same_inputs": t_drv / t_bat_sig,
             # the driver passes zero-filled scattering arrays, which take the reference's
             # product sequence; without them the batched entry takes the anti-Hermitian one
             "tr_max_relative_difference": float(np.max(np.abs(tr_drv - np.asarray(g["tr"]).reshape(-1))
